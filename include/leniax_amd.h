@@ -1,0 +1,153 @@
+/* leniax_amd -- C ABI of the MI355X (gfx950) Lenia FFT stepping engine.
+ *
+ * Every pointer argument named *_d is a device pointer (HBM); the caller owns
+ * all buffers.  Calls are asynchronous on the HIP stream passed in ("stream"
+ * may be NULL for the default stream).  Return value: LX_OK (0) or a negative
+ * errno-style code; lx_last_error() gives a thread-local message.  Array
+ * layouts are leniax's channel-first ones: state [B][C][dims...] float32.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the leniax repository).
+ */
+#ifndef LENIAX_AMD_H
+#define LENIAX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LX_OK 0
+#define LX_EINVAL (-22)
+#define LX_ENOTSUP (-95)
+#define LX_EDEVICE (-5)
+#define LX_ENOMEM (-12)
+
+#define LX_MAX_K 16 /* kernels (potential channels) per rule */
+#define LX_MAX_C 4  /* channels per world */
+#define LX_NSTAT_BASE 11 /* statistics per step before the C channel masses */
+
+/* growth functions, registry order of leniax/growth_functions.py:256-264 */
+enum lx_growth { LX_POLY_QUAD4 = 0, LX_GAUSSIAN = 1, LX_GAUSSIAN_TARGET = 2, LX_STEP = 3,
+                 LX_STAIRCASE = 4, LX_TRIANGLE = 5, LX_IDENTITY = 6 };
+/* state functions, leniax/core.py:322-326 */
+enum lx_state_fn { LX_STATE_V1 = 0, LX_STATE_V2 = 1, LX_STATE_SIMPLE = 2 };
+/* stop policies of lx_run */
+enum lx_stop { LX_STOP_NONE = 0,   /* run every step (leniax run_scan*), N from the heuristics   */
+               LX_STOP_LENIAX = 1, /* retire a world once leniax's heuristics fail (and t >= min_steps) */
+               LX_STOP_MASS = 2 }; /* retire when the mass is 0 or covers > 90% of the grid */
+
+/* The static part of leniax's update closure (leniax/helpers.py:401-515):
+ * world shape, channel mapping, growth function per kernel, weighting, state fn. */
+typedef struct lx_rule {
+  int32_t ndim;             /* 2 or 3 world dimensions */
+  int32_t dims[3];          /* world size; for ndim == 2 only dims[0..1] are used */
+  int32_t C;                /* channels */
+  int32_t K;                /* kernels == potential channels after leniax's tc_indices filter */
+  int32_t cin[LX_MAX_K];    /* input channel of kernel k (kernels sorted by c_in, leniax/kernels.py:90) */
+  int32_t gf[LX_MAX_K];     /* enum lx_growth of kernel k */
+  int32_t state_fn;         /* enum lx_state_fn */
+  int32_t average;          /* 1: weighted_mean (leniax/core.py:228-242), 0: weighted_sum (:202-225) */
+} lx_rule;
+
+/* Library / device information. */
+const char* lx_last_error(void);
+int lx_version(void);
+
+/* Hermitian half spectra from leniax's full spectra.
+ * Replaces the spectrum convention of leniax/kernels.py:145-149 (K = fftn(fftshift(k)))
+ * as consumed by leniax/core.py:80-87.
+ * K_full_d: complex64 [S][C][maxk][dims...]; slot[k] = c*maxk + j is the full-spectrum slot of
+ * kernel k (leniax's tc_indices, leniax/core.py:97-98).
+ * K_half_d: complex64 [S][K][dims[0..nd-2]][dims[nd-1]/2 + 1], holding (K(f) + conj(K(-f)))/2,
+ * so that the real-to-complex path reproduces real(ifftn(fftn(A) * K)) for any K. */
+int lx_prepare_spectra(const lx_rule* rule, const void* K_full_d, int64_t S, int32_t maxk,
+                       const int32_t* slot, void* K_half_d, void* stream);
+
+/* leniax/kernels.py:145-149: K = fftn(fftshift(k)) of real kernels on the device.
+ * k_d float32 [n][dims] holds kernels already zero-padded to the world and fftshift-ed
+ * (centre at index 0); K_full_d complex64 [n][dims]; work_d: n * prod(dims[:-1]) * (dims[-1]/2+1)
+ * complex64 elements. */
+int lx_kernel_spectrum(int32_t ndim, const int32_t* dims, int64_t n, const float* k_d, void* K_full_d,
+                       void* work_d, void* stream);
+
+/* Workspace bytes lx_potential / lx_update / lx_stats need for a batch of B worlds. */
+int lx_workspace_size(const lx_rule* rule, int64_t B, size_t* bytes);
+
+/* leniax/core.py:52-102 get_potential_fft: potential_d [B][K][dims] = real(ifftn(fftn(A) * K)).
+ * K_half_d is indexed by world: spectrum of world b starts at b * k_bstride complex elements
+ * (k_bstride == 0: one shared spectrum). */
+int lx_potential(const lx_rule* rule, const float* state_d, int64_t B, const void* K_half_d,
+                 int64_t k_bstride, float* potential_d, void* work_d, void* stream);
+
+/* leniax/core.py:163-199 get_field (+ weighted_sum/weighted_mean):
+ * field_d [B][C][dims] = sum_k w[c][k] g_k(potential_k) (/ sum_k w[c][k]).
+ * gf_d [K][2] (m, s) and w_d [C][K] per world with strides gf_bstride / w_bstride floats (0: shared). */
+int lx_field(const lx_rule* rule, const float* potential_d, int64_t B, const float* gf_d, int64_t gf_bstride,
+             const float* w_d, int64_t w_bstride, float* field_d, void* stream);
+
+/* leniax/core.py:245-319 get_state (v1 clip / v2 / simple); dt_d per world (dt_bstride 0: shared). */
+int lx_state(const lx_rule* rule, const float* state_d, const float* field_d, int64_t B, const float* dt_d,
+             int64_t dt_bstride, float* out_d, void* stream);
+
+/* leniax/core.py:13-49 update with the FFT potential: the plain multi-pass step
+ * (3 kernels, 2 half-spectrum passes through HBM).  field_d / potential_d may be NULL. */
+int lx_update(const lx_rule* rule, const float* state_d, int64_t B, const void* K_half_d, int64_t k_bstride,
+              const float* gf_d, int64_t gf_bstride, const float* w_d, int64_t w_bstride, const float* dt_d,
+              int64_t dt_bstride, float* state_out_d, float* field_d, float* potential_d, void* work_d,
+              void* stream);
+
+/* leniax/statistics.py:35-126 compute_stats (one step, B worlds) + the per-world carry
+ * (shift, centroid, angle: leniax/runner.py:309-323) kept in carry_d (lx_carry_bytes() each).
+ * stats_d [B][11 + C] floats (order: mass, mass_volume, mass_density, growth, growth_volume,
+ * growth_density, mass_speed, mass_angle_speed, mass_growth_dist, inertia, potential_volume,
+ * channel_mass[C]).  R and T are the config's values (leniax/statistics.py:22-24). */
+size_t lx_carry_bytes(void);
+int lx_carry_init(int64_t B, void* carry_d, void* stream);
+int lx_stats(const lx_rule* rule, const float* cells_d, const float* field_d, const float* potential_d,
+             int64_t B, float R, float T, void* carry_d, float* stats_d, void* stream);
+
+/* leniax/statistics.py:134-205 check_heuristics over stats_d [T][B][11 + C] (lx_stats layout):
+ * cont_d [T][B] (1.0 / 0.0) and n_d [B] = sum_t cont. */
+int lx_check_heuristics(const float* stats_d, int32_t T, int64_t B, int32_t C, float* cont_d, float* n_d,
+                        void* stream);
+
+/* Fused persistent batched run (leniax/runner.py:167-215 run_scan_mem_optimized semantics):
+ * one launch steps every world up to max_steps, world after world from a device queue,
+ * each world resident in registers + LDS, statistics and stop policy evaluated every step.
+ * Supported shapes: lx_run_supported(). */
+typedef struct lx_run_args {
+  const float* cells0_d;   /* [B][C][dims] initial states */
+  float* cells_d;          /* [B][C][dims] final states (may equal cells0_d) */
+  const void* K_half_d;    /* lx_prepare_spectra layout */
+  int64_t k_bstride;       /* complex elements between worlds' spectra, 0: shared */
+  const float* gf_d;       /* [K][2] per world, stride gf_bstride floats */
+  int64_t gf_bstride;
+  const float* w_d;        /* [C][K] per world, stride w_bstride floats */
+  int64_t w_bstride;
+  const float* dt_d;       /* update dt = 1/T per world, stride dt_bstride */
+  int64_t dt_bstride;
+  int64_t B;               /* worlds */
+  int32_t max_steps;
+  int32_t stop_policy;     /* enum lx_stop */
+  int32_t min_steps;       /* LX_STOP_LENIAX: never retire before this many steps */
+  float R;                 /* statistics R and T (config values, leniax/statistics.py:22-24) */
+  float T;
+  float* stats_d;          /* [B][max_steps][11 + C] or NULL */
+  int32_t* n_d;            /* [B] leniax N (leading steps whose heuristics held) or NULL */
+  int32_t* steps_d;        /* [B] steps executed or NULL */
+  float* cells_hist_d;     /* [max_steps][B][C][dims] state before each step, or NULL */
+  float* field_hist_d;     /* [max_steps][B][C][dims] or NULL */
+  float* potential_hist_d; /* [max_steps][B][K][dims] or NULL */
+  int32_t* queue_d;        /* one int32 of device scratch (the work queue head) */
+} lx_run_args;
+
+int lx_run_supported(const lx_rule* rule);
+int lx_run(const lx_rule* rule, const lx_run_args* args, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

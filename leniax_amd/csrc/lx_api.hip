@@ -1,0 +1,114 @@
+// C ABI entry points of libleniax_amd: argument checks, spectrum preparation,
+// dispatch of the fused run.  The plain step lives in lx_plain.hip.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "lx_internal.h"
+
+static thread_local char g_err[512] = "";
+
+int lx_fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+extern "C" const char* lx_last_error(void) { return g_err; }
+extern "C" int lx_version(void) { return 1; }
+
+static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int lx_check_rule(const lx_rule* rule) {
+  if (!rule) return lx_fail(LX_EINVAL, "rule is NULL");
+  if (rule->ndim != 2 && rule->ndim != 3) return lx_fail(LX_EINVAL, "ndim must be 2 or 3 (got %d)", rule->ndim);
+  for (int i = 0; i < rule->ndim; ++i)
+    if (rule->dims[i] < 1) return lx_fail(LX_EINVAL, "dims[%d] = %d", i, rule->dims[i]);
+  if (rule->C < 1 || rule->C > LX_MAX_C) return lx_fail(LX_EINVAL, "C must be in [1, %d] (got %d)", LX_MAX_C, rule->C);
+  if (rule->K < 1 || rule->K > LX_MAX_K) return lx_fail(LX_EINVAL, "K must be in [1, %d] (got %d)", LX_MAX_K, rule->K);
+  for (int k = 0; k < rule->K; ++k) {
+    if (rule->cin[k] < 0 || rule->cin[k] >= rule->C)
+      return lx_fail(LX_EINVAL, "cin[%d] = %d out of range", k, rule->cin[k]);
+    if (rule->gf[k] < 0 || rule->gf[k] > LX_IDENTITY) return lx_fail(LX_EINVAL, "gf[%d] = %d unknown", k, rule->gf[k]);
+  }
+  if (rule->state_fn < 0 || rule->state_fn > LX_STATE_SIMPLE)
+    return lx_fail(LX_EINVAL, "state_fn %d unknown", rule->state_fn);
+  return LX_OK;
+}
+
+// ------------------------------------------------------------------ spectra
+struct SlotList {
+  int32_t s[LX_MAX_K];
+};
+
+__global__ void prepare_spectra_kernel(const float2* __restrict__ kf, float2* __restrict__ kh, int64_t S, int K,
+                                       int maxk_C, int d0, int d1, int d2, SlotList slots) {
+  // d0, d1 leading dims (d0 == 1 for 2-D), d2 the last dim; output last dim d2/2+1
+  const int wh = d2 / 2 + 1;
+  const int64_t per_k = int64_t(d0) * d1 * wh;
+  const int64_t total = S * K * per_k;
+  const int64_t full_per_slot = int64_t(d0) * d1 * d2;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t rem = i;
+    const int c = int(rem % wh); rem /= wh;
+    const int h = int(rem % d1); rem /= d1;
+    const int d = int(rem % d0); rem /= d0;
+    const int k = int(rem % K);
+    const int64_t s = rem / K;
+    const float2* base = kf + (s * maxk_C + slots.s[k]) * full_per_slot;
+    const float2 v = base[(int64_t(d) * d1 + h) * d2 + c];
+    const int dn = (d0 - d) % d0, hn = (d1 - h) % d1, cn = (d2 - c) % d2;
+    const float2 u = base[(int64_t(dn) * d1 + hn) * d2 + cn];
+    kh[i] = make_float2(0.5f * (v.x + u.x), 0.5f * (v.y - u.y));
+  }
+}
+
+extern "C" int lx_prepare_spectra(const lx_rule* rule, const void* K_full_d, int64_t S, int32_t maxk,
+                                  const int32_t* slot, void* K_half_d, void* stream) {
+  int rc = lx_check_rule(rule);
+  if (rc) return rc;
+  if (!K_full_d || !K_half_d || !slot) return lx_fail(LX_EINVAL, "NULL pointer");
+  if (S < 1 || maxk < 1) return lx_fail(LX_EINVAL, "S and maxk must be positive");
+  SlotList sl;
+  for (int k = 0; k < rule->K; ++k) {
+    if (slot[k] < 0 || slot[k] >= rule->C * maxk) return lx_fail(LX_EINVAL, "slot[%d] = %d out of range", k, slot[k]);
+    sl.s[k] = slot[k];
+  }
+  const int d0 = rule->ndim == 3 ? rule->dims[0] : 1;
+  const int d1 = rule->ndim == 3 ? rule->dims[1] : rule->dims[0];
+  const int d2 = rule->ndim == 3 ? rule->dims[2] : rule->dims[1];
+  const int64_t total = S * rule->K * int64_t(d0) * d1 * (d2 / 2 + 1);
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(prepare_spectra_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     static_cast<const float2*>(K_full_d), static_cast<float2*>(K_half_d), S, rule->K,
+                     rule->C * maxk, d0, d1, d2, sl);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return lx_fail(LX_EDEVICE, "prepare_spectra launch: %s", hipGetErrorString(e));
+  return LX_OK;
+}
+
+// ------------------------------------------------------------------ fused run
+extern "C" int lx_run_supported(const lx_rule* rule) {
+  if (lx_check_rule(rule)) return 0;
+  return lx_fused2d_supported(rule);
+}
+
+extern "C" int lx_run(const lx_rule* rule, const lx_run_args* a, void* stream) {
+  int rc = lx_check_rule(rule);
+  if (rc) return rc;
+  if (!a) return lx_fail(LX_EINVAL, "args is NULL");
+  if (!a->cells0_d || !a->cells_d || !a->K_half_d || !a->gf_d || !a->w_d || !a->dt_d || !a->queue_d)
+    return lx_fail(LX_EINVAL, "NULL device pointer in lx_run_args");
+  if (a->B < 0 || a->max_steps < 0) return lx_fail(LX_EINVAL, "B and max_steps must be >= 0");
+  if (a->stop_policy < LX_STOP_NONE || a->stop_policy > LX_STOP_MASS)
+    return lx_fail(LX_EINVAL, "unknown stop policy %d", a->stop_policy);
+  if (!(a->R > 0.f) || !(a->T > 0.f)) return lx_fail(LX_EINVAL, "R and T must be positive");
+  if (a->B == 0) return LX_OK;
+  if (!lx_fused2d_supported(rule))
+    return lx_fail(LX_ENOTSUP, "lx_run: no fused kernel for ndim=%d dims=%dx%d C=%d K=%d (use lx_update)",
+                   rule->ndim, rule->dims[0], rule->dims[1], rule->C, rule->K);
+  return lx_fused2d_launch(rule, a, (hipStream_t)stream);
+}

@@ -1,0 +1,378 @@
+// Device-side building blocks for the Lenia FFT step on CDNA4 (gfx950).
+//
+//  * complex helpers on float2
+//  * compile-time twiddle constants (constexpr double series, rounded to float)
+//  * fft_reg<P>:       in-register radix-2 FFT of P points, natural order in/out
+//  * fft_dist<P,INV>:  a length-4P FFT spread over the 4 lanes of a DPP quad:
+//                      in-register P-point FFTs + one radix-4 step whose data
+//                      exchange uses DPP quad permutes (no LDS traffic)
+//  * growth functions (leniax/growth_functions.py) by id
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define LX_DEV __device__ __forceinline__
+
+// ----------------------------------------------------------------- complex
+LX_DEV float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+LX_DEV float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+LX_DEV float2 cmul(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+LX_DEV float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -a.x * b.y));
+}
+LX_DEV float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+LX_DEV float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// ----------------------------------------------------------------- constexpr twiddles
+// sin/cos of 2*pi*k/n from the Taylor series on [0, pi/4] after exact octant
+// reduction in integers; double precision, then rounded to float.
+namespace lxc {
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double sin_series(double x) {
+  double x2 = x * x, term = x, sum = x;
+  for (int n = 1; n < 14; ++n) { term *= -x2 / double((2 * n) * (2 * n + 1)); sum += term; }
+  return sum;
+}
+constexpr double cos_series(double x) {
+  double x2 = x * x, term = 1.0, sum = 1.0;
+  for (int n = 1; n < 14; ++n) { term *= -x2 / double((2 * n - 1) * (2 * n)); sum += term; }
+  return sum;
+}
+// cos(2*pi*k/n), sin(2*pi*k/n) with k reduced into the first octant.
+constexpr double cospi2(long k, long n) {
+  k %= n; if (k < 0) k += n;
+  // angle = 2*pi*k/n; work with 8k vs n to find the octant
+  long k8 = 8 * k;
+  long oct = k8 / n;           // 0..7
+  // reduce by symmetries: cos(a) for a in octant
+  switch (oct) {
+    case 0: return cos_series(2 * kPi * double(k) / double(n));
+    case 1: return sin_series(2 * kPi * double(n - 4 * k) / double(4 * n));           // cos(a)=sin(pi/2-a)
+    case 2: return -sin_series(2 * kPi * double(4 * k - n) / double(4 * n));          // a-pi/2
+    case 3: return -cos_series(2 * kPi * double(n - 2 * k) / double(2 * n));          // pi-a
+    case 4: return -cos_series(2 * kPi * double(2 * k - n) / double(2 * n));          // a-pi
+    case 5: return -sin_series(2 * kPi * double(3 * n - 4 * k) / double(4 * n));      // 3pi/2-a
+    case 6: return sin_series(2 * kPi * double(4 * k - 3 * n) / double(4 * n));       // a-3pi/2
+    default: return cos_series(2 * kPi * double(n - k) / double(n));                  // 2pi-a
+  }
+}
+}  // namespace lxc
+
+// Forward twiddle W_n^k = exp(-2*pi*i*k/n); INV flips the sign of the angle.
+template <int N, int K, bool INV>
+struct Tw {
+  static constexpr float re = float(lxc::cospi2(K, N));
+  // sin(2*pi*k/n) = cos(2*pi*(4k - n)/(4n))
+  static constexpr float s = float(lxc::cospi2(long(K) * 4 - N, 4L * N));
+  static constexpr float im = INV ? s : -s;
+};
+
+// ----------------------------------------------------------------- in-register FFT
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
+constexpr int bitrev_c(int i, int bits) {
+  int r = 0;
+  for (int b = 0; b < bits; ++b) r |= ((i >> b) & 1) << (bits - 1 - b);
+  return r;
+}
+
+template <int LEN, int K, bool INV>
+LX_DEV void bfly(float2& a, float2& b) {
+  // a' = a + w b, b' = a - w b with w = W_LEN^K (compile-time constant)
+  float2 wb;
+  if constexpr (K == 0) {
+    wb = b;
+  } else if constexpr (4 * K == LEN) {  // w = -i (fwd) / +i (inv)
+    wb = INV ? make_float2(-b.y, b.x) : make_float2(b.y, -b.x);
+  } else {
+    constexpr float wr = Tw<LEN, K, INV>::re;
+    constexpr float wi = Tw<LEN, K, INV>::im;
+    wb = make_float2(fmaf(b.x, wr, -b.y * wi), fmaf(b.x, wi, b.y * wr));
+  }
+  float2 t = a;
+  a = cadd(t, wb);
+  b = csub(t, wb);
+}
+
+template <int P, bool INV, int LEN, int I, int K>
+LX_DEV void fft_stage_body(float2* x) {
+  if constexpr (I < P) {
+    if constexpr (K < LEN / 2) {
+      bfly<LEN, K, INV>(x[I + K], x[I + K + LEN / 2]);
+      fft_stage_body<P, INV, LEN, I, K + 1>(x);
+    } else {
+      fft_stage_body<P, INV, LEN, I + LEN, 0>(x);
+    }
+  }
+}
+
+template <int P, bool INV, int LEN>
+LX_DEV void fft_stages(float2* x) {
+  if constexpr (LEN <= P) {
+    fft_stage_body<P, INV, LEN, 0, 0>(x);
+    fft_stages<P, INV, LEN * 2>(x);
+  }
+}
+
+template <int P, int I>
+LX_DEV void bitrev_perm(float2* x) {
+  if constexpr (I < P) {
+    constexpr int J = bitrev_c(I, ilog2c(P));
+    if constexpr (J > I) { float2 t = x[I]; x[I] = x[J]; x[J] = t; }
+    bitrev_perm<P, I + 1>(x);
+  }
+}
+
+// Natural-order in, natural-order out, unnormalised; INV uses exp(+2*pi*i*jk/P).
+template <int P, bool INV>
+LX_DEV void fft_reg(float2* x) {
+  bitrev_perm<P, 0>(x);
+  fft_stages<P, INV, 2>(x);
+}
+
+// ----------------------------------------------------------------- DPP quad exchange
+// quad_perm(a,b,c,d): lane i of each quad reads lane perm[i] of the same quad.
+#define LX_QP(a, b, c, d) ((a) | ((b) << 2) | ((c) << 4) | ((d) << 6))
+
+template <int CTRL>
+LX_DEV float dpp(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+LX_DEV float2 dpp2(float2 v) { return make_float2(dpp<CTRL>(v.x), dpp<CTRL>(v.y)); }
+
+// Length-4P FFT over the 4 lanes of a quad (q = lane & 3), unnormalised.
+//  forward: in  x[m]  = data[4m + q]                 (m = 0..P-1)
+//           out x[k1] = DATA[k1 + P*bitrev2(q)]     (k1 = 0..P-1)
+//  inverse: in  x[k1] = DATA[k1 + P*bitrev2(q)]
+//           out x[m]  = data[4m + q]
+// ``tw`` points at exp(-2*pi*i*j/(4P)), j = 0..4P-1 (LDS table).
+template <int P, bool INV>
+LX_DEV void fft_dist(float2* x, int q, const float2* tw) {
+  constexpr int N = 4 * P;
+  const float sgn_hi = (q & 2) ? -1.f : 1.f;  // (-1)^(q>>1)
+  const float sgn_lo = (q & 1) ? -1.f : 1.f;  // (-1)^(q&1)
+  if constexpr (!INV) {
+    fft_reg<P, false>(x);
+#pragma unroll
+    for (int k = 1; k < P; ++k) x[k] = cmul(x[k], tw[(q * k) & (N - 1)]);
+    // stage A: v_{t0}[s0] = y_{t0} + (-1)^{s0} y_{t0+2}, held at lane t0 + 2 s0
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      float2 a = dpp2<LX_QP(0, 1, 0, 1)>(x[k]);
+      float2 b = dpp2<LX_QP(2, 3, 2, 3)>(x[k]);
+      x[k] = make_float2(fmaf(sgn_hi, b.x, a.x), fmaf(sgn_hi, b.y, a.y));
+    }
+    // stage B: X_{s0+2s1} = v_0[s0] + W4^{s0} (-1)^{s1} v_1[s0], held at lane 2 s0 + s1
+    // coefficient c: q=0 -> 1, q=1 -> -1, q=2 -> -i, q=3 -> +i
+    const float cr = (q & 2) ? 0.f : sgn_lo;
+    const float ci = (q & 2) ? -sgn_lo : 0.f;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      float2 a = dpp2<LX_QP(0, 0, 2, 2)>(x[k]);
+      float2 b = dpp2<LX_QP(1, 1, 3, 3)>(x[k]);
+      x[k] = make_float2(fmaf(-ci, b.y, fmaf(cr, b.x, a.x)), fmaf(ci, b.x, fmaf(cr, b.y, a.y)));
+    }
+  } else {
+    // inner: u_{s0}[t0] = Q_{s0} + (-1)^{t0} Q_{s0+2}; Q_s sits at lane bitrev2(s)
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      float2 a = dpp2<LX_QP(0, 0, 2, 2)>(x[k]);
+      float2 b = dpp2<LX_QP(1, 1, 3, 3)>(x[k]);
+      x[k] = make_float2(fmaf(sgn_lo, b.x, a.x), fmaf(sgn_lo, b.y, a.y));
+    }
+    // outer: z_{t0+2t1} = u_0[t0] + W4^{-t0} (-1)^{t1} u_1[t0], output at lane t
+    // coefficient c: q=0 -> 1, q=1 -> +i, q=2 -> -1, q=3 -> -i
+    const float cr = (q & 1) ? 0.f : sgn_hi;
+    const float ci = (q & 1) ? sgn_hi : 0.f;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      float2 a = dpp2<LX_QP(0, 1, 0, 1)>(x[k]);
+      float2 b = dpp2<LX_QP(2, 3, 2, 3)>(x[k]);
+      x[k] = make_float2(fmaf(-ci, b.y, fmaf(cr, b.x, a.x)), fmaf(ci, b.x, fmaf(cr, b.y, a.y)));
+    }
+#pragma unroll
+    for (int k = 1; k < P; ++k) x[k] = cmulc(x[k], tw[(q * k) & (N - 1)]);
+    fft_reg<P, true>(x);
+  }
+}
+
+// ----------------------------------------------------------------- generic lane-group FFT
+// Partner value of lane t ^ H inside groups of 2H lanes (H = 1, 2: DPP quad permutes;
+// H = 4: row shifts by 4; H = 8: row rotate by 8).
+template <int H>
+LX_DEV float xor_fetch(float v, int lane) {
+  if constexpr (H == 1) return dpp<LX_QP(1, 0, 3, 2)>(v);
+  else if constexpr (H == 2) return dpp<LX_QP(2, 3, 0, 1)>(v);
+  else if constexpr (H == 4) {
+    const float up = dpp<0x104>(v);   // row_shl:4 -> lane i reads i+4
+    const float dn = dpp<0x114>(v);   // row_shr:4 -> lane i reads i-4
+    return (lane & 4) ? dn : up;
+  } else {
+    static_assert(H == 8, "lane groups up to 16");
+    return dpp<0x128>(v);             // row_ror:8 -> lane i reads (i+8) mod 16
+  }
+}
+template <int H>
+LX_DEV float2 xor_fetch2(float2 v, int lane) { return make_float2(xor_fetch<H>(v.x, lane), xor_fetch<H>(v.y, lane)); }
+
+constexpr int bitrev_n(int v, int n) {  // reverse the low log2(n) bits
+  int r = 0;
+  for (int b = 1; b < n; b <<= 1) { r <<= 1; r |= (v & 1); v >>= 1; }
+  return r;
+}
+
+// Cross-lane DFT of size TPR over lanes t = lane % TPR, as a radix-2 network.
+// forward (decimation in frequency): natural lane order in, X_s at lane bitrev(s) out.
+// inverse (decimation in time):      X_s at lane bitrev(s) in, natural order out, x TPR.
+template <int P, int TPR, bool INV, int H>
+LX_DEV void lane_network(float2* x, int t, int lane) {
+  if constexpr (H >= 1 && H < TPR) {
+    // twiddle W_{2H}^{t mod H} on the upper lane of each pair
+    const bool upper = (t & H) != 0;
+    const float sgn = upper ? -1.f : 1.f;
+    double sn, cs;
+    sincospi(double(t % H) / double(H), &sn, &cs);  // angle 2*pi*(t mod H)/(2H)
+    float2 w = upper ? make_float2((float)cs, INV ? (float)sn : (float)-sn) : make_float2(1.f, 0.f);
+    if constexpr (!INV) {
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const float2 p = xor_fetch2<H>(x[k], lane);
+        const float2 v = make_float2(fmaf(sgn, x[k].x, p.x), fmaf(sgn, x[k].y, p.y));
+        x[k] = upper ? cmul(v, w) : v;
+      }
+      lane_network<P, TPR, INV, H / 2>(x, t, lane);
+    } else {
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const float2 o = upper ? cmul(x[k], w) : x[k];
+        const float2 p = xor_fetch2<H>(o, lane);
+        x[k] = make_float2(fmaf(sgn, o.x, p.x), fmaf(sgn, o.y, p.y));
+      }
+    }
+  }
+}
+
+template <int P, int TPR, int H>
+LX_DEV void lane_network_inv(float2* x, int t, int lane) {
+  if constexpr (H < TPR) {
+    lane_network<P, TPR, true, H>(x, t, lane);
+    lane_network_inv<P, TPR, H * 2>(x, t, lane);
+  }
+}
+
+// Length-(TPR*P) FFT over TPR lanes, unnormalised.
+//  forward: in  x[m]  = data[TPR*m + t];  out x[k1] = DATA[k1 + P*bitrev(t)]
+//  inverse: in  x[k1] = DATA[k1 + P*bitrev(t)];  out x[m] = data[TPR*m + t]
+// ``tw`` = exp(-2*pi*i*j/(TPR*P)), j < TPR*P (LDS or global table).
+template <int P, int TPR, bool INV>
+LX_DEV void fft_lanes(float2* x, int t, int lane, const float2* tw) {
+  constexpr int L = TPR * P;
+  if constexpr (!INV) {
+    fft_reg<P, false>(x);
+#pragma unroll
+    for (int k = 1; k < P; ++k) x[k] = cmul(x[k], tw[(t * k) & (L - 1)]);
+    lane_network<P, TPR, false, TPR / 2>(x, t, lane);
+  } else {
+    lane_network_inv<P, TPR, 1>(x, t, lane);
+#pragma unroll
+    for (int k = 1; k < P; ++k) x[k] = cmulc(x[k], tw[(t * k) & (L - 1)]);
+    fft_reg<P, true>(x);
+  }
+}
+
+// ----------------------------------------------------------------- growth functions
+// Ids follow the registry order of leniax/growth_functions.py:256-264.
+enum : int {
+  LX_GF_POLY_QUAD4 = 0,
+  LX_GF_GAUSSIAN = 1,
+  LX_GF_GAUSSIAN_TARGET = 2,
+  LX_GF_STEP = 3,
+  LX_GF_STAIRCASE = 4,
+  LX_GF_TRIANGLE = 5,
+  LX_GF_IDENTITY = 6,
+};
+
+LX_DEV float staircase01(float m, float s, float X) {
+  float out = 0.f;
+  out += (X >= m - s && X < m - s / 2.f) ? 0.5f : 0.f;
+  out += (X >= m - s / 2.f && X <= m + s / 2.f) ? 1.f : 0.f;
+  out += (X > m + s / 2.f && X <= m + s) ? 0.5f : 0.f;
+  return out;
+}
+
+LX_DEV float triangle01(float m, float s, float X) {
+  float left = m - s, right = m + s, out = 0.f;
+  out += (X >= left && X < m) ? (X - left) / (m - left) : 0.f;
+  out += (X >= m && X <= right) ? (X - right) / (m - right) : 0.f;
+  return out;
+}
+
+// g(U); ``inv9s2`` = 1/(9 s^2) precomputed for poly_quad4.
+LX_DEV float growth(int kind, float m, float s, float U) {
+  switch (kind) {
+    case LX_GF_POLY_QUAD4: {  // leniax/growth_functions.py:6-45
+      float d = U - m;
+      float o = 1.f - (d * d) / (9.f * (s * s));
+      o = fmaxf(0.f, o);
+      o = o * o;
+      o = o * o;
+      return 2.f * o - 1.f;
+    }
+    case LX_GF_GAUSSIAN: {  // :48-82
+      float z = (U - m) / s;
+      return 2.f * __expf(-(z * z) / 2.f) - 1.f;
+    }
+    case LX_GF_GAUSSIAN_TARGET: {  // :85-119
+      float z = (U - m) / s;
+      return __expf(-(z * z) / 2.f);
+    }
+    case LX_GF_STEP:  // :122-160
+      return fabsf(U - m) <= s ? 1.f : -1.f;
+    case LX_GF_STAIRCASE:  // :163-201
+      return 2.f * staircase01(m, s, U) - 1.f;
+    case LX_GF_TRIANGLE:  // :204-243
+      return 2.f * triangle01(m, s, U) - 1.f;
+    default:  // identity :246-253
+      return U;
+  }
+}
+
+// dg/dU, dg/dm, dg/ds for the differentiable growth functions (adjoint).
+LX_DEV void growth_grad(int kind, float m, float s, float U, float& dU, float& dm, float& ds) {
+  switch (kind) {
+    case LX_GF_POLY_QUAD4: {
+      float d = U - m, s2 = s * s;
+      float o = 1.f - (d * d) / (9.f * s2);
+      if (o <= 0.f) { dU = dm = ds = 0.f; return; }
+      float o3 = o * o * o;
+      float dgdo = 8.f * o3;
+      dU = dgdo * (-2.f * d / (9.f * s2));
+      dm = -dU;
+      ds = dgdo * (2.f * d * d / (9.f * s2 * s));
+      return;
+    }
+    case LX_GF_GAUSSIAN:
+    case LX_GF_GAUSSIAN_TARGET: {
+      float z = (U - m) / s;
+      float e = __expf(-(z * z) / 2.f);
+      float amp = kind == LX_GF_GAUSSIAN ? 2.f : 1.f;
+      dU = -amp * e * z / s;
+      dm = -dU;
+      ds = amp * e * z * z / s;
+      return;
+    }
+    case LX_GF_IDENTITY:
+      dU = 1.f; dm = ds = 0.f; return;
+    default:  // piecewise constant / linear pieces: gradient of the smooth parts only
+      dU = dm = ds = 0.f; return;
+  }
+}
+
+// ----------------------------------------------------------------- wave reductions
+LX_DEV float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}

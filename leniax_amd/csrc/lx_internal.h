@@ -1,0 +1,167 @@
+// Internal declarations shared by the HIP translation units of libleniax_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "../../include/leniax_amd.h"
+#include "lx_device.h"
+
+#define LX_EPS 1e-7f  // leniax/constant.py:8
+#define LX_MONOTONIC_STOP_STEP 128  // leniax/statistics.py:284
+#define LX_MASS_VOLUME_THRESHOLD 10.f  // leniax/statistics.py:313
+#define LX_MASS_VOLUME_STOP_STEP 128  // leniax/statistics.py:314
+
+// Per-step statistic slots, in the order of leniax/statistics.py:102-115
+// (channel masses last).
+enum : int {
+  LX_ST_MASS = 0,
+  LX_ST_MASS_VOLUME,
+  LX_ST_MASS_DENSITY,
+  LX_ST_GROWTH,
+  LX_ST_GROWTH_VOLUME,
+  LX_ST_GROWTH_DENSITY,
+  LX_ST_MASS_SPEED,
+  LX_ST_MASS_ANGLE_SPEED,
+  LX_ST_MASS_GROWTH_DIST,
+  LX_ST_INERTIA,
+  LX_ST_POTENTIAL_VOLUME,
+  LX_ST_CHANNEL_MASS0,
+};
+
+// Per-world statistics carry (leniax/runner.py:271-292) and the online form of
+// the heuristic carry of leniax/statistics.py:144-194.
+struct lx_carry {
+  int32_t shift[3];
+  float centroid[3];
+  float angle;
+  float init_cm[LX_MAX_C];
+  float prev_mass;
+  float prev_sign;
+  int32_t mono;
+  int32_t vol;
+  int32_t cont;
+  int32_t n;
+  int32_t t;
+  int32_t pad;
+};
+
+// Per-world reductions one step's statistics need.  Coordinates are the
+// re-centred ones of leniax/statistics.py:28-33,64 (index - shift mod size - size//2).
+struct lx_sums {
+  float sA, sG, nA, nG, nU;
+  float sAy[3], sGy[3], sAyy[3];
+  float sAc[LX_MAX_C];
+};
+
+// Python-style float remainder (jnp.remainder): result has the divisor's sign.
+LX_DEV float lx_fmod_floor(float x, float y) {
+  float r = fmodf(x, y);
+  if (r != 0.f && ((r < 0.f) != (y < 0.f))) r += y;
+  return r;
+}
+
+// One step of leniax's heuristic scan (leniax/statistics.py:144-194, carry of :186-194):
+// updates the counters and the cumulative continue flag; c.n counts continue steps.
+LX_DEV void lx_heuristic_step(lx_carry& c, float mass, const float* cm, float mass_volume, int C) {
+  if (c.t == 0) {
+    for (int ch = 0; ch < C; ++ch) c.init_cm[ch] = cm[ch];
+    c.prev_mass = mass;
+    c.prev_sign = 0.f;
+    c.mono = 0;
+    c.vol = 0;
+    c.cont = 1;
+    c.n = 0;
+  }
+  int cond = 1;
+  for (int ch = 0; ch < C; ++ch) {
+    cond &= (cm[ch] >= LX_EPS) ? 1 : 0;               // min_channel_mass_heuristic :224-236
+    cond &= (cm[ch] <= 3.f * c.init_cm[ch]) ? 1 : 0;  // max_channel_mass_heuristic :239-251
+  }
+  const float dm = mass - c.prev_mass;
+  const float sign = dm > 0.f ? 1.f : (dm < 0.f ? -1.f : dm * 0.f);
+  c.mono = (sign == c.prev_sign) ? c.mono + 1 : 1;  // monotonic_heuristic :287-306
+  cond &= (c.mono <= LX_MONOTONIC_STOP_STEP) ? 1 : 0;
+  c.vol = (mass_volume > LX_MASS_VOLUME_THRESHOLD) ? c.vol + 1 : 1;  // mass_volume_heuristic :317-333
+  cond &= (c.vol <= LX_MASS_VOLUME_STOP_STEP) ? 1 : 0;
+  c.cont = c.cont & cond;
+  c.n += c.cont;
+  c.prev_mass = mass;
+  c.prev_sign = sign;
+  c.t += 1;
+}
+
+// Statistics of one step from the sums, carry update and stop decision.
+// Follows leniax/statistics.py:64-126 (stats), :134-205 (heuristics, online).
+// R2 = float(R**2), R, dt = float(1/T) are the config's values.
+LX_DEV int lx_stats_finish(const lx_sums& s, lx_carry& c, int ndim, const int32_t* dims, int C, float R2, float R,
+                           float dt, int stop_policy, int min_steps, float cells_total, float* out) {
+  const float mass = s.sA / R2;
+  const float mass_volume = s.nA / R2;
+  const float mass_density = mass / (mass_volume + LX_EPS);
+  const float growth_ = s.sG / R2;
+  const float growth_volume = s.nG / R2;
+  const float growth_density = growth_ / (growth_volume + LX_EPS);
+  const float potential_volume = s.nU / R2;
+  float cen[3], d[3], gc[3];
+  float dist2 = 0.f, gd2 = 0.f, inertia = 0.f;
+  const float m00e = s.sA + LX_EPS, g00e = s.sG + LX_EPS, m002e = s.sA * s.sA + LX_EPS;
+  for (int i = 0; i < ndim; ++i) {
+    cen[i] = s.sAy[i] / m00e;
+    d[i] = cen[i] - c.centroid[i];
+    dist2 += d[i] * d[i];
+    gc[i] = s.sGy[i] / g00e;
+    float e = gc[i] - cen[i];
+    gd2 += e * e;
+    inertia += (s.sAyy[i] - cen[i] * s.sAy[i]) / m002e;
+  }
+  const float dist = sqrtf(dist2);
+  const float mass_speed = dist / R / dt;
+  const float keep = (dist / R > 0.001f) ? 1.f : 0.f;
+  const float ang = (atan2f(d[1], d[0]) * 57.29577951308232f) * keep;
+  const float mass_angle_speed = (lx_fmod_floor(ang - c.angle + 540.f, 360.f) - 180.f) / dt;
+  const float mass_growth_dist = sqrtf(gd2) / R;
+
+  float cm[LX_MAX_C];
+  for (int ch = 0; ch < C; ++ch) cm[ch] = s.sAc[ch] / R2;
+  if (out) {
+    out[LX_ST_MASS] = mass;
+    out[LX_ST_MASS_VOLUME] = mass_volume;
+    out[LX_ST_MASS_DENSITY] = mass_density;
+    out[LX_ST_GROWTH] = growth_;
+    out[LX_ST_GROWTH_VOLUME] = growth_volume;
+    out[LX_ST_GROWTH_DENSITY] = growth_density;
+    out[LX_ST_MASS_SPEED] = mass_speed;
+    out[LX_ST_MASS_ANGLE_SPEED] = mass_angle_speed;
+    out[LX_ST_MASS_GROWTH_DIST] = mass_growth_dist;
+    out[LX_ST_INERTIA] = inertia;
+    out[LX_ST_POTENTIAL_VOLUME] = potential_volume;
+    for (int ch = 0; ch < C; ++ch) out[LX_ST_CHANNEL_MASS0 + ch] = cm[ch];
+  }
+  // carry: leniax/statistics.py:117-124 (truncating shift, floor-mod wrap)
+  for (int i = 0; i < ndim; ++i) {
+    int si = (int)cen[i];
+    int v = (c.shift[i] + si) % dims[i];
+    c.shift[i] = v < 0 ? v + dims[i] : v;
+    c.centroid[i] = cen[i] - (float)si;
+  }
+  c.angle = ang;
+
+  lx_heuristic_step(c, mass, cm, mass_volume, C);
+  switch (stop_policy) {
+    case LX_STOP_LENIAX:
+      return (!c.cont && c.t >= min_steps) ? 1 : 0;
+    case LX_STOP_MASS:
+      return (s.sA == 0.f || s.nA > 0.9f * cells_total) ? 1 : 0;
+    default:
+      return 0;
+  }
+}
+
+// Error reporting (thread-local message), lx_api.hip
+int lx_fail(int code, const char* fmt, ...);
+int lx_check_rule(const lx_rule* rule);
+
+// Fused persistent 2-D run (lx_fused2d.hip)
+int lx_fused2d_supported(const lx_rule* rule);
+int lx_fused2d_launch(const lx_rule* rule, const lx_run_args* a, hipStream_t stream);

@@ -1,0 +1,107 @@
+"""Builders that glue the step together (leniax/helpers.py:35-189, 401-515)."""
+import copy
+import functools
+from typing import Callable, Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import core, engine
+from . import kernels as lx_kernels
+from . import loader as lx_loader
+from . import runner as lx_runner
+from . import statistics as lx_stat
+from . import utils as lx_utils
+from .growth_functions import register as gf_register
+
+
+def create_init_cells(world_size: List[int], nb_channels: int, other_cells=(), offsets=()):
+    """leniax/helpers.py:91-128: zero world with ``other_cells`` merged in, ``[1, C, *world]``."""
+    cells = np.zeros([nb_channels] + list(world_size), np.float32)
+    if isinstance(other_cells, (list, tuple)):
+        if len(offsets) == len(other_cells) and len(offsets):
+            for c, off in zip(other_cells, offsets):
+                if len(c) != 0:
+                    cells = lx_utils.merge_cells(cells, c, off)
+        else:
+            for c in other_cells:
+                if len(c) != 0:
+                    cells = lx_utils.merge_cells(cells, c)
+        return engine.as_device(cells[np.newaxis])
+    return engine.as_device(other_cells)
+
+
+def init(config: Dict, use_init_cells: bool = True, fft: bool = True):
+    """leniax/helpers.py:35-88: initial state, spectra ``K`` and the kernel mapping."""
+    nb_dims = config['world_params']['nb_dims']
+    nb_channels = config['world_params']['nb_channels']
+    world_size = config['render_params']['world_size']
+    assert len(world_size) == nb_dims and nb_channels > 0
+    raw_cells = lx_loader.load_raw_cells(config, use_init_cells)
+    scale = config['world_params'].get('scale', 1.)
+    if scale != 1.:
+        import scipy.ndimage
+        raw_cells = np.asarray([scipy.ndimage.zoom(raw_cells[i], scale, order=0) for i in range(nb_channels)],
+                               dtype=np.float32)
+        config['world_params']['R'] *= scale
+    if raw_cells.ndim > 1 + nb_dims:
+        init_cells = create_init_cells(world_size, nb_channels, list(raw_cells))
+    else:
+        init_cells = create_init_cells(world_size, nb_channels, [raw_cells])
+    K, mapping = lx_kernels.get_kernels_and_mapping(config['kernels_params'], world_size, nb_channels,
+                                                    config['world_params']['R'], fft)
+    return init_cells, K, mapping
+
+
+def build_get_potential_fn(kernel_shape, true_channels: Optional[List[bool]] = None, fft: bool = True,
+                           channel_first: bool = True) -> Callable:
+    """leniax/helpers.py:430-488."""
+    tc_indices = None if true_channels is None else tuple(i for i, t in enumerate(true_channels) if t)
+    if fft is not True:
+        return functools.partial(core.get_potential, padding=None, tc_indices=tc_indices, channel_first=channel_first)
+    return functools.partial(core.get_potential_fft, tc_indices=tc_indices, channel_first=channel_first)
+
+
+def build_get_field_fn(cin_gfs: List[List[str]], average: bool = True) -> Callable:
+    """leniax/helpers.py:491-515."""
+    growth_fn_t = tuple(gf_register[slug] for per_channel in cin_gfs for slug in per_channel)
+    weighted_fn = core.weighted_mean if average else core.weighted_sum
+    return functools.partial(core.get_field, growth_fn_t=growth_fn_t, weighted_fn=weighted_fn)
+
+
+def build_update_fn(kernel_shape, mapping: lx_kernels.KernelMapping, get_state_fn_slug: str = 'v1',
+                    average_weight: bool = True, fft: bool = True) -> Callable:
+    """leniax/helpers.py:401-427."""
+    get_potential_fn = build_get_potential_fn(kernel_shape, mapping.true_channels, fft)
+    get_field_fn = build_get_field_fn(mapping.cin_gfs, average_weight)
+    get_state_fn = core.register[get_state_fn_slug]
+    return functools.partial(core.update, get_potential_fn=get_potential_fn, get_field_fn=get_field_fn,
+                             get_state_fn=get_state_fn)
+
+
+def init_and_run(rng_key, config: Dict, use_init_cells: bool = True, with_jit: bool = True, fft: bool = True,
+                 stat_trunc: bool = False):
+    """leniax/helpers.py:131-189."""
+    config = copy.deepcopy(config)
+    cells, K, mapping = init(config, use_init_cells, fft)
+    gf_params = mapping.get_gf_params()
+    weights = mapping.get_kernels_weight_per_channel()
+    wp = config['world_params']
+    slug = wp.get('get_state_fn_slug', 'v1')
+    average = wp.get('weighted_average', True)
+    R = wp['R']
+    T = np.float32(wp['T'])
+    max_run_iter = config['run_params']['max_run_iter']
+    update_fn = build_update_fn(K.shape, mapping, slug, average, fft)
+    compute_stats_fn = lx_stat.build_compute_stats_fn(wp, config['render_params'])
+    if with_jit:
+        cells_, fields, pots, stats = lx_runner.run_scan(rng_key, cells, K, gf_params, weights, T, max_run_iter, R,
+                                                         update_fn, compute_stats_fn)
+    else:
+        cells_, fields, pots, stats = lx_runner.run(rng_key, cells, K, gf_params, weights, T, max_run_iter, R,
+                                                    update_fn, compute_stats_fn, stat_trunc)
+    stats = {k: v.squeeze() for k, v in stats.items()}
+    if stat_trunc:
+        n = int(stats['N'])
+        cells_, fields, pots = cells_[:n], fields[:n], pots[:n]
+    return cells_, fields, pots, stats

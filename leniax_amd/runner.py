@@ -1,0 +1,235 @@
+"""Scan runners (leniax/runner.py) on the GPU.
+
+* ``run_scan`` / ``run_scan_mem_optimized`` with the standard update function
+  and the standard statistics run as ONE persistent HIP launch (lx_run) when the
+  rule has a fused kernel (2-D N x N, one channel, one kernel, N <= 128): every
+  world stays on-chip for all steps, statistics and heuristics are evaluated
+  every step.  Other rules step with the plain HIP step (lx_update + lx_stats)
+  from a host loop, then scan the heuristics on the device.
+* ``run`` keeps leniax's Python loop (different heuristics, optional truncation).
+"""
+import ctypes
+import functools
+from typing import Callable, Dict
+
+import numpy as np
+import torch
+
+from . import _lib, engine
+from . import statistics as lx_stat
+from .constant import EPSILON, START_CHECK_STOP
+from .core import standard_parts, fused_update, _spectra, _slots, _rule, _pad_gf
+
+
+def _fused_rule(update_fn, compute_stats_fn, cells0, K):
+    """The lx_rule of a standard update_fn, or None when update_fn is custom."""
+    if not (isinstance(update_fn, functools.partial) and update_fn.func.__name__ == 'update'):
+        return None
+    kw = update_fn.keywords
+    parts = standard_parts(kw.get('get_potential_fn'), kw.get('get_field_fn'), kw.get('get_state_fn'))
+    if parts is None or not getattr(compute_stats_fn, 'lx_stats', False):
+        return None
+    tc_indices, growth_fn_t, weighted_fn, slug = parts
+    C = cells0.shape[-len(compute_stats_fn.world_size) - 1]
+    maxk = K.shape[-len(compute_stats_fn.world_size) - 1]
+    slots = _slots(C, maxk, tc_indices)
+    rule = _rule(compute_stats_fn.world_size, C, len(slots), [s // maxk for s in slots], growth_fn_t, weighted_fn,
+                 slug)
+    return rule, slots
+
+
+def _dt_of(T):
+    # leniax computes dt = 1. / T with T a float32 array (leniax/runner.py:307)
+    T = np.asarray(T.detach().cpu() if isinstance(T, torch.Tensor) else T, dtype=np.float32)
+    return np.float32(1) / T
+
+
+def _scan_plain(rule, slots, cells0, K_half, kbs, gf, w, dt, max_run_iter, R, T, keep):
+    """Host loop of plain HIP steps over B worlds (all arrays already flattened per world)."""
+    B = cells0.shape[0]
+    dims = tuple(cells0.shape[2:])
+    dev = cells0.device
+    C, Ke = rule.C, rule.K
+    nst = _lib.LX_NSTAT_BASE + C
+    rows = torch.empty((max_run_iter, B, nst), dtype=torch.float32, device=dev)
+    carry = engine.carry_init(B)
+    work_n = ctypes.c_size_t(0)
+    _lib.call('lx_workspace_size', ctypes.byref(rule), B, ctypes.byref(work_n))
+    work = torch.empty(max(work_n.value, 1), dtype=torch.uint8, device=dev)
+    gfs = int(gf[0].numel()) if gf.dim() == 3 else 0
+    ws = int(w[0].numel()) if w.dim() == 3 else 0
+    dts = 1 if dt.numel() > 1 else 0
+    if keep:
+        cells_h = torch.empty((max_run_iter, B, C) + dims, dtype=torch.float32, device=dev)
+        field_h = torch.empty((max_run_iter, B, C) + dims, dtype=torch.float32, device=dev)
+        pot_h = torch.empty((max_run_iter, B, Ke) + dims, dtype=torch.float32, device=dev)
+    else:
+        field_h = torch.empty((1, B, C) + dims, dtype=torch.float32, device=dev)
+        pot_h = torch.empty((1, B, Ke) + dims, dtype=torch.float32, device=dev)
+    cur = cells0.clone()
+    nxt = torch.empty_like(cur)
+    for t in range(max_run_iter):
+        f = field_h[t if keep else 0]
+        p = pot_h[t if keep else 0]
+        if keep:
+            cells_h[t].copy_(cur)
+        _lib.call('lx_update', ctypes.byref(rule), engine.ptr(cur), B, engine.ptr(K_half), kbs, engine.ptr(gf), gfs,
+                  engine.ptr(w), ws, engine.ptr(dt), dts, engine.ptr(nxt), engine.ptr(f), engine.ptr(p),
+                  engine.ptr(work), engine.stream_ptr())
+        _lib.call('lx_stats', ctypes.byref(rule), engine.ptr(cur), engine.ptr(f), engine.ptr(p), B, float(R), float(T),
+                  engine.ptr(carry), engine.ptr(rows[t]), engine.stream_ptr())
+        cur, nxt = nxt, cur
+    cont = torch.empty((max_run_iter, B), dtype=torch.float32, device=dev)
+    n = torch.empty((B,), dtype=torch.float32, device=dev)
+    _lib.call('lx_check_heuristics', engine.ptr(rows), max_run_iter, B, C, engine.ptr(cont), engine.ptr(n),
+              engine.stream_ptr())
+    hist = (cells_h, field_h, pot_h) if keep else None
+    return rows, n, cur, hist
+
+
+def run_scan(rng_key, cells0, K, gf_params, kernels_weight_per_channel, T, max_run_iter: int, R: float,
+             update_fn: Callable, compute_stats_fn: Callable):
+    """leniax/runner.py:119-164.  Returns ``cells [T, N, C, ...]`` (state before each step),
+    ``field``, ``potential`` and the stats dict (``[T, N]`` arrays, ``N`` from the heuristics)."""
+    cells0 = engine.as_device(cells0)
+    K = engine.as_device(K, torch.complex64)
+    fr = _fused_rule(update_fn, compute_stats_fn, cells0, K)
+    if fr is None:
+        return _run_scan_generic(rng_key, cells0, K, gf_params, kernels_weight_per_channel, T, max_run_iter, R,
+                                 update_fn, compute_stats_fn)
+    rule, slots = fr
+    B = cells0.shape[0]
+    Kh = _spectra(rule, K, slots)
+    kbs = 0 if K.shape[0] == 1 else int(Kh[0].numel())
+    gf = _pad_gf(engine.as_device(gf_params), rule.K)
+    w = engine.as_device(kernels_weight_per_channel)
+    dt = engine.as_device(_dt_of(T)).reshape(-1)
+    Rs, Ts = compute_stats_fn.R, compute_stats_fn.T
+    if _lib.lib().lx_run_supported(ctypes.byref(rule)):
+        res = engine.run_fused(rule, cells0, Kh, kbs != 0, gf.reshape((1,) + tuple(gf.shape)),
+                               w.reshape((1,) + tuple(w.shape)), dt, max_run_iter, 'none', 0, Rs, Ts,
+                               want_stats=True, want_hist=True)
+        rows = res['stats'].transpose(0, 1)  # [T, B, nst]
+        n = res['n'].to(torch.float32)
+        cells_h, field_h, pot_h = res['hist']
+    else:
+        rows, n, _, (cells_h, field_h, pot_h) = _scan_plain(rule, slots, cells0, Kh, kbs, gf, w, dt, max_run_iter,
+                                                            Rs, Ts, keep=True)
+    stats = lx_stat.stats_rows_to_dict(rows, rule.C)
+    stats['N'] = n
+    return cells_h, field_h, pot_h, stats
+
+
+def _run_scan_generic(rng_key, cells0, K, gf_params, w, T, max_run_iter, R, update_fn, compute_stats_fn):
+    """run_scan for custom update / statistics functions: call them step by step."""
+    N = cells0.shape[0]
+    nd = cells0.dim() - 2
+    dt = _dt_of(T)
+    shift = torch.zeros((N, nd), dtype=torch.int32, device=cells0.device)
+    centroid = torch.zeros((nd, N), dtype=torch.float32, device=cells0.device)
+    angle = torch.zeros((N,), dtype=torch.float32, device=cells0.device)
+    cells = cells0
+    ys_c, ys_f, ys_p, ys_s = [], [], [], []
+    for _ in range(max_run_iter):
+        new_cells, field, potential = update_fn(None, cells, K, gf_params, w, dt)
+        st, shift, centroid, angle = compute_stats_fn(cells, field, potential, shift, centroid, angle)
+        ys_c.append(cells)
+        ys_f.append(field)
+        ys_p.append(potential)
+        ys_s.append(st)
+        cells = new_cells
+    stats = lx_stat.stats_list_to_dict(ys_s)
+    stats['N'] = lx_stat.check_heuristics(stats).sum(dim=0)
+    return torch.stack(ys_c), torch.stack(ys_f), torch.stack(ys_p), stats
+
+
+def run_scan_mem_optimized(rng_key, cells0, K, gf_params, kernels_weight_per_channel, T, max_run_iter: int,
+                           R: float, update_fn: Callable, compute_stats_fn: Callable):
+    """leniax/runner.py:167-215 (vmapped over solutions S).
+
+    cells0 ``[S, N_init, C, ...]``, K ``[S, 1, C, max_k, ...]``, gf ``[S, nb_k, 2]``,
+    weights ``[S, C, nb_k]``, T ``[S]``.  Returns ``(stats [S, T, N_init], final_cells)``.
+    The per-solution T drives the update; the statistics use the config's R and T
+    (leniax/statistics.py:22-24).
+    """
+    cells0 = engine.as_device(cells0)
+    S, N_init = cells0.shape[:2]
+    K = engine.as_device(K, torch.complex64)
+    Kf = K.reshape((S,) + tuple(K.shape[-(cells0.dim() - 1):]))  # [S, C, maxk, *world]
+    fr = _fused_rule(update_fn, compute_stats_fn, cells0[0], Kf)
+    if fr is None:
+        raise NotImplementedError('run_scan_mem_optimized needs the standard update and statistics functions')
+    rule, slots = fr
+    B = S * N_init
+    Kh = engine.prepare_spectra(rule, Kf, slots)  # [S, K, ...]
+    if N_init > 1:
+        Kh = Kh.repeat_interleave(N_init, dim=0).contiguous()
+    kbs = int(Kh[0].numel())
+    gf = _pad_gf(engine.as_device(gf_params), rule.K).repeat_interleave(N_init, dim=0).contiguous()
+    w = engine.as_device(kernels_weight_per_channel).repeat_interleave(N_init, dim=0).contiguous()
+    dt = engine.as_device(_dt_of(T)).reshape(-1).repeat_interleave(N_init).contiguous()
+    flat = cells0.reshape((B,) + tuple(cells0.shape[2:])).contiguous()
+    Rs, Ts = compute_stats_fn.R, compute_stats_fn.T
+    if _lib.lib().lx_run_supported(ctypes.byref(rule)):
+        res = engine.run_fused(rule, flat, Kh, True, gf, w, dt, max_run_iter, 'none', 0, Rs, Ts, want_stats=True)
+        rows = res['stats']  # [B, T, nst]
+        n = res['n'].to(torch.float32)
+        final = res['cells']
+    else:
+        rows, n, final, _ = _scan_plain(rule, slots, flat, Kh, kbs, gf, w, dt, max_run_iter, Rs, Ts, keep=False)
+        rows = rows.transpose(0, 1)
+    nst = rows.shape[-1]
+    rows = rows.reshape(S, N_init, max_run_iter, nst).transpose(1, 2)  # [S, T, N_init, nst]
+    stats = lx_stat.stats_rows_to_dict(rows, rule.C)
+    stats['N'] = n.reshape(S, N_init)
+    return stats, final.reshape(cells0.shape)
+
+
+def run(rng_key, cells, K, gf_params, kernels_weight_per_channel, T, max_run_iter: int, R: float,
+        update_fn: Callable, compute_stats_fn: Callable, stat_trunc: bool = False):
+    """leniax/runner.py:16-116: Python loop, total-mass heuristics, ``N`` = last iteration."""
+    assert max_run_iter > 0, f'max_run_iter must be positive, value given: {max_run_iter}'
+    cells = engine.as_device(cells)
+    assert cells.shape[0] == 1
+    nd = cells.dim() - 2
+    dev = cells.device
+    dt = _dt_of(T)
+    all_cells, all_fields, all_pots, all_stats = [cells], [], [], []
+    init_mass = float(cells.sum())
+    previous_mass = init_mass
+    previous_sign = 0.
+    monotone = volume = 0
+    should_continue = 1
+    shift = torch.zeros((1, nd), dtype=torch.int32, device=dev)
+    centroid = torch.zeros((nd, 1), dtype=torch.float32, device=dev)
+    angle = torch.zeros((1,), dtype=torch.float32, device=dev)
+    current_iter = 0
+    for current_iter in range(max_run_iter):
+        new_cells, field, potential = update_fn(None, cells, K, gf_params, kernels_weight_per_channel, dt)
+        stat_t, shift, centroid, angle = compute_stats_fn(cells, field, potential, shift, centroid, angle)
+        cells = new_cells
+        all_cells.append(cells)
+        all_fields.append(field)
+        all_pots.append(potential)
+        all_stats.append(stat_t)
+        if stat_trunc:
+            # leniax/runner.py:84-104 (previous_mass / previous_sign are never updated there)
+            mass = float(stat_t['mass'][0])
+            cond = mass >= EPSILON and mass <= 3 * init_mass
+            sign = float(np.sign(mass - previous_mass))
+            monotone = monotone * (sign == previous_sign) + 1
+            cond = cond and monotone <= lx_stat.MONOTONIC_STOP_STEP
+            volume = volume * (float(stat_t['mass_volume'][0]) > lx_stat.MASS_VOLUME_THRESHOLD) + 1
+            cond = cond and volume <= lx_stat.MASS_VOLUME_STOP_STEP
+            should_continue *= int(cond)
+            if current_iter >= START_CHECK_STOP and should_continue == 0:
+                break
+    all_cells.pop()
+    stats = lx_stat.stats_list_to_dict(all_stats)
+    stats['N'] = torch.tensor(current_iter)
+    return torch.stack(all_cells), torch.stack(all_fields), torch.stack(all_pots), stats
+
+
+def normalize_fn(g):
+    """leniax/runner.py:447-448."""
+    return g / (torch.linalg.norm(g) + 1e-8)

@@ -1,0 +1,233 @@
+"""GPU parity of the leniax-interface functions (core, kernels, statistics, runner, helpers)
+against the oracle, the reference's known answers and its golden frames."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from oracle import lenia as L
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip('torch')
+
+TOL_GOLD = {'orbium': 1.5e-4, 'orbium_scutium': 1.5e-4, 'aquarium': 1.5e-3}  # tests/test_pipeline.py
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _config(name):
+    meta, cells0, frame = load_golden(name)
+    cfg = {
+        'world_params': {'nb_dims': 2, 'nb_channels': meta['nb_channels'], 'R': meta['R'], 'T': meta['T'],
+                         'weighted_average': meta['weighted_average'], 'get_state_fn_slug': meta['get_state_fn_slug'],
+                         'scale': 1.},
+        'render_params': {'world_size': meta['world_size']},
+        'kernels_params': [dict(k) for k in meta['kernels_params']],
+        'run_params': {'cells': cells0[0], 'max_run_iter': meta['max_run_iter']},
+    }
+    return cfg, meta, cells0, frame
+
+
+@pytest.mark.parametrize('name', ['orbium', 'orbium_scutium', 'aquarium'])
+def test_init_and_run_golden(name):
+    """tests/test_pipeline.py:37-130 through leniax_amd.helpers.init_and_run (scan runner)."""
+    from leniax_amd import helpers
+    cfg, meta, cells0, frame = _config(name)
+    cells, field, pot, stats = helpers.init_and_run(None, cfg, with_jit=True)
+    assert cells.shape[0] == meta['max_run_iter']
+    err = float(np.abs(_np(cells[-1, 0]) - frame).max())
+    print(name, 'golden err', err)
+    assert err < TOL_GOLD[name]
+
+
+def test_init_and_run_python_loop_golden():
+    """tests/test_pipeline.py:18-35 (runner.run, with_jit=False)."""
+    from leniax_amd import helpers
+    cfg, meta, cells0, frame = _config('orbium')
+    cells, _, _, stats = helpers.init_and_run(None, cfg, with_jit=False)
+    assert cells.shape[0] == 128
+    assert float(np.abs(_np(cells[-1, 0]) - frame).max()) < TOL_GOLD['orbium']
+
+
+def test_kernel_spectrum_matches_oracle():
+    from leniax_amd import kernels
+    cfg, meta, cells0, frame = _config('aquarium')
+    K, mp = kernels.get_kernels_and_mapping([dict(k) for k in meta['kernels_params']], meta['world_size'], 3,
+                                            meta['R'])
+    Ko, mpo = L.get_kernels_and_mapping([dict(k) for k in meta['kernels_params']], meta['world_size'], 3, meta['R'],
+                                        np.float32)
+    assert tuple(K.shape) == Ko.shape
+    assert float(np.abs(_np(K) - Ko).max()) < 2e-6
+    assert mp.true_channels == mpo.true_channels
+
+
+def test_potential_constant_field_known_answer():
+    """FFT analogue of tests/test_core.py:54-101: 0.09 / 0.18 / 0.54."""
+    from leniax_amd import core
+    H = W = 16
+    A = np.ones([1, 2, H, W], np.float32)
+    A[0, 0] = 0.1
+    A[0, 1] = 0.2
+    K = np.zeros([1, 2, 2, H, W], np.complex64)
+    K[0, 0, 0, 0, 0] = 0.9
+    K[0, 0, 1, 0, 0] = 1.8
+    K[0, 1, 0, 0, 0] = 2.7
+    U = _np(core.get_potential_fft(A, K, tc_indices=(0, 1, 2)))
+    assert U.shape == (1, 3, H, W)
+    np.testing.assert_allclose(U[0, 0], 0.09, rtol=1e-6)
+    np.testing.assert_allclose(U[0, 1], 0.18, rtol=1e-6)
+    np.testing.assert_allclose(U[0, 2], 0.54, rtol=1e-6)
+
+
+@pytest.mark.parametrize('shape', [(2, 64, 64), (1, 128, 32), (1, 256, 256), (2, 512, 64)])
+def test_potential_field_state_match_oracle(shape):
+    from leniax_amd import core
+    C, H, W = shape
+    rng = np.random.default_rng(H + W)
+    kp = [{'k_slug': 'circle_2d', 'k_params': [r, b], 'kf_slug': 'gauss_bump', 'kf_params': [1.],
+           'gf_slug': g, 'gf_params': [m, s], 'h': h, 'c_in': ci, 'c_out': co}
+          for (r, b, g, m, s, h, ci, co) in [(1., [1.], 'gaussian', 0.15, 0.015, 1., 0, 0),
+                                             (0.7, [1., 0.5], 'poly_quad4', 0.25, 0.03, 0.5, 0, C - 1),
+                                             (0.9, [0.3, 1.], 'gaussian', 0.3, 0.05, 0.7, C - 1, 0)]]
+    R = min(H, W) / 6.
+    Ko, mp = L.get_kernels_and_mapping([dict(k) for k in kp], [H, W], C, R, np.float32)
+    A = (rng.random((2, C, H, W), dtype=np.float32) * (rng.random((2, C, H, W)) > 0.5)).astype(np.float32)
+    Uo = L.get_potential_fft(A, Ko, mp.tc_indices())
+    U = _np(core.get_potential_fft(A, Ko, mp.tc_indices()))
+    print(shape, 'potential err', float(np.abs(U - Uo).max()))
+    assert float(np.abs(U - Uo).max()) < 2e-6
+    from leniax_amd.growth_functions import register as gfr
+    gfn = tuple(gfr[s] for s in mp.gf_slugs())
+    Go = L.get_field(Uo, mp.get_gf_params(), mp.get_kernels_weight_per_channel(), mp.gf_slugs(), True)
+    G = _np(core.get_field(Uo, mp.get_gf_params(), mp.get_kernels_weight_per_channel(), gfn, core.weighted_mean))
+    assert float(np.abs(G - Go).max()) < 1e-5
+    Go_s = L.get_field(Uo, mp.get_gf_params(), mp.get_kernels_weight_per_channel(), mp.gf_slugs(), False)
+    G_s = _np(core.get_field(Uo, mp.get_gf_params(), mp.get_kernels_weight_per_channel(), gfn, core.weighted_sum))
+    assert float(np.abs(G_s - Go_s).max()) < 1e-5
+    dt = np.float32(0.1)
+    np.testing.assert_array_equal(_np(core.get_state(None, A, Go, dt)), L.get_state(A, Go, dt))
+    np.testing.assert_array_equal(_np(core.get_state_v2(None, A, Go, dt)), L.get_state_v2(A, Go, dt))
+    np.testing.assert_array_equal(_np(core.get_state_simple(None, A, Go, dt)), L.get_state_simple(A, Go, dt))
+    # the fused plain update equals the three calls
+    new, field, pot = core.fused_update(A, Ko, mp.get_gf_params(), mp.get_kernels_weight_per_channel(), dt,
+                                        mp.tc_indices(), gfn, core.weighted_mean, 'v1')
+    assert float(np.abs(_np(pot) - Uo).max()) < 2e-6
+    assert float(np.abs(_np(new) - L.get_state(A, Go, dt)).max()) < 1e-5
+
+
+def test_get_state_known_answers():
+    """tests/test_core.py:161-185."""
+    from leniax_amd import core
+    out1 = core.get_state(None, np.full((25, 25), .5, np.float32), np.full((25, 25), 3., np.float32), 1. / 3.)
+    out2 = core.get_state(None, np.full((25, 25), .2, np.float32), np.full((25, 25), 3., np.float32), 1. / 6.)
+    np.testing.assert_array_almost_equal(_np(out1), np.ones((25, 25)))
+    np.testing.assert_array_almost_equal(_np(out2), np.full((25, 25), .7))
+
+
+def test_growth_functions_on_device():
+    from leniax_amd import growth_functions as gf
+    x = np.linspace(0, 1, 1001, dtype=np.float32)
+    for slug in ['poly_quad4', 'gaussian', 'gaussian_target', 'step', 'staircase', 'triangle', 'identity']:
+        got = _np(gf.register[slug]([0.3, 0.05], x))
+        ref = L.GF[slug]([0.3, 0.05], x, np.float32)
+        assert float(np.abs(got - ref).max()) < 2e-6, slug
+
+
+def test_plain_update_per_step_aquarium():
+    """3 channels, 15 kernels, weighted_sum, cross-channel c_out: plain step per step vs the oracle."""
+    from leniax_amd import core, helpers, kernels
+    cfg, meta, cells0, frame = _config('aquarium')
+    K, mp = kernels.get_kernels_and_mapping([dict(k) for k in meta['kernels_params']], meta['world_size'], 3,
+                                            meta['R'])
+    Ko, mpo = L.get_kernels_and_mapping([dict(k) for k in meta['kernels_params']], meta['world_size'], 3, meta['R'],
+                                        np.float32)
+    upd = helpers.build_update_fn(K.shape, mp, 'v1', False)
+    rule_o = L.Rule(mpo.gf_slugs(), mpo.tc_indices(), False, 'v1')
+    dt = np.float32(1) / np.float32(meta['T'])
+    a = cells0.copy()
+    g = torch.as_tensor(cells0).cuda()
+    errs = []
+    for t in range(16):
+        a, fo, po = rule_o.update(a, Ko, mpo.get_gf_params(), mpo.get_kernels_weight_per_channel(), dt)
+        g, f, p = upd(None, g, K, mp.get_gf_params(), mp.get_kernels_weight_per_channel(), dt)
+        errs.append(float(np.abs(_np(g) - a).max()))
+    print('aquarium plain per-step', ['%.1e' % e for e in errs])
+    assert max(errs[:8]) < 1e-5
+
+
+def test_compute_stats_matches_oracle():
+    from leniax_amd import statistics as S
+    meta, cells0, frame = load_golden('orbium_scutium')
+    Ko, mp = L.get_kernels_and_mapping([dict(k) for k in meta['kernels_params']], meta['world_size'], 2, meta['R'],
+                                       np.float32)
+    rule = L.Rule(mp.gf_slugs(), mp.tc_indices(), True, 'v1')
+    fo = L.build_compute_stats_fn(meta['world_size'], meta['R'], meta['T'], np.float32)
+    fn = S.build_compute_stats_fn({'R': meta['R'], 'T': meta['T']}, {'world_size': meta['world_size']})
+    a = cells0
+    carry_o = L.init_stats_carry(1, 2)
+    carry = (torch.zeros(1, 2, dtype=torch.int32).cuda(), torch.zeros(2, 1).cuda(), torch.zeros(1).cuda())
+    for t in range(40):
+        new, field, pot = rule.update(a, Ko, mp.get_gf_params(), mp.get_kernels_weight_per_channel(), np.float32(0.1))
+        so, *carry_o = fo(a, field, pot, *carry_o)
+        sg, *carry = fn(a, field, pot, *carry)
+        for k in S.STAT_KEYS:
+            o, gv = float(so[k][0]), float(_np(sg[k])[0])
+            assert abs(gv - o) <= 2e-3 * max(1., abs(o)), (t, k, gv, o)
+        np.testing.assert_allclose(_np(sg['channel_mass'])[0], so['channel_mass'][0], rtol=1e-5, atol=1e-7)
+        assert _np(carry[0]).tolist() == carry_o[0].tolist()
+        a = new
+
+
+def test_check_heuristics_known_answers():
+    from leniax_amd import statistics as S
+    T, N = 300, 4
+    mass = np.ones([T, N], np.float32)
+    mass[np.arange(T) % 2 == 1] = 1.01
+    cm = mass[..., None] / 2
+    mv = np.zeros([T, N], np.float32)
+    cm[50:, 1] = 0.
+    cm[70:, 2] = 3.1
+    mv[:, 3] = 11.
+    mass[:, 0] = np.linspace(1, 2, T)
+    cont = S.check_heuristics({'mass': mass, 'channel_mass': cm, 'mass_volume': mv})
+    np.testing.assert_array_equal(_np(cont).sum(axis=0), [129, 50, 70, 128])
+    np.testing.assert_array_equal(_np(cont), L.check_heuristics({'mass': mass, 'channel_mass': cm,
+                                                                 'mass_volume': mv}))
+
+
+@pytest.mark.parametrize('plain', [False, True])
+def test_run_scan_mem_optimized_matches_oracle(plain):
+    """leniax/runner.py:167-215 with S solutions x N_init inits, per-solution spectra / T."""
+    from leniax_amd import helpers, kernels, runner, statistics as S
+    N = 64 if not plain else 32  # plain: 2 channels forces the multi-pass path
+    C = 1 if not plain else 2
+    rng = np.random.default_rng(5)
+    Ks, gfs, ws, Ko_l = [], [], [], []
+    for s in range(2):
+        kp = [{'k_slug': 'circle_2d', 'k_params': [1., [1.]], 'kf_slug': 'gauss_bump', 'kf_params': [1.],
+               'gf_slug': 'gaussian', 'gf_params': [0.15 + 0.02 * s, 0.02], 'h': 1., 'c_in': c, 'c_out': c}
+              for c in range(C)]
+        K, mp = kernels.get_kernels_and_mapping([dict(k) for k in kp], [N, N], C, 8.)
+        Ko, mpo = L.get_kernels_and_mapping([dict(k) for k in kp], [N, N], C, 8., np.float32)
+        Ks.append(K)
+        Ko_l.append((Ko, mpo))
+        gfs.append(mp.get_gf_params())
+        ws.append(mp.get_kernels_weight_per_channel())
+    cells0 = np.zeros((2, 2, C, N, N), np.float32)
+    cells0[..., N // 4:3 * N // 4, N // 4:3 * N // 4] = rng.random((2, 2, C, N // 2, N // 2), dtype=np.float32)
+    T = np.array([10., 5.], np.float32)
+    upd = helpers.build_update_fn(Ks[0].shape, mp, 'v1', True)
+    sfn = S.build_compute_stats_fn({'R': 8., 'T': 10.}, {'world_size': [N, N]})
+    stats, final = runner.run_scan_mem_optimized(None, cells0, torch.stack(Ks), torch.stack(gfs), torch.stack(ws), T,
+                                                 24, 8., upd, sfn)
+    assert tuple(stats['mass'].shape) == (2, 24, 2) and tuple(final.shape) == cells0.shape
+    rule_o = L.Rule(mpo.gf_slugs(), None, True, 'v1')
+    so, fo = L.run_scan_mem_optimized(cells0, np.stack([k for k, _ in Ko_l]), np.stack([m.get_gf_params() for _, m in Ko_l]),
+                                      np.stack([m.get_kernels_weight_per_channel() for _, m in Ko_l]), T, 24, 8., rule_o,
+                                      L.build_compute_stats_fn([N, N], 8., 10., np.float32))
+    err = float(np.abs(_np(final) - fo).max())
+    print('mem_optimized final err', err)
+    assert err < 1e-3
+    np.testing.assert_allclose(_np(stats['mass']), so['mass'], rtol=1e-3, atol=1e-5)
+    np.testing.assert_array_equal(_np(stats['N']), so['N'])

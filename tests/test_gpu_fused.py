@@ -1,0 +1,116 @@
+"""GPU parity of the fused persistent run (lx_run) against the oracle and the golden frames."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from oracle import lenia as L
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip('torch')
+
+# per-step state bound over the first 32 steps (float32 FFT rounding vs the float32 oracle)
+STEP_TOL = 1e-5
+
+
+def _engine():
+    from leniax_amd import engine, _lib
+    return engine, _lib
+
+
+def _bump_rule(R=13., m=0.15, s=0.015):
+    return [{'k_slug': 'circle_2d', 'k_params': [1., [1.]], 'kf_slug': 'gauss_bump', 'kf_params': [1.],
+             'gf_slug': 'gaussian', 'gf_params': [m, s], 'h': 1., 'c_in': 0, 'c_out': 0}]
+
+
+def _run_both(kernels_params, cells0, R, T, steps, hist=True):
+    engine, _lib = _engine()
+    dims = list(cells0.shape[2:])
+    K, mp = L.get_kernels_and_mapping(kernels_params, dims, 1, R, np.float32)
+    gf_np = mp.get_gf_params()
+    w_np = mp.get_kernels_weight_per_channel()
+    rule_o = L.Rule(mp.gf_slugs(), mp.tc_indices(), True, 'v1')
+    stats_fn = L.build_compute_stats_fn(dims, R, T, np.float32)
+    o_cells, o_field, o_pot, o_stats = L.run_scan(cells0, K, gf_np, w_np, T, steps, R, rule_o, stats_fn)
+
+    rule = engine.make_rule(dims, 1, [0], [_lib.GROWTH_IDS[mp.gf_slugs()[0]]], 0, 1)
+    Kh = engine.prepare_spectra(rule, K, [0])
+    dev = engine.device()
+    c0 = torch.as_tensor(cells0).to(dev)
+    gf = torch.as_tensor(gf_np).reshape(1, 1, 2).to(dev)
+    w = torch.as_tensor(w_np).reshape(1, 1, 1).to(dev)
+    dt = torch.tensor([np.float32(1) / np.float32(T)], device=dev)
+    res = engine.run_fused(rule, c0, Kh, False, gf, w, dt, steps, 'none', 0, R, T, want_stats=True, want_hist=hist)
+    torch.cuda.synchronize()
+    return (o_cells, o_field, o_pot, o_stats), res
+
+
+def _per_step_err(o_cells, hist_cells, nsteps):
+    g = hist_cells.cpu().numpy()
+    return [float(np.abs(g[t] - o_cells[t]).max()) for t in range(nsteps)]
+
+
+def test_fused_orbium_golden_frame_and_per_step():
+    meta, cells0, frame = load_golden('orbium')
+    (o_cells, o_field, o_pot, o_stats), res = _run_both(meta['kernels_params'], cells0, meta['R'], meta['T'], 128)
+    errs = _per_step_err(o_cells, res['hist'][0], 32)
+    print('orbium per-step max err (32 steps):', max(errs), 'first steps', errs[:4])
+    g = res['hist'][0].cpu().numpy()
+    gold_err = float(np.abs(g[127, 0] - frame).max())
+    print('orbium golden err', gold_err)
+    assert gold_err < 1.5e-4  # tests/test_pipeline.py:35 decimal=4
+    assert max(errs) < STEP_TOL
+    # potential and field of step 0
+    assert float(np.abs(res['hist'][2].cpu().numpy()[0] - o_pot[0]).max()) < 1e-5
+    assert float(np.abs(res['hist'][1].cpu().numpy()[0] - o_field[0]).max()) < 1e-4
+
+
+def _oracle64_cells(kernels_params, cells0, R, T, steps):
+    dims = list(cells0.shape[2:])
+    K, mp = L.get_kernels_and_mapping(kernels_params, dims, 1, R, np.float64)
+    rule = L.Rule(mp.gf_slugs(), mp.tc_indices(), True, 'v1')
+    sf = L.build_compute_stats_fn(dims, R, T, np.float64)
+    return L.run_scan(cells0.astype(np.float64), K, mp.get_gf_params(np.float64),
+                      mp.get_kernels_weight_per_channel(np.float64), T, steps, R, rule, sf)[0]
+
+
+@pytest.mark.parametrize('N', [32, 64, 128])
+def test_fused_random_worlds_per_step(N):
+    """Random soups under a steep growth (s = 0.015) are chaotic: two float32 FFTs (scipy vs
+    numpy pocketfft) already differ by ~1e-4 after 32 steps.  The bound is therefore 1e-5 for
+    the first 8 steps, then: no further from the float64 oracle than 4x the float32 oracle is."""
+    rng = np.random.default_rng(N)
+    R = {32: 5., 64: 9., 128: 13.}[N]
+    cells0 = np.zeros((3, 1, N, N), np.float32)
+    for b in range(3):
+        side = N // 4 + b
+        o = (N - side) // 2
+        cells0[b, 0, o:o + side, o:o + side] = rng.random((side, side), dtype=np.float32)
+    (o_cells, o_field, o_pot, o_stats), res = _run_both(_bump_rule(R), cells0, R, 10., 32)
+    o64 = _oracle64_cells(_bump_rule(R), cells0, R, 10., 32)
+    g = res['hist'][0].cpu().numpy()
+    e32 = [float(np.abs(g[t] - o_cells[t]).max()) for t in range(32)]
+    e64 = [float(np.abs(g[t] - o64[t]).max()) for t in range(32)]
+    f64 = [float(np.abs(o_cells[t] - o64[t]).max()) for t in range(32)]
+    print(N, 'gpu-o32', max(e32[:8]), max(e32), 'gpu-o64', max(e64), 'o32-o64', max(f64))
+    assert max(e32[:8]) < STEP_TOL
+    for t in range(32):
+        assert e64[t] <= max(STEP_TOL, 4 * f64[t]), (t, e64[t], f64[t])
+
+
+def test_fused_stats_match_oracle():
+    meta, cells0, frame = load_golden('orbium')
+    (o_cells, o_field, o_pot, o_stats), res = _run_both(meta['kernels_params'], cells0, meta['R'], meta['T'], 64,
+                                                        hist=False)
+    from leniax_amd import engine
+    st = engine.stats_to_dict(res['stats'][0].cpu().numpy(), 1)
+    for k in engine.STAT_KEYS:
+        o = o_stats[k][:, 0]
+        g = st[k]
+        scale = np.maximum(1., np.abs(o))
+        err = float((np.abs(g - o) / scale).max())
+        print(f'stat {k:18s} max rel err {err:.2e}')
+        assert err < 2e-3, k
+    np.testing.assert_allclose(st['channel_mass'][:, 0], o_stats['channel_mass'][:, 0, 0], rtol=1e-5)
+    assert int(res['n'][0]) == int(o_stats['N'][0])
+    assert int(res['steps'][0]) == 64

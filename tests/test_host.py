@@ -1,0 +1,98 @@
+"""CPU tests of the host side: the C ABI library loads and exports what include/leniax_amd.h
+declares, and the host-side parts of leniax_amd (kernel build, codecs, config) agree with
+the oracle.  No device computation here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from oracle import codec
+from oracle import lenia as L
+
+HEADER = os.path.join(ROOT, 'include', 'leniax_amd.h')
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(lx_[a-z0-9_]+)\s*\(', text)))
+
+
+def test_header_declares_the_entry_points():
+    names = _declared()
+    for n in ['lx_prepare_spectra', 'lx_kernel_spectrum', 'lx_potential', 'lx_field', 'lx_state', 'lx_update',
+              'lx_stats', 'lx_check_heuristics', 'lx_run', 'lx_last_error']:
+        assert n in names
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    from leniax_amd import _lib
+    if not _lib.available():
+        pytest.skip('libleniax_amd.so not built (python -m leniax_amd.build)')
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    # every declared symbol has a ctypes signature in the binding
+    assert not [n for n in _declared() if n not in _lib.SIGNATURES]
+    L_ = _lib.lib()
+    assert L_.lx_version() >= 1
+    assert L_.lx_carry_bytes() % 4 == 0 and L_.lx_carry_bytes() >= 64
+
+
+def test_argument_errors_are_reported_without_a_device():
+    from leniax_amd import _lib
+    if not _lib.available():
+        pytest.skip('libleniax_amd.so not built')
+    r = _lib.lx_rule()
+    r.ndim = 4
+    with pytest.raises(_lib.LxError, match='ndim'):
+        _lib.call('lx_workspace_size', ctypes.byref(r), 1,
+                  ctypes.byref(ctypes.c_size_t()))
+    r.ndim = 2
+    r.dims[0] = r.dims[1] = 128
+    r.C = 1
+    r.K = 1
+    r.gf[0] = 9
+    with pytest.raises(_lib.LxError, match='gf'):
+        _lib.call('lx_workspace_size', ctypes.byref(r), 1, ctypes.byref(ctypes.c_size_t()))
+    r.gf[0] = 1
+    assert _lib.lib().lx_run_supported(ctypes.byref(r)) == 1
+    r.C = 2
+    r.K = 2
+    r.cin[1] = 1
+    assert _lib.lib().lx_run_supported(ctypes.byref(r)) == 0
+
+
+def test_circle_2d_matches_oracle():
+    from leniax_amd import kernels
+    for R, kp, kf, kfp in [(13, [1., [1.]], 'poly_quad', [4]), (12, [0.82, [1 / 6, 1., 0.]], 'gauss', [0.5]),
+                           (10, [1., [1., 0.5]], 'gauss_bump', [1.])]:
+        np.testing.assert_array_equal(kernels.circle_2d(R, kp, kf, kfp), L.circle_2d(R, kp, kf, kfp, np.float32))
+    assert list(kernels.circle_2d(5., [1., [1.]], 'poly_quad', [4]).shape) == [1, 10, 10]  # tests/test_kernels.py:12-20
+
+
+def test_rle_and_gzip_codecs():
+    from leniax_amd import loader
+    rng = np.random.default_rng(0)
+    a = loader.make_array_compressible(rng.random((2, 5, 7)).astype(np.float32))
+    np.testing.assert_allclose(loader.decompress_array(loader.compress_array(a)), a, atol=1e-7)
+    code = '7.MD6.qL$6.pKqEqFURpApBRAqQ$5.VqTrSsBrOpXpWpTpWpUpCrQ!'
+    np.testing.assert_array_equal(loader.decompress_array(code, 3), codec.rle_decode(code, 3))
+
+
+def test_config_migration_matches_oracle():
+    from leniax_amd import utils
+    cfg = {'world_params': {'nb_dims': 2, 'nb_channels': 1, 'R': 13, 'T': 10},
+           'render_params': {'size_power2': 7, 'pixel_size_power2': 2},
+           'kernels_params': {'k': [{'r': 1, 'b': '1,1/4', 'm': 0.15, 's': 0.015, 'h': 1, 'k_id': 0, 'q': 4,
+                                     'gf_id': 0, 'c_in': 0, 'c_out': 0}]}}
+    out = utils.get_container(cfg)
+    assert out['render_params']['world_size'] == [128, 128]
+    assert out['kernels_params'] == codec.update_config_v1_v2(cfg['kernels_params']['k'])
+    d = {'a': [{'b': 1.}]}
+    utils.set_param(d, 'a.0.b', 2.)
+    utils.set_param(d, 'c.1', 3.)
+    assert utils.get_param(d, 'a.0.b') == 2. and d['c'] == [None, 3.]
