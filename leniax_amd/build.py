@@ -11,7 +11,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 OUT = os.path.join(HERE, 'libleniax_amd.so')
-SOURCES = ['lx_api.hip', 'lx_fused2d.hip', 'lx_plain.hip']
+SOURCES = ['lx_api.hip', 'lx_fused2d.hip', 'lx_fused2d_n32.hip', 'lx_fused2d_n64.hip', 'lx_fused2d_n128.hip', 'lx_plain.hip']
 ARCH = os.environ.get('LENIAX_AMD_ARCH', 'gfx950')
 
 FLAGS = [
@@ -45,7 +45,7 @@ def build(verbose=False):
             sys.stderr.write(r.stderr)
         return obj
 
-    with ThreadPoolExecutor(max_workers=min(4, len(srcs))) as ex:
+    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(compile_one, srcs))
     tmp = OUT + '.tmp'
     cmd = [hipcc, f'--offload-arch={ARCH}', '-shared', '-fPIC', '-o', tmp] + objs

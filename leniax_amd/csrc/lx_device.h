@@ -223,61 +223,85 @@ constexpr int bitrev_n(int v, int n) {  // reverse the low log2(n) bits
   return r;
 }
 
-// Cross-lane DFT of size TPR over lanes t = lane % TPR, as a radix-2 network.
+// Cross-lane DFT of size TPR over the lanes of a group, as a radix-2 network.
 // forward (decimation in frequency): natural lane order in, X_s at lane bitrev(s) out.
 // inverse (decimation in time):      X_s at lane bitrev(s) in, natural order out, x TPR.
-template <int P, int TPR, bool INV, int H>
-LX_DEV void lane_network(float2* x, int t, int lane) {
-  if constexpr (H >= 1 && H < TPR) {
-    // twiddle W_{2H}^{t mod H} on the upper lane of each pair
-    const bool upper = (t & H) != 0;
-    const float sgn = upper ? -1.f : 1.f;
-    double sn, cs;
-    sincospi(double(t % H) / double(H), &sn, &cs);  // angle 2*pi*(t mod H)/(2H)
-    float2 w = upper ? make_float2((float)cs, INV ? (float)sn : (float)-sn) : make_float2(1.f, 0.f);
-    if constexpr (!INV) {
+// Stage H pairs lane t with t ^ H; the upper lane's twiddle W_{2H}^(t mod H) is read from
+// the length-L table ``tw``.  Lower lanes use the twiddle 1 (an exact multiply), so the
+// stage needs no lane selects.  SPREAD = true: for 8-lane groups whose bit 2 is lane
+// bit 3 (lanes {0..3, 8..11} + offset), so that t ^ 4 is one DPP row rotate.
+template <int H, bool SPREAD>
+LX_DEV float2 group_xor(float2 v, int lane) {
+  if constexpr (SPREAD && H == 4) return dpp2<0x128>(v);  // row_ror:8 == lane ^ 8
+  else return xor_fetch2<H>(v, lane);
+}
+
+template <int P, int TPR, int L, bool INV, int H, bool SPREAD>
+LX_DEV void lane_stage(float2* x, int t, int lane, const float2* tw) {
+  const bool upper = (t & H) != 0;
+  const float sgn = upper ? -1.f : 1.f;
+  float2 w = make_float2(1.f, 0.f);
+  if constexpr (H > 1) {
+    const float2 tv = tw[(t & (H - 1)) * (L / (2 * H))];
+    w = upper ? make_float2(tv.x, INV ? -tv.y : tv.y) : w;
+  }
+  if constexpr (!INV) {
 #pragma unroll
-      for (int k = 0; k < P; ++k) {
-        const float2 p = xor_fetch2<H>(x[k], lane);
-        const float2 v = make_float2(fmaf(sgn, x[k].x, p.x), fmaf(sgn, x[k].y, p.y));
-        x[k] = upper ? cmul(v, w) : v;
-      }
-      lane_network<P, TPR, INV, H / 2>(x, t, lane);
-    } else {
+    for (int k = 0; k < P; ++k) {
+      const float2 p = group_xor<H, SPREAD>(x[k], lane);
+      const float2 v = make_float2(fmaf(sgn, x[k].x, p.x), fmaf(sgn, x[k].y, p.y));
+      x[k] = (H > 1) ? cmul(v, w) : v;
+    }
+  } else {
 #pragma unroll
-      for (int k = 0; k < P; ++k) {
-        const float2 o = upper ? cmul(x[k], w) : x[k];
-        const float2 p = xor_fetch2<H>(o, lane);
-        x[k] = make_float2(fmaf(sgn, o.x, p.x), fmaf(sgn, o.y, p.y));
-      }
+    for (int k = 0; k < P; ++k) {
+      const float2 o = (H > 1) ? cmul(x[k], w) : x[k];
+      const float2 p = group_xor<H, SPREAD>(o, lane);
+      x[k] = make_float2(fmaf(sgn, o.x, p.x), fmaf(sgn, o.y, p.y));
     }
   }
 }
 
-template <int P, int TPR, int H>
-LX_DEV void lane_network_inv(float2* x, int t, int lane) {
-  if constexpr (H < TPR) {
-    lane_network<P, TPR, true, H>(x, t, lane);
-    lane_network_inv<P, TPR, H * 2>(x, t, lane);
+template <int P, int TPR, int L, int H, bool SPREAD>
+LX_DEV void lane_network_fwd(float2* x, int t, int lane, const float2* tw) {
+  if constexpr (H >= 1) {
+    lane_stage<P, TPR, L, false, H, SPREAD>(x, t, lane, tw);
+    lane_network_fwd<P, TPR, L, H / 2, SPREAD>(x, t, lane, tw);
   }
 }
 
-// Length-(TPR*P) FFT over TPR lanes, unnormalised.
+template <int P, int TPR, int L, int H, bool SPREAD>
+LX_DEV void lane_network_inv(float2* x, int t, int lane, const float2* tw) {
+  if constexpr (H < TPR) {
+    lane_stage<P, TPR, L, true, H, SPREAD>(x, t, lane, tw);
+    lane_network_inv<P, TPR, L, H * 2, SPREAD>(x, t, lane, tw);
+  }
+}
+
+// Length-(TPR*P) FFT over a group of TPR lanes, unnormalised.
 //  forward: in  x[m]  = data[TPR*m + t];  out x[k1] = DATA[k1 + P*bitrev(t)]
 //  inverse: in  x[k1] = DATA[k1 + P*bitrev(t)];  out x[m] = data[TPR*m + t]
 // ``tw`` = exp(-2*pi*i*j/(TPR*P)), j < TPR*P (LDS or global table).
-template <int P, int TPR, bool INV>
+template <int P, int TPR, bool INV, bool SPREAD = false>
 LX_DEV void fft_lanes(float2* x, int t, int lane, const float2* tw) {
   constexpr int L = TPR * P;
   if constexpr (!INV) {
     fft_reg<P, false>(x);
+    int j = 0;
 #pragma unroll
-    for (int k = 1; k < P; ++k) x[k] = cmul(x[k], tw[(t * k) & (L - 1)]);
-    lane_network<P, TPR, false, TPR / 2>(x, t, lane);
+    for (int k = 1; k < P; ++k) {
+      j += t;  // t * k < L
+      x[k] = cmul(x[k], tw[j]);
+    }
+    lane_network_fwd<P, TPR, L, TPR / 2, SPREAD>(x, t, lane, tw);
   } else {
-    lane_network_inv<P, TPR, 1>(x, t, lane);
+    lane_network_inv<P, TPR, L, 1, SPREAD>(x, t, lane, tw);
+    int j = 0;
 #pragma unroll
-    for (int k = 1; k < P; ++k) x[k] = cmulc(x[k], tw[(t * k) & (L - 1)]);
+    for (int k = 1; k < P; ++k) {
+      j += t;
+      x[k] = cmulc(x[k], tw[j]);
+    }
     fft_reg<P, true>(x);
   }
 }
@@ -309,7 +333,70 @@ LX_DEV float triangle01(float m, float s, float X) {
   return out;
 }
 
-// g(U); ``inv9s2`` = 1/(9 s^2) precomputed for poly_quad4.
+// Products and sums that must round separately (no FMA contraction), as leniax's
+// ``state + dt * field`` and ``w * g`` do.
+LX_DEV float mul_nc(float a, float b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+LX_DEV float add_nc(float a, float b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+LX_DEV float sub_nc(float a, float b) {
+#pragma clang fp contract(off)
+  return a - b;
+}
+
+// g(U) for a compile-time growth function id.
+template <int KIND>
+LX_DEV float growth_k(float m, float s, float U) {
+  if constexpr (KIND == LX_GF_POLY_QUAD4) {  // leniax/growth_functions.py:6-45
+    float d = U - m;
+    float o = 1.f - (d * d) / (9.f * (s * s));
+    o = fmaxf(0.f, o);
+    o = o * o;
+    o = o * o;
+    return 2.f * o - 1.f;
+  } else if constexpr (KIND == LX_GF_GAUSSIAN) {  // :48-82
+    float z = (U - m) / s;
+    return 2.f * __expf(-(z * z) / 2.f) - 1.f;
+  } else if constexpr (KIND == LX_GF_GAUSSIAN_TARGET) {  // :85-119
+    float z = (U - m) / s;
+    return __expf(-(z * z) / 2.f);
+  } else if constexpr (KIND == LX_GF_STEP) {  // :122-160
+    return fabsf(U - m) <= s ? 1.f : -1.f;
+  } else if constexpr (KIND == LX_GF_STAIRCASE) {  // :163-201
+    return 2.f * staircase01(m, s, U) - 1.f;
+  } else if constexpr (KIND == LX_GF_TRIANGLE) {  // :204-243
+    return 2.f * triangle01(m, s, U) - 1.f;
+  } else {  // identity :246-253
+    return U;
+  }
+}
+
+// g(U) with per-rule reciprocals precomputed: inv_s = 1/s, inv_9s2 = 1/(9 s^2)
+// (one rounding away from leniax's divisions).
+template <int KIND>
+LX_DEV float growth_pre(float m, float s, float inv_s, float inv_9s2, float U) {
+  if constexpr (KIND == LX_GF_POLY_QUAD4) {
+    float d = U - m;
+    float o = fmaxf(0.f, 1.f - (d * d) * inv_9s2);
+    o = o * o;
+    o = o * o;
+    return 2.f * o - 1.f;
+  } else if constexpr (KIND == LX_GF_GAUSSIAN) {
+    float z = (U - m) * inv_s;
+    return 2.f * __expf(-0.5f * (z * z)) - 1.f;
+  } else if constexpr (KIND == LX_GF_GAUSSIAN_TARGET) {
+    float z = (U - m) * inv_s;
+    return __expf(-0.5f * (z * z));
+  } else {
+    return growth_k<KIND>(m, s, U);
+  }
+}
+
+// g(U) by runtime id (plain path).
 LX_DEV float growth(int kind, float m, float s, float U) {
   switch (kind) {
     case LX_GF_POLY_QUAD4: {  // leniax/growth_functions.py:6-45

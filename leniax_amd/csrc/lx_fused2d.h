@@ -1,0 +1,461 @@
+// Fused persistent run for 2-D N x N one-channel one-kernel worlds (N = 32, 64, 128).
+//
+// One workgroup owns one world at a time and steps it through max_steps Lenia
+// updates (leniax/core.py:13-49 with get_potential_fft, get_field, get_state)
+// plus the per-step statistics (leniax/statistics.py:35-126) and stop heuristics
+// (leniax/statistics.py:134-205), then takes the next world from a device-side
+// queue.  The state never leaves the chip between steps.
+//
+// Lane groups of TPR lanes own one length-N transform (P = N / TPR points per
+// lane, fft_lanes: in-register FFT + a radix-2 network across the group through
+// DPP).  Thread (r, t) -- r = tid / TPR, t = tid % TPR -- holds rows 2r and 2r+1
+// of the state at columns TPR*m + t.  A step:
+//   rows fwd  two real rows packed as one complex row -> Z (LDS)
+//   cols fwd  unpack Z into the half spectrum column c = r (columns 0 and N/2
+//             are real and share one transform), FFT, x kernel spectrum
+//   cols inv  inverse FFT -> Y (LDS)
+//   rows inv  repack rows 2r, 2r+1 from their half spectra, inverse FFT -> U;
+//             growth, weighted field, clip update, statistics sums in registers.
+// LDS: one complex buffer of N*N/2 (+padding) elements reused by both transposes.
+#pragma once
+#include "lx_internal.h"
+
+namespace lx_fused {
+
+template <int N>
+struct F2 {
+  static constexpr int TPR = N >= 128 ? 8 : 4;   // lanes per transform
+  static constexpr int P = N / TPR;              // complex points per lane
+  static constexpr int HALF = N / 2;
+  static constexpr int NT = HALF * TPR;          // one lane group per packed row / column
+  static constexpr int NW = (NT + 63) / 64;      // waves
+  static constexpr int ZRS = TPR * (P + 1);      // packed-row spectrum stride (blocks of P padded by one)
+  static constexpr int YRS = HALF + 2;           // column-spectrum row stride
+  static constexpr int ZSZ = HALF * ZRS;
+  static constexpr int YSZ = N * YRS;
+  static constexpr int BUF = ZSZ > YSZ ? ZSZ : YSZ;
+  static constexpr int NRED = 12;
+  LX_DEV static int zidx(int hp, int k) { return hp * ZRS + (k / P) * (P + 1) + (k % P); }
+  LX_DEV static int ycol(int c) { return c + (c >= N / 4 ? 1 : 0); }
+};
+
+struct FusedParams {
+  const float* cells0;
+  float* cells;
+  const float2* kh;
+  int64_t k_bstride;
+  const float* gf;
+  int64_t gf_bstride;
+  const float* w;
+  int64_t w_bstride;
+  const float* dt;
+  int64_t dt_bstride;
+  int64_t B;
+  int max_steps;
+  int stop_policy;
+  int min_steps;
+  float R2, R, dt_stats;
+  float* stats;
+  int32_t* n_out;
+  int32_t* steps_out;
+  float* cells_hist;
+  float* field_hist;
+  float* pot_hist;
+  int32_t* queue;
+  int state_fn;
+  int average;
+};
+
+// An opaque copy: stops the compiler from sharing twiddle loads between the four
+// transforms of a step (which would keep them live through the whole step).
+LX_DEV int fresh(int v) {
+  __asm__ volatile("" : "+v"(v));
+  return v;
+}
+
+LX_DEV float maxnan(float a, float b) { return (a != a) ? a : fmaxf(a, b); }
+
+LX_DEV float state_update(int fn, float a, float dt, float G) {
+  // leniax/core.py:245-319; products and sums round separately as in the reference
+  if (fn == LX_STATE_V2) return add_nc(mul_nc(a, sub_nc(1.f, dt)), mul_nc(dt, G));
+  const float v = add_nc(a, mul_nc(dt, G));
+  if (fn == LX_STATE_SIMPLE) return v;
+  return v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
+}
+
+// value held by group member t ^ (TPR - 1)
+template <int TPR>
+LX_DEV float2 group_mirror(float2 v) {
+  if constexpr (TPR == 4) return dpp2<LX_QP(3, 2, 1, 0)>(v);
+  else return dpp2<0x128>(dpp2<LX_QP(3, 2, 1, 0)>(v));  // spread group: lane ^ 3, then lane ^ 8
+}
+
+// lane (within the wave) of group member t, given any lane of the group
+template <int TPR>
+LX_DEV int group_lane(int lane, int t) {
+  if constexpr (TPR == 4) return (lane & ~3) | t;
+  else return (lane & ~11) | (t & 3) | ((t & 4) << 1);
+}
+
+template <int N, int GF, bool HIST>
+__global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
+  using G = F2<N>;
+  constexpr int P = G::P, TPR = G::TPR, NT = G::NT, NW = G::NW, HALF = G::HALF;
+  constexpr int KS = HALF + 1;  // half-spectrum row length
+  constexpr bool SPREAD = (TPR == 8);
+  __shared__ float2 buf[G::BUF];
+  __shared__ float2 tw[N];
+  __shared__ float red[NW][G::NRED];
+  __shared__ int ctrl[4];
+  __shared__ lx_carry carry;  // statistics + heuristics carry of the current world (thread 0)
+
+  const int tid = threadIdx.x;
+  // lane group of a transform: TPR = 4: 4 consecutive lanes; TPR = 8 (SPREAD): lanes
+  // {0..3, 8..11} + 4*(r & 1) + 16*(r >> 1), so that t ^ 4 is lane ^ 8 (one DPP rotate)
+  const int t0 = SPREAD ? ((tid & 3) | (((tid >> 3) & 1) << 2)) : (tid % TPR);
+  const int r0 = SPREAD ? (((tid >> 2) & 1) | ((tid >> 4) << 1)) : (tid / TPR);
+  const int s0 = bitrev_n(t0, TPR);
+  const int lane = tid & 63, wave = tid >> 6;
+  const float inv_n2 = 1.f / float(N * N);
+  const int64_t cells_per_world = int64_t(N) * N;
+
+  for (int j = tid; j < N; j += NT) {
+    double sn, cs;
+    sincospi(2.0 * j / N, &sn, &cs);
+    tw[j] = make_float2((float)cs, (float)-sn);
+  }
+
+  float a0[P], a1[P];
+  float2 x[P];
+
+  for (;;) {
+    if (tid == 0) ctrl[0] = atomicAdd(p.queue, 1);
+    __syncthreads();
+    const int64_t world = ctrl[0];
+    if (world >= p.B) break;
+
+    const float* A0 = p.cells0 + world * cells_per_world;
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      a0[m] = A0[(2 * r0) * N + TPR * m + t0];
+      a1[m] = A0[(2 * r0 + 1) * N + TPR * m + t0];
+    }
+    const float2* kh = p.kh + world * p.k_bstride;
+    const float gm = p.gf[world * p.gf_bstride + 0];
+    const float gs = p.gf[world * p.gf_bstride + 1];
+    const float wk = p.w[world * p.w_bstride];
+    const float inv_wden = 1.f / (p.average ? wk : 1.f);
+    const float inv_s = 1.f / gs, inv_9s2 = 1.f / (9.f * (gs * gs));
+    const float dtw = p.dt[world * p.dt_bstride];
+    if (tid == 0) {
+      for (int i = 0; i < 3; ++i) { carry.shift[i] = 0; carry.centroid[i] = 0.f; }
+      carry.angle = 0.f;
+      carry.t = 0;
+      carry.cont = 1;
+      carry.n = 0;
+      ctrl[2] = 0;
+      ctrl[3] = 0;
+    }
+    __syncthreads();
+
+    int t = 0;
+    while (t < p.max_steps) {
+      // The lane indices are laundered through an empty asm each step so that the
+      // per-lane LDS/global addresses of the transposes are recomputed inside the
+      // loop instead of being hoisted out of it and pinned in registers.
+      int r = r0, q = t0, s = s0;
+      __asm__ volatile("" : "+v"(r), "+v"(q), "+v"(s));
+      const int sh0 = ctrl[2], sh1 = ctrl[3];
+
+      // ---- rows forward: packed row r = A[2r] + i A[2r+1]
+#pragma unroll
+      for (int m = 0; m < P; ++m) x[m] = make_float2(a0[m], a1[m]);
+      fft_lanes<P, TPR, false, SPREAD>(x, fresh(q), lane, tw);  // x[k1] = Z_r[k1 + P s]
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) buf[r * G::ZRS + s * (P + 1) + k1] = x[k1];
+      __syncthreads();
+
+      // ---- columns forward: c = r, rows h = TPR m + q; unpack the packed rows:
+      //      row 2h' = (za + conj zb)/2, row 2h'+1 = (za - conj zb)/2i; column 0 packs
+      //      the real columns 0 and N/2: (Re/Im za, Re/Im zb)
+      {
+        const int c = r;
+        const bool partb = (q & 1) != 0;
+        const float sig = partb ? -1.f : 1.f;
+        const float2 cm = partb ? make_float2(0.f, -0.5f) : make_float2(0.5f, 0.f);
+        const int ia = G::zidx(q >> 1, c);
+        const int ib = G::zidx(q >> 1, c == 0 ? HALF : N - c);
+        if (wave != 0) {
+#pragma unroll
+          for (int m = 0; m < P; ++m) {
+            const float2 za = buf[ia + (TPR / 2) * m * G::ZRS];
+            const float2 zb = buf[ib + (TPR / 2) * m * G::ZRS];
+            x[m] = cmul(make_float2(fmaf(sig, zb.x, za.x), fmaf(-sig, zb.y, za.y)), cm);
+            if ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+          }
+        } else {
+          const bool c0 = (c == 0);
+#pragma unroll
+          for (int m = 0; m < P; ++m) {
+            const float2 za = buf[ia + (TPR / 2) * m * G::ZRS];
+            const float2 zb = buf[ib + (TPR / 2) * m * G::ZRS];
+            const float2 v = cmul(make_float2(fmaf(sig, zb.x, za.x), fmaf(-sig, zb.y, za.y)), cm);
+            x[m] = c0 ? make_float2(partb ? za.y : za.x, partb ? zb.y : zb.x) : v;
+            if ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+      }
+      fft_lanes<P, TPR, false, SPREAD>(x, fresh(q), lane, tw);  // x[k1] = F[kh = k1 + P s][c]
+
+      // ---- multiply by the half spectrum; column 0 carries columns 0 and N/2:
+      //      Q[f] = X[f] a[f] + conj(X[-f]) b[f], a = (K0 + KN)/2, b = (K0 - KN)/2
+      {
+        const float2* kcol = kh + r;
+        if (wave != 0) {
+#pragma unroll
+          for (int k1 = 0; k1 < P; ++k1) {
+            x[k1] = cmul(x[k1], kcol[(k1 + P * s) * KS]);
+            if ((k1 & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+          }
+        } else {
+          const bool c0 = (r == 0);
+          {
+            // X[-f] for k1 = 0 sits in block (TPR - s) mod TPR, register 0
+            const int src = group_lane<TPR>(lane, bitrev_n((TPR - s) & (TPR - 1), TPR));
+            const float2 pm = make_float2(__shfl(x[0].x, src, 64), __shfl(x[0].y, src, 64));
+            const int row = P * s;
+            const float2 kv = kcol[row * KS];
+            const float2 kn = kh[row * KS + HALF];
+            if (c0) {
+              const float2 a = cscale(cadd(kv, kn), 0.5f), b = cscale(csub(kv, kn), 0.5f);
+              x[0] = cadd(cmul(x[0], a), cmul(cconj(pm), b));
+            } else {
+              x[0] = cmul(x[0], kv);
+            }
+          }
+#pragma unroll
+          for (int k1 = 1; k1 <= P / 2; ++k1) {
+            // X[-f] for k1 >= 1 sits in lane t ^ (TPR-1), register P - k1
+            const int k2 = P - k1;
+            const float2 p1 = group_mirror<TPR>(x[k2]);
+            const float2 p2 = group_mirror<TPR>(x[k1]);
+            const int row1 = k1 + P * s, row2 = k2 + P * s;
+            const float2 kv1 = kcol[row1 * KS], kn1 = kh[row1 * KS + HALF];
+            const float2 kv2 = kcol[row2 * KS], kn2 = kh[row2 * KS + HALF];
+            float2 q1, q2;
+            if (c0) {
+              q1 = cadd(cmul(x[k1], cscale(cadd(kv1, kn1), 0.5f)), cmul(cconj(p1), cscale(csub(kv1, kn1), 0.5f)));
+              q2 = cadd(cmul(x[k2], cscale(cadd(kv2, kn2), 0.5f)), cmul(cconj(p2), cscale(csub(kv2, kn2), 0.5f)));
+            } else {
+              q1 = cmul(x[k1], kv1);
+              q2 = cmul(x[k2], kv2);
+            }
+            x[k1] = q1;
+            if (k2 != k1) x[k2] = q2;
+          }
+        }
+      }
+
+      // ---- columns inverse -> Y
+      fft_lanes<P, TPR, true, SPREAD>(x, fresh(q), lane, tw);  // x[m] = column c at row h = TPR m + q (x N)
+      __syncthreads();  // every column gather of Z is done before Y overwrites it
+#pragma unroll
+      for (int m = 0; m < P; ++m) buf[(TPR * m + q) * G::YRS + G::ycol(r)] = x[m];
+      __syncthreads();
+
+      // ---- rows inverse: repack rows 2r, 2r+1 from their half spectra, k = k1 + P s:
+      //      k < N/2 read directly, k > N/2 through conj(Y[N-k]); k = 0 and N/2 are real
+      {
+        const bool mirror = s >= TPR / 2;
+        const float sg = mirror ? -1.f : 1.f;
+        const int dir = mirror ? -1 : 1;
+        const int kbase = mirror ? N - P * s : P * s;
+        const int rowa = (2 * r) * G::YRS;
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) {
+          int col = kbase + dir * k1;
+          if (k1 == 0 && (s == 0 || s == TPR / 2)) col = 0;
+          const int ia = rowa + G::ycol(col);
+          const float2 ya = buf[ia];
+          const float2 yb = buf[ia + G::YRS];
+          float2 z = make_float2(fmaf(-sg, yb.y, ya.x), fmaf(sg, ya.y, yb.x));
+          if (k1 == 0) {
+            const bool d0 = s == 0, dn = s == TPR / 2;
+            z.x = d0 ? ya.x : (dn ? ya.y : z.x);
+            z.y = d0 ? yb.x : (dn ? yb.y : z.y);
+          }
+          x[k1] = z;
+          if ((k1 & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      fft_lanes<P, TPR, true, SPREAD>(x, fresh(q), lane, tw);  // x[m] = (U[2r][TPR m+q], U[2r+1][TPR m+q]) * N^2
+
+      // ---- growth, field, update, statistics sums
+      float sA = 0.f, sG = 0.f;
+      uint32_t nA = 0, nG = 0, nU = 0;  // per-lane counts
+      float sAy0 = 0.f, sAy1 = 0.f, sGy0 = 0.f, sGy1 = 0.f, sAyy0 = 0.f, sAyy1 = 0.f;
+      const int64_t hist_off = (int64_t(t) * p.B + world) * cells_per_world;
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const int col = TPR * m + q;
+        const float xc = float(((col - sh1) & (N - 1)) - N / 2);
+#pragma unroll
+        for (int part = 0; part < 2; ++part) {
+          const int row = 2 * r + part;
+          const float U = (part ? x[m].y : x[m].x) * inv_n2;
+          const float a = part ? a1[m] : a0[m];
+          const float g = growth_pre<GF>(gm, gs, inv_s, inv_9s2, U);
+          const float Gf = mul_nc(wk, g) * inv_wden;
+          const float gp = maxnan(Gf, 0.f);
+          const float yc = float(((row - sh0) & (N - 1)) - N / 2);
+          sA += a;
+          sG += gp;
+          nA += (a > LX_EPS) ? 1u : 0u;
+          nG += (gp > LX_EPS) ? 1u : 0u;
+          nU += (U > LX_EPS) ? 1u : 0u;
+          sAy0 = fmaf(a, yc, sAy0);
+          sAy1 = fmaf(a, xc, sAy1);
+          sGy0 = fmaf(gp, yc, sGy0);
+          sGy1 = fmaf(gp, xc, sGy1);
+          sAyy0 = fmaf(a * yc, yc, sAyy0);
+          sAyy1 = fmaf(a * xc, xc, sAyy1);
+          if constexpr (HIST) {
+            if (p.cells_hist) p.cells_hist[hist_off + row * N + col] = a;
+            if (p.field_hist) p.field_hist[hist_off + row * N + col] = Gf;
+            if (p.pot_hist) p.pot_hist[hist_off + row * N + col] = U;
+          }
+          const float an = state_update(p.state_fn, a, dtw, Gf);
+          if (part) a1[m] = an; else a0[m] = an;
+        }
+        if ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      }
+      const float vals[8] = {sA, sG, sAy0, sAy1, sGy0, sGy1, sAyy0, sAyy1};
+      constexpr int slot[8] = {0, 1, 5, 6, 7, 8, 9, 10};
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const float v = wave_sum(vals[i]);
+        if (lane == 0) red[wave][slot[i]] = v;
+      }
+      {
+        const uint32_t cnt[3] = {nA, nG, nU};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          uint32_t v = cnt[i];
+#pragma unroll
+          for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+          if (lane == 0) red[wave][2 + i] = float(v);
+        }
+      }
+      __syncthreads();
+      if (tid == 0) {
+        lx_sums sm;
+        float tot[11];
+#pragma unroll
+        for (int i = 0; i < 11; ++i) {
+          float acc = 0.f;
+#pragma unroll
+          for (int wv = 0; wv < NW; ++wv) acc += red[wv][i];
+          tot[i] = acc;
+        }
+        sm.sA = tot[0]; sm.sG = tot[1]; sm.nA = tot[2]; sm.nG = tot[3]; sm.nU = tot[4];
+        sm.sAy[0] = tot[5]; sm.sAy[1] = tot[6]; sm.sAy[2] = 0.f;
+        sm.sGy[0] = tot[7]; sm.sGy[1] = tot[8]; sm.sGy[2] = 0.f;
+        sm.sAyy[0] = tot[9]; sm.sAyy[1] = tot[10]; sm.sAyy[2] = 0.f;
+        sm.sAc[0] = tot[0];
+        float* out = p.stats ? p.stats + (world * p.max_steps + t) * (LX_NSTAT_BASE + 1) : nullptr;
+        const int32_t dims[3] = {N, N, 1};
+        const int stop = lx_stats_finish(sm, carry, 2, dims, 1, p.R2, p.R, p.dt_stats, p.stop_policy, p.min_steps,
+                                         float(N * N), out);
+        ctrl[1] = stop;
+        ctrl[2] = carry.shift[0];
+        ctrl[3] = carry.shift[1];
+      }
+      __syncthreads();
+      ++t;
+      if (ctrl[1]) break;
+    }
+
+    float* A1 = p.cells + world * cells_per_world;
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      A1[(2 * r0) * N + TPR * m + t0] = a0[m];
+      A1[(2 * r0 + 1) * N + TPR * m + t0] = a1[m];
+    }
+    if (tid == 0) {
+      if (p.n_out) p.n_out[world] = carry.n;
+      if (p.steps_out) p.steps_out[world] = t;
+    }
+    __syncthreads();
+  }
+}
+
+template <int UNUSED>
+__global__ void zero_i32(int32_t* p) { *p = 0; }
+
+template <int N, int GF, bool HIST>
+int launch_k(const FusedParams& p, int64_t B, hipStream_t stream) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return lx_fail(LX_EDEVICE, "hipGetDevice failed");
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return lx_fail(LX_EDEVICE, "hipDeviceGetAttribute failed");
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused2d_kernel<N, GF, HIST>, F2<N>::NT, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    return lx_fail(LX_EDEVICE, "fused kernel does not fit on a compute unit");
+  int64_t grid = int64_t(cus) * per_cu;
+  if (grid > B) grid = B;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(zero_i32<0>, dim3(1), dim3(1), 0, stream, p.queue);
+  hipLaunchKernelGGL((fused2d_kernel<N, GF, HIST>), dim3((unsigned)grid), dim3(F2<N>::NT), 0, stream, p);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return lx_fail(LX_EDEVICE, "fused launch failed: %s", hipGetErrorString(e));
+  return LX_OK;
+}
+
+template <int N, bool HIST>
+int launch_gf(const FusedParams& p, int64_t B, int gf, hipStream_t stream) {
+  switch (gf) {
+    case LX_GF_POLY_QUAD4: return launch_k<N, LX_GF_POLY_QUAD4, HIST>(p, B, stream);
+    case LX_GF_GAUSSIAN: return launch_k<N, LX_GF_GAUSSIAN, HIST>(p, B, stream);
+    case LX_GF_GAUSSIAN_TARGET: return launch_k<N, LX_GF_GAUSSIAN_TARGET, HIST>(p, B, stream);
+    case LX_GF_STEP: return launch_k<N, LX_GF_STEP, HIST>(p, B, stream);
+    case LX_GF_STAIRCASE: return launch_k<N, LX_GF_STAIRCASE, HIST>(p, B, stream);
+    case LX_GF_TRIANGLE: return launch_k<N, LX_GF_TRIANGLE, HIST>(p, B, stream);
+    default: return launch_k<N, LX_GF_IDENTITY, HIST>(p, B, stream);
+  }
+}
+
+template <int N>
+int launch(const lx_rule* rule, const lx_run_args* a, hipStream_t stream) {
+  FusedParams p;
+  p.cells0 = a->cells0_d;
+  p.cells = a->cells_d;
+  p.kh = static_cast<const float2*>(a->K_half_d);
+  p.k_bstride = a->k_bstride;
+  p.gf = a->gf_d;
+  p.gf_bstride = a->gf_bstride;
+  p.w = a->w_d;
+  p.w_bstride = a->w_bstride;
+  p.dt = a->dt_d;
+  p.dt_bstride = a->dt_bstride;
+  p.B = a->B;
+  p.max_steps = a->max_steps;
+  p.stop_policy = a->stop_policy;
+  p.min_steps = a->min_steps;
+  p.R2 = (float)((double)a->R * (double)a->R);
+  p.R = a->R;
+  p.dt_stats = (float)(1.0 / (double)a->T);
+  p.stats = a->stats_d;
+  p.n_out = a->n_d;
+  p.steps_out = a->steps_d;
+  p.cells_hist = a->cells_hist_d;
+  p.field_hist = a->field_hist_d;
+  p.pot_hist = a->potential_hist_d;
+  p.queue = a->queue_d;
+  p.state_fn = rule->state_fn;
+  p.average = rule->average;
+  const bool hist = a->cells_hist_d || a->field_hist_d || a->potential_hist_d;
+  return hist ? launch_gf<N, true>(p, a->B, rule->gf[0], stream) : launch_gf<N, false>(p, a->B, rule->gf[0], stream);
+}
+
+}  // namespace lx_fused

@@ -1,0 +1,6 @@
+// Fused persistent run, 128 x 128 worlds (all growth functions, with and without history output).
+#include "lx_fused2d.h"
+
+int lx_fused2d_launch_128(const lx_rule* rule, const lx_run_args* a, hipStream_t stream) {
+  return lx_fused::launch<128>(rule, a, stream);
+}

@@ -51,10 +51,10 @@ __global__ void field_kernel(RuleDev rd, const float* __restrict__ pot, int64_t 
     for (int c = 0; c < rd.C; ++c) {
       float acc = 0.f, den = 0.f;
       for (int k = 0; k < rd.K; ++k) {
-        acc = __fadd_rn(acc, __fmul_rn(pw[c * rd.K + k], g[k]));
-        den = __fadd_rn(den, pw[c * rd.K + k]);
+        acc = add_nc(acc, mul_nc(pw[c * rd.K + k], g[k]));
+        den = add_nc(den, pw[c * rd.K + k]);
       }
-      field[(b * rd.C + c) * n + cell] = rd.average ? __fdiv_rn(acc, den) : acc;
+      field[(b * rd.C + c) * n + cell] = rd.average ? (acc / den) : acc;
     }
   }
 }
@@ -67,9 +67,9 @@ __global__ void state_kernel(int fn, const float* __restrict__ A, const float* _
     const float a = A[i], g = G[i];
     float v;
     if (fn == LX_STATE_V2) {
-      v = __fadd_rn(__fmul_rn(a, __fsub_rn(1.f, d)), __fmul_rn(d, g));
+      v = add_nc(mul_nc(a, sub_nc(1.f, d)), mul_nc(d, g));
     } else {
-      v = __fadd_rn(a, __fmul_rn(d, g));
+      v = add_nc(a, mul_nc(d, g));
       if (fn == LX_STATE_V1) v = v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
     }
     out[i] = v;
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
   float2 x[P];
   for (int k = 0; k < a.K; ++k) {
     const float wk = pw[c * a.K + k];
-    den = __fadd_rn(den, wk);
+    den = add_nc(den, wk);
     int owner = 0;
     for (int cc = a.C - 1; cc >= 0; --cc)
       if (pw[cc * a.K + k] != 0.f) owner = cc;
@@ -370,8 +370,8 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
         po0[W + TPR * m + t] = U1;
       }
       if (a.update) {
-        G0[m] = __fadd_rn(G0[m], __fmul_rn(wk, growth(a.gfk[k], gm, gs, U0)));
-        G1[m] = __fadd_rn(G1[m], __fmul_rn(wk, growth(a.gfk[k], gm, gs, U1)));
+        G0[m] = add_nc(G0[m], mul_nc(wk, growth(a.gfk[k], gm, gs, U0)));
+        G1[m] = add_nc(G1[m], mul_nc(wk, growth(a.gfk[k], gm, gs, U1)));
       }
     }
   }
@@ -382,16 +382,16 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
   for (int m = 0; m < P; ++m) {
     const int col = TPR * m + t;
     float g0 = G0[m], g1 = G1[m];
-    if (a.average) { g0 = __fdiv_rn(g0, den); g1 = __fdiv_rn(g1, den); }
+    if (a.average) { g0 = (g0 / den); g1 = (g1 / den); }
     if (a.field) { a.field[off + col] = g0; a.field[off + W + col] = g1; }
     const float v0 = a.A[off + col], v1 = a.A[off + W + col];
     float n0, n1;
     if (a.state_fn == LX_STATE_V2) {
-      n0 = __fadd_rn(__fmul_rn(v0, __fsub_rn(1.f, dtw)), __fmul_rn(dtw, g0));
-      n1 = __fadd_rn(__fmul_rn(v1, __fsub_rn(1.f, dtw)), __fmul_rn(dtw, g1));
+      n0 = add_nc(mul_nc(v0, sub_nc(1.f, dtw)), mul_nc(dtw, g0));
+      n1 = add_nc(mul_nc(v1, sub_nc(1.f, dtw)), mul_nc(dtw, g1));
     } else {
-      n0 = __fadd_rn(v0, __fmul_rn(dtw, g0));
-      n1 = __fadd_rn(v1, __fmul_rn(dtw, g1));
+      n0 = add_nc(v0, mul_nc(dtw, g0));
+      n1 = add_nc(v1, mul_nc(dtw, g1));
       if (a.state_fn == LX_STATE_V1) {
         n0 = n0 < 0.f ? 0.f : (n0 > 1.f ? 1.f : n0);
         n1 = n1 < 0.f ? 0.f : (n1 > 1.f ? 1.f : n1);

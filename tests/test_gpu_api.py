@@ -200,6 +200,7 @@ def test_check_heuristics_known_answers():
 def test_run_scan_mem_optimized_matches_oracle(plain):
     """leniax/runner.py:167-215 with S solutions x N_init inits, per-solution spectra / T."""
     from leniax_amd import helpers, kernels, runner, statistics as S
+    STEPS = 10  # final states only: keep the chaotic horizon short (dt up to 0.2)
     N = 64 if not plain else 32  # plain: 2 channels forces the multi-pass path
     C = 1 if not plain else 2
     rng = np.random.default_rng(5)
@@ -220,14 +221,14 @@ def test_run_scan_mem_optimized_matches_oracle(plain):
     upd = helpers.build_update_fn(Ks[0].shape, mp, 'v1', True)
     sfn = S.build_compute_stats_fn({'R': 8., 'T': 10.}, {'world_size': [N, N]})
     stats, final = runner.run_scan_mem_optimized(None, cells0, torch.stack(Ks), torch.stack(gfs), torch.stack(ws), T,
-                                                 24, 8., upd, sfn)
-    assert tuple(stats['mass'].shape) == (2, 24, 2) and tuple(final.shape) == cells0.shape
+                                                 STEPS, 8., upd, sfn)
+    assert tuple(stats['mass'].shape) == (2, STEPS, 2) and tuple(final.shape) == cells0.shape
     rule_o = L.Rule(mpo.gf_slugs(), None, True, 'v1')
     so, fo = L.run_scan_mem_optimized(cells0, np.stack([k for k, _ in Ko_l]), np.stack([m.get_gf_params() for _, m in Ko_l]),
-                                      np.stack([m.get_kernels_weight_per_channel() for _, m in Ko_l]), T, 24, 8., rule_o,
+                                      np.stack([m.get_kernels_weight_per_channel() for _, m in Ko_l]), T, STEPS, 8., rule_o,
                                       L.build_compute_stats_fn([N, N], 8., 10., np.float32))
     err = float(np.abs(_np(final) - fo).max())
     print('mem_optimized final err', err)
-    assert err < 1e-3
+    assert err < 1e-4
     np.testing.assert_allclose(_np(stats['mass']), so['mass'], rtol=1e-3, atol=1e-5)
     np.testing.assert_array_equal(_np(stats['N']), so['N'])
