@@ -58,7 +58,7 @@ def flops_per_world_step(n_side=N, C=1, K=1):
     return 2.5 * n * math.log2(n) * (C + K) + 6 * K * nh + 8 * K * n + 2 * K * n + 3 * C * n
 
 
-def cpu_baseline(sample_worlds=24, steps=64):
+def cpu_baseline(sample_worlds=64, steps=384):
     """float32 oracle (scipy pocketfft, all host cores for the FFTs) on a bounded sample."""
     from oracle import lenia as L
     cores = min(len(os.sched_getaffinity(0)), 16)

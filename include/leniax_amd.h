@@ -114,6 +114,44 @@ int lx_stats(const lx_rule* rule, const float* cells_d, const float* field_d, co
 int lx_check_heuristics(const float* stats_d, int32_t T, int64_t B, int32_t C, float* cont_d, float* n_d,
                         void* stream);
 
+/* numpy.fft.rfft2 / irfft2 of n real 2-D arrays [n][H][W] <-> half spectra [n][H][W/2+1]
+ * (the transforms leniax/kernels.py:140 and leniax/core.py:80-92 apply; irfft2 scales by 1/(H W)).
+ * lx_irfft2 leaves X_d unchanged; work_d holds n * H * (W/2+1) complex64. */
+int lx_rfft2(int32_t ndim, const int32_t* dims, int64_t n, const float* x_d, void* X_d, void* stream);
+int lx_irfft2(int32_t ndim, const int32_t* dims, int64_t n, const void* X_d, float* x_d, void* work_d, void* stream);
+
+/* Reverse mode of one update step: what jax.grad computes through leniax/core.py:13-49 inside
+ * runner.make_pipeline_fn / make_gradient_fn (leniax/runner.py:363-444).  A step's adjoint is
+ * lx_adjoint_pointwise followed by lx_adjoint_potential.
+ *
+ * lx_adjoint_pointwise: state_d, field_d, potential_d are the step's input state and the field and
+ * potential lx_update wrote for it; lambda_d = dL/d(state out).  Writes the direct part of
+ * dL/d(state in) to lambda_out_d (may alias lambda_d) and dL/d(potential) [B][K][dims] to dpot_d,
+ * and ADDS dL/dgf, dL/dw, dL/ddt into grad_gf_d / grad_w_d / grad_dt_d (laid out and strided like
+ * gf_d / w_d / dt_d; each may be NULL).  ste != 0: v1's clip passes gradients straight through, as
+ * leniax/core.py:265-270 does; ste == 0: the derivative of the clip itself (1 inside (0, 1)).
+ * dfield_d / dpot_in_d (NULL: zero) are cotangents of the step's field and potential outputs. */
+int lx_adjoint_pointwise(const lx_rule* rule, int64_t B, const float* state_d, const float* field_d,
+                         const float* potential_d, const float* gf_d, int64_t gf_bstride, const float* w_d,
+                         int64_t w_bstride, const float* dt_d, int64_t dt_bstride, const float* lambda_d,
+                         const float* dfield_d, const float* dpot_in_d, int32_t ste, float* lambda_out_d,
+                         float* dpot_d, float* grad_gf_d, float* grad_w_d, float* grad_dt_d, void* stream);
+
+/* Hermitian extension of n half spectra [n][H][W/2+1] to full spectra [n][H][W], times scale
+ * (the full complex K layout of leniax/kernels.py:140 from a real array's half spectrum). */
+int lx_half_to_full(int32_t ndim, const int32_t* dims, int64_t n, const void* half_d, void* full_d, float scale,
+                    void* stream);
+
+/* Adjoint of the FFT potential (leniax/core.py:52-102): lambda_d [B][C][dims] += for each kernel k
+ * the correlation of dpot_d[b][k] with kernel k, i.e. irfft2(rfft2(dpot) * conj(K_half)).
+ * With state_spec_d = rfft2 of the step's input state ([B][C][H][W/2+1]), also ADDS
+ * rfft2(dpot[b][k]) * conj(state_spec[b][cin k]) into dspec_d [B][K][H][W/2+1] (per world, never
+ * shared); irfft2 of its sum over steps is dL/d(real kernel), and its Hermitian extension divided
+ * by H W is dL/dK for leniax's full complex K.  work_d: lx_workspace_size(rule, B) bytes. */
+int lx_adjoint_potential(const lx_rule* rule, int64_t B, const void* K_half_d, int64_t k_bstride,
+                         const float* dpot_d, const void* state_spec_d, void* dspec_d, float* lambda_d,
+                         void* work_d, void* stream);
+
 /* Fused persistent batched run (leniax/runner.py:167-215 run_scan_mem_optimized semantics):
  * one launch steps every world up to max_steps, world after world from a device queue,
  * each world resident in registers + LDS, statistics and stop policy evaluated every step.

@@ -1,0 +1,478 @@
+"""Reverse mode of the Lenia step on the GPU (what ``jax.grad`` gives leniax).
+
+leniax differentiates its step with ``jax.grad`` through ``core.update``
+(``leniax/core.py:13-49``), inside ``runner.make_pipeline_fn`` /
+``make_gradient_fn`` (``leniax/runner.py:363-444``); v1's clip is a
+straight-through estimator (``leniax/core.py:265-270``).  Here the reverse
+mode is hand-written HIP (``lx_adjoint_pointwise`` + ``lx_adjoint_potential``)
+wrapped as ``torch.autograd.Function``s:
+
+* ``update``: one differentiable step with the arguments and outputs of
+  ``core.update`` (gradients for state, K, gf_params, weights and dt);
+* ``run``: ``steps`` updates as ONE autograd node, checkpointed every
+  ``ckpt_every`` steps: the backward recomputes each segment from its
+  checkpoint (field and potential kept), then walks it in reverse, so memory is
+  ``steps / ckpt_every + ckpt_every`` states instead of ``steps``;
+* ``kernel_spectrum`` / ``ring_kernel``: differentiable ``K = fftn(kernel)`` and
+  the circle_2d kernel as a function of its ring weights ``b``
+  (``leniax/kernels.py:176-212``), so gradients reach ``b``.
+
+``ste=True`` is leniax's gradient; ``ste=False`` differentiates the true clip
+(what central finite differences of the forward measure).  Complex gradients
+use torch's convention (``dL/dRe + i dL/dIm``), the conjugate of JAX's.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib, core, engine
+from .kernel_functions import register as kf_register
+
+
+# ------------------------------------------------------------------ device transforms
+def _cdims(dims):
+    return (ctypes.c_int32 * len(dims))(*dims)
+
+
+def rfft2(x):
+    """``numpy.fft.rfft2`` over the last two axes of a float32 device tensor (HIP)."""
+    x = x.contiguous()
+    dims = list(x.shape[-2:])
+    n = int(x.numel() // (dims[0] * dims[1]))
+    out = torch.empty(tuple(x.shape[:-1]) + (dims[1] // 2 + 1,), dtype=torch.complex64, device=x.device)
+    _lib.call('lx_rfft2', 2, _cdims(dims), n, engine.ptr(x), engine.ptr(out), engine.stream_ptr())
+    return out
+
+
+def irfft2(X, W):
+    """``numpy.fft.irfft2`` (last axis of length ``W``) of a complex64 device tensor (HIP)."""
+    X = X.contiguous()
+    dims = [X.shape[-2], W]
+    n = int(X.numel() // (dims[0] * (W // 2 + 1)))
+    out = torch.empty(tuple(X.shape[:-1]) + (W,), dtype=torch.float32, device=X.device)
+    work = torch.empty_like(X)
+    _lib.call('lx_irfft2', 2, _cdims(dims), n, engine.ptr(X), engine.ptr(out), engine.ptr(work), engine.stream_ptr())
+    return out
+
+
+def half_to_full(X, W, scale=1.0):
+    """Hermitian extension of half spectra ``[..., H, W/2+1]`` to ``[..., H, W]``, times ``scale``."""
+    X = X.contiguous()
+    dims = [X.shape[-2], W]
+    n = int(X.numel() // (dims[0] * (W // 2 + 1)))
+    out = torch.empty(tuple(X.shape[:-1]) + (W,), dtype=torch.complex64, device=X.device)
+    _lib.call('lx_half_to_full', 2, _cdims(dims), n, engine.ptr(X), engine.ptr(out), float(scale),
+              engine.stream_ptr())
+    return out
+
+
+# ------------------------------------------------------------------ shared step context
+class _Ctx:
+    """Rule, spectra and parameter layouts of a step (one per autograd node)."""
+
+    def __init__(self, A, K, gf, w, dt, tc_indices, growth_fn_t, weighted_fn, state_slug):
+        N, C = A.shape[:2]
+        if A.dim() != 4:
+            raise NotImplementedError('the adjoint is built for 2-D worlds')
+        self.N, self.C = N, C
+        self.dims = tuple(A.shape[2:])
+        self.maxk = K.shape[2]
+        self.S = K.shape[0]
+        self.slots = core._slots(C, self.maxk, tc_indices)
+        self.Keff = len(self.slots)
+        self.rule = core._rule(self.dims, C, self.Keff, [s // self.maxk for s in self.slots], growth_fn_t,
+                               weighted_fn, state_slug)
+        self.Kh = core._spectra(self.rule, K, self.slots)
+        self.kbs = 0 if self.S == 1 else int(self.Kh[0].numel())
+        self.gf = core._pad_gf(gf, self.Keff)
+        self.gfs = core._per_world_stride(self.gf, N, 'gf')
+        self.w = w.contiguous()
+        self.ws = core._per_world_stride(self.w, N, 'weights')
+        self.dt, self.dts = core._dt_tensor(dt, N)
+
+    def step(self, A, out, field, pot):
+        _lib.call('lx_update', ctypes.byref(self.rule), engine.ptr(A), self.N, engine.ptr(self.Kh), self.kbs,
+                  engine.ptr(self.gf), self.gfs, engine.ptr(self.w), self.ws, engine.ptr(self.dt), self.dts,
+                  engine.ptr(out), engine.ptr(field), engine.ptr(pot),
+                  engine.ptr(core._workspace(self.rule, self.N)), engine.stream_ptr())
+
+    def accumulators(self, need_K):
+        acc = {'gf': torch.zeros_like(self.gf), 'w': torch.zeros_like(self.w), 'dt': torch.zeros_like(self.dt)}
+        if need_K:
+            H, W = self.dims
+            acc['dspec'] = torch.zeros((self.N, self.Keff, H, W // 2 + 1), dtype=torch.complex64, device=self.Kh.device)
+        return acc
+
+    def adjoint(self, A, field, pot, lam, acc, ste, dfield=None, dpot_in=None):
+        """dL/dA of one step given lam = dL/d(state out); parameter gradients added into acc."""
+        lam_out = torch.empty_like(A)
+        dpot = torch.empty_like(pot)
+        _lib.call('lx_adjoint_pointwise', ctypes.byref(self.rule), self.N, engine.ptr(A), engine.ptr(field),
+                  engine.ptr(pot), engine.ptr(self.gf), self.gfs, engine.ptr(self.w), self.ws, engine.ptr(self.dt),
+                  self.dts, engine.ptr(lam), engine.ptr(dfield), engine.ptr(dpot_in), int(bool(ste)),
+                  engine.ptr(lam_out), engine.ptr(dpot), engine.ptr(acc['gf']), engine.ptr(acc['w']),
+                  engine.ptr(acc['dt']), engine.stream_ptr())
+        dspec = acc.get('dspec')
+        FA = rfft2(A) if dspec is not None else None
+        _lib.call('lx_adjoint_potential', ctypes.byref(self.rule), self.N, engine.ptr(self.Kh), self.kbs,
+                  engine.ptr(dpot), engine.ptr(FA), engine.ptr(dspec), engine.ptr(lam_out),
+                  engine.ptr(core._workspace(self.rule, self.N)), engine.stream_ptr())
+        return lam_out
+
+    def grad_K(self, dspec, K):
+        """dL/dK (full complex spectra, torch convention) from the accumulated dspec."""
+        H, W = self.dims
+        if self.S == 1 and self.N > 1:
+            dspec = dspec.sum(dim=0, keepdim=True)
+        full = half_to_full(dspec, W, 1.0 / (H * W))  # [S, Keff, H, W]
+        g = torch.zeros_like(K)
+        g.view(self.S, -1, H, W)[:, self.slots] = full
+        return g
+
+    def grad_gf(self, g_pad, gf_orig):
+        return _grad_gf(g_pad, gf_orig)
+
+    def grad_dt(self, g, dt_orig):
+        return _grad_dt(g, dt_orig)
+
+
+def _grad_gf(g_pad, gf_orig):
+    """Gradient of the padded [.., K, 2] growth parameters in the caller's gf layout."""
+    g = torch.zeros_like(gf_orig)
+    if gf_orig.dim() == 1:
+        g.view(-1, 2)[:] = g_pad.view(-1, 2)
+    else:
+        P = min(2, gf_orig.shape[-1])
+        g[..., :P] = g_pad.view(gf_orig.shape[:-1] + (2,))[..., :P]
+    return g
+
+
+def _grad_dt(g, dt_orig):
+    if not isinstance(dt_orig, torch.Tensor):
+        return None
+    return g.reshape(dt_orig.shape).to(dt_orig.dtype)
+
+
+def _as_tensors(state, K, gf, w):
+    return (engine.as_device(state), engine.as_device(K, torch.complex64), engine.as_device(gf),
+            engine.as_device(w))
+
+
+# ------------------------------------------------------------------ one step
+class _Step(torch.autograd.Function):
+
+    @staticmethod
+    def forward(ctx, state, K, gf, w, dt, meta):
+        tc_indices, growth_fn_t, weighted_fn, slug, ste = meta
+        A, K_, gf_, w_ = _as_tensors(state, K, gf, w)
+        sc = _Ctx(A, K_, gf_, w_, dt, tc_indices, growth_fn_t, weighted_fn, slug)
+        new = torch.empty_like(A)
+        field = torch.empty_like(A)
+        pot = torch.empty((sc.N, sc.Keff) + sc.dims, dtype=torch.float32, device=A.device)
+        sc.step(A, new, field, pot)
+        ctx.sc, ctx.ste = sc, ste
+        ctx.dt_orig = dt
+        ctx.save_for_backward(A, K_, gf_, field, pot)
+        return new, field, pot
+
+    @staticmethod
+    def backward(ctx, g_new, g_field, g_pot):
+        A, K, gf, field, pot = ctx.saved_tensors
+        sc = ctx.sc
+        need = ctx.needs_input_grad
+        lam = g_new.contiguous() if g_new is not None else torch.zeros_like(A)
+        acc = sc.accumulators(need[1])
+        dA = sc.adjoint(A, field, pot, lam, acc, ctx.ste,
+                        g_field.contiguous() if g_field is not None else None,
+                        g_pot.contiguous() if g_pot is not None else None)
+        return (dA if need[0] else None,
+                sc.grad_K(acc['dspec'], K) if need[1] else None,
+                sc.grad_gf(acc['gf'], gf) if need[2] else None,
+                acc['w'] if need[3] else None,
+                sc.grad_dt(acc['dt'], ctx.dt_orig) if need[4] else None,
+                None)
+
+
+def update(state, K, gf_params, kernels_weight_per_channel, dt, tc_indices=None, growth_fn_t=('gaussian',),
+           weighted_fn=None, state_slug='v1', ste=True):
+    """Differentiable ``core.update`` (leniax/core.py:13-49): ``(state', field, potential)``."""
+    if weighted_fn is None:
+        weighted_fn = core.weighted_mean
+    return _Step.apply(state, K, gf_params, kernels_weight_per_channel, dt,
+                       (tc_indices, tuple(growth_fn_t), weighted_fn, state_slug, ste))
+
+
+# ------------------------------------------------------------------ checkpointed run
+class _Run(torch.autograd.Function):
+
+    @staticmethod
+    def forward(ctx, state0, K, gf, w, dt, meta):
+        tc_indices, growth_fn_t, weighted_fn, slug, ste, steps, ckpt = meta
+        A, K_, gf_, w_ = _as_tensors(state0, K, gf, w)
+        sc = _Ctx(A, K_, gf_, w_, dt, tc_indices, growth_fn_t, weighted_fn, slug)
+        ckpts = [A]
+        bufs = [torch.empty_like(A), torch.empty_like(A)]
+        cur = A
+        for t in range(steps):
+            if t > 0 and t % ckpt == 0:
+                ckpts.append(cur.clone())
+            nxt = bufs[t % 2]
+            sc.step(cur, nxt, None, None)
+            cur = nxt
+        out = cur.clone() if steps > 0 else A.clone()
+        ctx.sc, ctx.ste, ctx.steps, ctx.ckpt, ctx.ckpts, ctx.dt_orig = sc, ste, steps, ckpt, ckpts, dt
+        ctx.save_for_backward(K_, gf_)
+        return out
+
+    @staticmethod
+    def backward(ctx, g_out):
+        K, gf = ctx.saved_tensors
+        sc, steps, c = ctx.sc, ctx.steps, ctx.ckpt
+        need = ctx.needs_input_grad
+        acc = sc.accumulators(need[1])
+        lam = g_out.contiguous().clone()
+        if steps > 0:
+            A0 = ctx.ckpts[0]
+            seg_A = torch.empty((c,) + tuple(A0.shape), dtype=torch.float32, device=A0.device)
+            seg_G = torch.empty_like(seg_A)
+            seg_U = torch.empty((c, sc.N, sc.Keff) + sc.dims, dtype=torch.float32, device=A0.device)
+            nxt = torch.empty_like(A0)
+            for s in range((steps - 1) // c, -1, -1):
+                t0, t1 = s * c, min(s * c + c, steps)
+                seg_A[0].copy_(ctx.ckpts[s])
+                for i in range(t1 - t0):  # recompute the segment, keeping field and potential
+                    out = seg_A[i + 1] if i + 1 < t1 - t0 else nxt
+                    sc.step(seg_A[i], out, seg_G[i], seg_U[i])
+                for i in range(t1 - t0 - 1, -1, -1):
+                    lam = sc.adjoint(seg_A[i], seg_G[i], seg_U[i], lam, acc, ctx.ste)
+        ctx.ckpts = None
+        return (lam if need[0] else None,
+                sc.grad_K(acc['dspec'], K) if need[1] else None,
+                sc.grad_gf(acc['gf'], gf) if need[2] else None,
+                acc['w'] if need[3] else None,
+                sc.grad_dt(acc['dt'], ctx.dt_orig) if need[4] else None,
+                None)
+
+
+def run(state0, K, gf_params, kernels_weight_per_channel, dt, steps, tc_indices=None, growth_fn_t=('gaussian',),
+        weighted_fn=None, state_slug='v1', ste=True, ckpt_every=16):
+    """Final state after ``steps`` updates, differentiable with checkpointed recompute."""
+    if weighted_fn is None:
+        weighted_fn = core.weighted_mean
+    if ckpt_every < 1:
+        raise ValueError('ckpt_every must be >= 1')
+    return _Run.apply(state0, K, gf_params, kernels_weight_per_channel, dt,
+                      (tc_indices, tuple(growth_fn_t), weighted_fn, state_slug, ste, int(steps),
+                       int(min(ckpt_every, max(steps, 1)))))
+
+
+# ------------------------------------------------------------------ kernels
+class _Spectrum(torch.autograd.Function):
+
+    @staticmethod
+    def forward(ctx, kernel):
+        k = kernel.detach().to(torch.float32).contiguous()
+        H, W = k.shape[-2:]
+        n = int(k.numel() // (H * W))
+        out = torch.empty(k.shape, dtype=torch.complex64, device=k.device)
+        work = torch.empty((max(n, 1) * H * (W // 2 + 1),), dtype=torch.complex64, device=k.device)
+        _lib.call('lx_kernel_spectrum', 2, _cdims([H, W]), n, engine.ptr(k), engine.ptr(out), engine.ptr(work),
+                  engine.stream_ptr())
+        return out
+
+    @staticmethod
+    def backward(ctx, gK):
+        # kernel real: dL/dk = Re(H W ifftn(gK)) = H W irfft2 of gK's Hermitian part
+        gK = gK.contiguous()
+        H, W = gK.shape[-2:]
+        n = int(gK.numel() // (H * W))
+        rule = engine.make_rule([H, W], 1, [0], [0])
+        half = torch.empty((n, H, W // 2 + 1), dtype=torch.complex64, device=gK.device)
+        slot = (ctypes.c_int32 * 1)(0)
+        _lib.call('lx_prepare_spectra', ctypes.byref(rule), engine.ptr(gK), n, 1, slot, engine.ptr(half),
+                  engine.stream_ptr())
+        return (irfft2(half, W) * float(H * W)).reshape(gK.shape)
+
+
+def kernel_spectrum(kernel):
+    """Differentiable ``fftn`` of real kernels already padded and fftshift-ed, ``[..., H, W]``."""
+    return _Spectrum.apply(kernel)
+
+
+def ring_basis(R, r, nb_rings, kf_slug, kf_params, world_size):
+    """Per-ring cores of circle_2d (leniax/kernels.py:176-212), padded to the world and
+    fftshift-ed: ``raw = sum_i b_i basis_i`` before the normalisation.  float32 like leniax."""
+    f32 = np.float32
+    rad = math.ceil(r * R)
+    coords = np.indices([rad * 2, rad * 2]).astype(np.int32)
+    centered = (coords - rad).astype(f32) / f32(r * R)
+    dist = np.sqrt(np.sum(centered * centered, axis=0))
+    bd = f32(nb_rings) * dist
+    ring = np.minimum(np.floor(bd).astype(np.int32), nb_rings - 1)
+    core_k = (dist < 1).astype(f32) * kf_register[kf_slug](kf_params, np.fmod(bd, f32(1)))
+    pads = []
+    for ws, ks in zip(world_size, core_k.shape):
+        p = ws - ks
+        pads.append((p // 2, p // 2) if p % 2 == 0 else (p // 2, p // 2 + 1))
+    basis = np.stack([np.fft.fftshift(np.pad(core_k * (ring == i), pads)) for i in range(nb_rings)])
+    return engine.as_device(basis.astype(f32))
+
+
+def ring_kernel(b, basis):
+    """circle_2d's kernel, padded and fftshift-ed, as a differentiable function of ``b``."""
+    raw = torch.einsum('i,ihw->hw', b, basis)
+    return raw / raw.sum()
+
+
+# ------------------------------------------------------------------ the update as a module
+class Step:
+    """A differentiable leniax update with trainable leaves, the role the flax ``Step``
+    module plays in leniax's gradient example (examples/optimization.py:40-55).
+
+    ``variables['params']`` may hold any of ``K`` (full spectra ``[1, C, max_k, H, W]``),
+    ``b`` (ring weights of a single circle_2d kernel, through ``ring_basis``),
+    ``gf_params``, ``kernels_weight_per_channel`` and ``dt``; missing leaves come from
+    the constructor.  ``__call__`` has flax's ``apply`` signature; ``run`` is the
+    checkpointed multi-step node ``runner.make_pipeline_fn`` uses when only the final
+    state is kept.
+    """
+
+    def __init__(self, K=None, gf_params=None, kernels_weight_per_channel=None, basis=None, tc_indices=None,
+                 growth_fn_t=('gaussian',), weighted_fn=None, state_slug='v1', ste=True):
+        self.K, self.gf_params, self.w, self.basis = K, gf_params, kernels_weight_per_channel, basis
+        self.tc_indices, self.growth_fn_t = tc_indices, tuple(growth_fn_t)
+        self.weighted_fn = weighted_fn if weighted_fn is not None else core.weighted_mean
+        self.state_slug, self.ste = state_slug, ste
+
+    def _leaves(self, variables, dt):
+        p = variables.get('params', {})
+        K = p.get('K', self.K)
+        if 'b' in p:
+            K = kernel_spectrum(ring_kernel(p['b'], self.basis))[None, None, None]
+        return (K, p.get('gf_params', self.gf_params), p.get('kernels_weight_per_channel', self.w),
+                p.get('dt', dt))
+
+    def __call__(self, variables, rng_key, cells, dt=None):
+        K, gf, w, dt = self._leaves(variables, dt)
+        return update(cells, K, gf, w, dt, self.tc_indices, self.growth_fn_t, self.weighted_fn, self.state_slug,
+                      self.ste)
+
+    def run(self, variables, state0, dt, steps, ckpt_every=16):
+        K, gf, w, dt = self._leaves(variables, dt)
+        return run(state0, K, gf, w, dt, steps, self.tc_indices, self.growth_fn_t, self.weighted_fn,
+                   self.state_slug, self.ste, ckpt_every)
+
+
+# ------------------------------------------------------------------ the step's parts
+class _Potential(torch.autograd.Function):
+    """get_potential_fft (leniax/core.py:52-102) with its adjoint (lx_adjoint_potential)."""
+
+    @staticmethod
+    def forward(ctx, state, K, tc_indices):
+        A = engine.as_device(state)
+        K_ = engine.as_device(K, torch.complex64)
+        pot = core.get_potential_fft(A, K_, tc_indices)
+        ctx.tc = tc_indices
+        ctx.save_for_backward(A, K_)
+        return pot
+
+    @staticmethod
+    def backward(ctx, g):
+        A, K = ctx.saved_tensors
+        N, C = A.shape[:2]
+        gf = torch.zeros((K.shape[2] * C, 2), device=A.device)
+        w = torch.zeros((C, K.shape[2] * C), device=A.device)
+        sc = _Ctx(A, K, gf, w, 0.0, ctx.tc, None, None, 'v1')
+        acc = sc.accumulators(ctx.needs_input_grad[1])
+        lam = torch.zeros_like(A)
+        dspec = acc.get('dspec')
+        FA = rfft2(A) if dspec is not None else None
+        _lib.call('lx_adjoint_potential', ctypes.byref(sc.rule), N, engine.ptr(sc.Kh), sc.kbs,
+                  engine.ptr(g.contiguous()), engine.ptr(FA), engine.ptr(dspec), engine.ptr(lam),
+                  engine.ptr(core._workspace(sc.rule, N)), engine.stream_ptr())
+        return lam, (sc.grad_K(dspec, K) if dspec is not None else None), None
+
+
+class _Field(torch.autograd.Function):
+    """get_field (leniax/core.py:163-199) with its adjoint (lx_adjoint_pointwise, lambda = 0)."""
+
+    @staticmethod
+    def forward(ctx, potential, gf_params, w, growth_fn_t, weighted_fn):
+        pot = engine.as_device(potential)
+        gf = engine.as_device(gf_params)
+        w_ = engine.as_device(w)
+        field = core.get_field(pot, gf, w_, growth_fn_t, weighted_fn)
+        ctx.meta = (growth_fn_t, weighted_fn)
+        ctx.save_for_backward(pot, gf, w_, field)
+        return field
+
+    @staticmethod
+    def backward(ctx, g):
+        pot, gf, w, field = ctx.saved_tensors
+        growth_fn_t, weighted_fn = ctx.meta
+        N, Keff = pot.shape[:2]
+        C = w.shape[-2]
+        rule = core._rule(pot.shape[2:], C, Keff, [0] * Keff, growth_fn_t, weighted_fn, 'simple')
+        gf2 = core._pad_gf(gf, Keff)
+        ggf, gw = torch.zeros_like(gf2), torch.zeros_like(w)
+        zero_dt = torch.zeros(1, device=pot.device)
+        dpot = torch.empty_like(pot)
+        scratch = torch.empty_like(field)
+        _lib.call('lx_adjoint_pointwise', ctypes.byref(rule), N, engine.ptr(field), engine.ptr(field),
+                  engine.ptr(pot), engine.ptr(gf2), core._per_world_stride(gf2, N, 'gf'), engine.ptr(w),
+                  core._per_world_stride(w, N, 'weights'), engine.ptr(zero_dt), 0, engine.ptr(field),
+                  engine.ptr(g.contiguous()), None, 1, engine.ptr(scratch), engine.ptr(dpot), engine.ptr(ggf),
+                  engine.ptr(gw), None, engine.stream_ptr())
+        return dpot, _grad_gf(ggf, gf), gw, None, None
+
+
+class _State(torch.autograd.Function):
+    """get_state v1 / v2 / simple (leniax/core.py:245-319) with its adjoint: the pointwise
+    adjoint of a rule whose kernels are the channels themselves (identity growth, unit weights)."""
+
+    @staticmethod
+    def forward(ctx, state, field, dt, slug, ste):
+        A = engine.as_device(state)
+        G = engine.as_device(field)
+        out = core._state(slug, A, G, dt)
+        ctx.meta = (slug, ste, dt)
+        ctx.save_for_backward(A, G)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        A, G = ctx.saved_tensors
+        slug, ste, dt = ctx.meta
+        shape = A.shape
+        N = shape[0] if A.dim() >= 3 else 1
+        C = shape[1] if A.dim() >= 3 else 1
+        cells = A.numel() // (N * C)
+        dt_t, dts = core._dt_tensor(dt, N)
+        Cr = min(C, _lib.LX_MAX_C)
+        if Cr != C:  # fold channels into cells: the adjoint of the state is channel-independent
+            C, cells = 1, cells * C
+        rule = engine.make_rule([1, cells], C, list(range(C)), [_lib.GROWTH_IDS['identity']] * C,
+                                _lib.STATE_FN[slug], True)
+        gf = torch.zeros((C, 2), device=A.device)
+        w = torch.eye(C, device=A.device)
+        dA = torch.empty_like(A)
+        dG = torch.empty_like(A)
+        gdt = torch.zeros_like(dt_t)
+        _lib.call('lx_adjoint_pointwise', ctypes.byref(rule), N, engine.ptr(A), engine.ptr(G), engine.ptr(G),
+                  engine.ptr(gf), 0, engine.ptr(w), 0, engine.ptr(dt_t), dts, engine.ptr(g.contiguous()), None, None,
+                  int(bool(ste)), engine.ptr(dA), engine.ptr(dG), None, None, engine.ptr(gdt), engine.stream_ptr())
+        return dA, dG, _grad_dt(gdt, dt), None, None
+
+
+def get_potential_fft(state, K, tc_indices=None):
+    return _Potential.apply(state, K, tc_indices)
+
+
+def get_field(potential, gf_params, w, growth_fn_t, weighted_fn):
+    return _Field.apply(potential, gf_params, w, growth_fn_t, weighted_fn)
+
+
+def get_state(slug, state, field, dt, ste=True):
+    return _State.apply(state, field, dt, slug, ste)

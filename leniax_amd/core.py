@@ -21,6 +21,11 @@ from . import _lib, engine
 from .growth_functions import GrowthFn, growth_id
 
 
+def _needs_grad(*xs):
+    """True when torch autograd must record the call (a tensor argument requires grad)."""
+    return torch.is_grad_enabled() and any(isinstance(x, torch.Tensor) and x.requires_grad for x in xs)
+
+
 # ------------------------------------------------------------------ caches
 _SPEC_CACHE: 'OrderedDict' = OrderedDict()
 _WORK = {}
@@ -85,6 +90,9 @@ def get_potential_fft(state, K, tc_indices=None, channel_first=True):
     ``[N, C * max_k, *world]`` and filtered by ``tc_indices``."""
     if not channel_first:
         raise NotImplementedError('leniax_amd keeps channel-first arrays ([N, C, *world])')
+    if _needs_grad(state, K):
+        from . import autograd
+        return autograd.get_potential_fft(state, K, tc_indices)
     state = engine.as_device(state)
     K = engine.as_device(K, torch.complex64)
     N, C = state.shape[:2]
@@ -109,6 +117,9 @@ def get_potential(state, K, padding, tc_indices=None, channel_first=True):
 # ------------------------------------------------------------------ field
 def get_field(potential, gf_params, kernels_weight_per_channel, growth_fn_t, weighted_fn):
     """leniax/core.py:163-199: growth per potential channel, then weighted_fn over kernels."""
+    if _needs_grad(potential, gf_params, kernels_weight_per_channel):
+        from . import autograd
+        return autograd.get_field(potential, gf_params, kernels_weight_per_channel, growth_fn_t, weighted_fn)
     pot = engine.as_device(potential)
     gf = engine.as_device(gf_params)
     w = engine.as_device(kernels_weight_per_channel)
@@ -186,19 +197,27 @@ def _state(fn, state, field, dt):
     return out
 
 
+def _state_fn(slug, state, field, dt):
+    if _needs_grad(state, field, dt):
+        from . import autograd
+        return autograd.get_state(slug, state, field, dt)
+    return _state(slug, state, field, dt)
+
+
 def get_state(rng_key, state, field, dt):
-    """leniax/core.py:245-272: ``clip(state + dt * field, 0, 1)`` (forward of the STE clip)."""
-    return _state('v1', state, field, dt)
+    """leniax/core.py:245-272: ``clip(state + dt * field, 0, 1)``; its gradient is the
+    straight-through one (the clip's backward is the identity, leniax/core.py:265-270)."""
+    return _state_fn('v1', state, field, dt)
 
 
 def get_state_v2(rng_key, state, field, dt):
     """leniax/core.py:275-297: ``state (1 - dt) + dt field``."""
-    return _state('v2', state, field, dt)
+    return _state_fn('v2', state, field, dt)
 
 
 def get_state_simple(rng_key, state, field, dt):
     """leniax/core.py:300-319: ``state + dt field``."""
-    return _state('simple', state, field, dt)
+    return _state_fn('simple', state, field, dt)
 
 
 register = {
@@ -236,6 +255,10 @@ def update(rng_key, state, K, gf_params, kernels_weight_per_channel, dt, get_pot
         field = get_field_fn(potential, gf_params, kernels_weight_per_channel)
         return get_state_fn(rng_key, state, field, dt), field, potential
     tc_indices, growth_fn_t, weighted_fn, slug = parts
+    if _needs_grad(state, K, gf_params, kernels_weight_per_channel, dt):
+        from . import autograd
+        return autograd.update(state, K, gf_params, kernels_weight_per_channel, dt, tc_indices, growth_fn_t,
+                               weighted_fn, slug)
     return fused_update(state, K, gf_params, kernels_weight_per_channel, dt, tc_indices, growth_fn_t, weighted_fn, slug)
 
 

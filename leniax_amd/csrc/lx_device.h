@@ -450,9 +450,20 @@ LX_DEV void growth_grad(int kind, float m, float s, float U, float& dU, float& d
       ds = amp * e * z * z / s;
       return;
     }
+    case LX_GF_TRIANGLE: {  // 2 (U - (m - s)) / s on [m - s, m), 2 (U - (m + s)) / (-s) on [m, m + s]
+      const float d = U - m;
+      if (U >= m - s && U < m) {
+        dU = 2.f / s; dm = -dU; ds = -2.f * d / (s * s);
+      } else if (U >= m && U <= m + s) {
+        dU = -2.f / s; dm = -dU; ds = 2.f * d / (s * s);
+      } else {
+        dU = dm = ds = 0.f;
+      }
+      return;
+    }
     case LX_GF_IDENTITY:
       dU = 1.f; dm = ds = 0.f; return;
-    default:  // piecewise constant / linear pieces: gradient of the smooth parts only
+    default:  // step, staircase: piecewise constant
       dU = dm = ds = 0.f; return;
   }
 }

@@ -233,3 +233,77 @@ def run(rng_key, cells, K, gf_params, kernels_weight_per_channel, T, max_run_ite
 def normalize_fn(g):
     """leniax/runner.py:447-448."""
     return g / (torch.linalg.norm(g) + 1e-8)
+
+
+# ------------------------------------------------------------------ differentiable pipeline
+def make_pipeline_fn(max_run_iter: int, dt, apply_fn: Callable, loss_fn: Callable,
+                     keep_intermediary_data: bool = False, keep_all_timesteps: bool = True, ckpt_every: int = 16):
+    """leniax/runner.py:363-423: ``fn(rng_key, params, variables, state0, targets) -> (loss, rest)``.
+
+    ``apply_fn(variables, rng_key, state, dt=dt) -> (state, field, potential)`` plays the
+    flax ``apply``.  As in leniax's scan (leniax/runner.py:337-357) ``preds['cells'][t]``
+    is the state BEFORE step t when every timestep is kept, the final state otherwise.
+    When only the final state is kept and ``apply_fn`` has a ``run`` method
+    (``autograd.Step``), the whole scan is one autograd node with checkpointed recompute
+    every ``ckpt_every`` steps instead of ``max_run_iter`` stored states.
+    """
+
+    def fn(rng_key, params, variables, state0, targets):
+        all_vars = {'params': params, **(variables or {})}
+        if not keep_all_timesteps and hasattr(apply_fn, 'run'):
+            preds = {'cells': apply_fn.run(all_vars, state0, dt, max_run_iter, ckpt_every)}
+        else:
+            state = state0
+            ys = []
+            for _ in range(max_run_iter):
+                new, field, potential = apply_fn(all_vars, rng_key, state, dt=dt)
+                if keep_all_timesteps:
+                    ys.append({'cells': state, 'field': field, 'potential': potential}
+                              if keep_intermediary_data else {'cells': state})
+                state = new
+            if keep_all_timesteps:
+                preds = {k: torch.stack([y[k] for y in ys]) for k in ys[0]} if ys else {}
+            else:
+                preds = {'cells': state}
+        loss, *rest = loss_fn(rng_key, preds, targets)
+        return loss, rest
+
+    return fn
+
+
+def _tree_leaves(tree, prefix=()):
+    if isinstance(tree, dict):
+        out = []
+        for k in tree:
+            out += _tree_leaves(tree[k], prefix + (k,))
+        return out
+    return [(prefix, tree)]
+
+
+def _tree_set(tree, path, value):
+    if not path:
+        return value
+    return {k: (_tree_set(v, path[1:], value) if k == path[0] else v) for k, v in tree.items()}
+
+
+def make_gradient_fn(pipeline_fn: Callable, normalize: bool = True) -> Callable:
+    """leniax/runner.py:426-444: ``fn(...) -> ((loss, aux), grads)`` with grads w.r.t. ``params``
+    (same tree), each leaf divided by its norm + 1e-8 when ``normalize``."""
+
+    def fn(rng_key, params: Dict, variables: Dict, state0, targets):
+        leaves = _tree_leaves(params)
+        tracked = params
+        ins = []
+        for path, leaf in leaves:
+            t = leaf.detach().clone().requires_grad_(True)
+            ins.append(t)
+            tracked = _tree_set(tracked, path, t)
+        loss, aux = pipeline_fn(rng_key, tracked, variables, state0, targets)
+        gs = torch.autograd.grad(loss, ins, allow_unused=True)
+        grads = params
+        for (path, leaf), t, g in zip(leaves, ins, gs):
+            g = torch.zeros_like(t) if g is None else g
+            grads = _tree_set(grads, path, normalize_fn(g) if normalize else g)
+        return (loss.detach(), aux), grads
+
+    return fn

@@ -1,0 +1,308 @@
+"""GPU reverse mode (lx_adjoint_pointwise / lx_adjoint_potential) against the float64
+oracle adjoint, itself pinned by finite differences (tests/test_oracle_adjoint.py).
+
+Tolerances: one step is compared at 1e-4 of each gradient's largest entry (float32
+FFTs and growth derivatives up to 2/s); runs of many steps are bounded by the
+float32 envelope: |gpu - oracle64| <= 4 |oracle32 - oracle64| + 1e-4 max|oracle64|.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import adjoint as ADJ
+from oracle import lenia as L
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip('torch')
+
+
+def _dev(x, dtype=None):
+    t = torch.as_tensor(np.asarray(x))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda().contiguous()
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def _world(seed, N, C, H, W, patch):
+    rng = np.random.default_rng(seed)
+    A = np.zeros((N, C, H, W), np.float32)
+    y0, x0 = (H - patch) // 2, (W - patch) // 2
+    A[:, :, y0:y0 + patch, x0:x0 + patch] = rng.uniform(0, 1, (N, C, patch, patch))
+    return A
+
+
+def _asym_K(seed, C, maxk, H, W, R=5):
+    rng = np.random.default_rng(seed)
+    K = np.zeros((1, C, maxk, H, W), np.complex64)
+    for c in range(C):
+        for j in range(maxk):
+            k = np.zeros((H, W))
+            k[H // 2 - R:H // 2 + R, W // 2 - R:W // 2 + R + 1] = rng.uniform(0, 1, (2 * R, 2 * R + 1))
+            k /= k.sum()
+            K[0, c, j] = np.fft.fftn(np.fft.fftshift(k))
+    return K
+
+
+def test_device_rfft2_irfft2_half_to_full():
+    from leniax_amd import autograd as AG
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(3, 64, 32)).astype(np.float32)
+    X = AG.rfft2(_dev(x))
+    ref = np.fft.rfft2(x.astype(np.float64))
+    assert _rel(_np(X), ref) < 2e-6
+    back = AG.irfft2(X, 32)
+    assert np.abs(_np(back) - x).max() < 1e-5
+    full = AG.half_to_full(X, 32, 0.5)
+    assert _rel(_np(full), 0.5 * np.fft.fft2(x.astype(np.float64))) < 2e-6
+
+
+CASES = [('1c1k', 'v1', True, True), ('1c1k', 'v1', True, False), ('1c1k', 'v2', True, False),
+         ('1c1k', 'simple', False, False), ('2c3k', 'v1', True, True), ('2c3k', 'v1', False, False)]
+
+
+def _case(name):
+    if name == '1c1k':
+        return 1, ['gaussian'], np.array([[1.0]], np.float32), None, 1, np.array([[0.25, 0.05]], np.float32)
+    return (2, ['gaussian', 'poly_quad4', 'triangle'], np.array([[0.7, 0.0, 0.4], [0.3, 0.9, 0.5]], np.float32),
+            (0, 1, 2), 2, np.array([[0.25, 0.05], [0.22, 0.06], [0.27, 0.08]], np.float32))
+
+
+@pytest.mark.parametrize('name,state_fn,average,ste', CASES)
+def test_step_adjoint_matches_oracle(name, state_fn, average, ste):
+    from leniax_amd import autograd as AG
+    from leniax_amd import core
+    C, gfs, w, tc, maxk, gf = _case(name)
+    N, H, W = 2, 64, 32
+    A = _world(1, N, C, H, W, 20)
+    K = _asym_K(2, C, maxk, H, W)
+    if tc is not None:
+        K[0, 1, 1] = 0
+    dt = np.float32(0.1)
+    Rw = np.random.default_rng(3).normal(size=A.shape).astype(np.float32)
+    print(f'case {name} {state_fn} average={average} ste={ste}', flush=True)
+    tA, tK, tgf, tw = (_dev(A).requires_grad_(), _dev(K).requires_grad_(), _dev(gf).requires_grad_(),
+                       _dev(w).requires_grad_())
+    tdt = torch.tensor(0.1, device='cuda', requires_grad=True)
+    new, field, pot = AG.update(tA, tK, tgf, tw, tdt, tc, gfs, core.weighted_mean if average else core.weighted_sum,
+                                state_fn, ste)
+    (new * _dev(Rw)).sum().backward()
+    rule = L.Rule(gfs, tc, average, state_fn)
+    ref = ADJ.step_vjp(rule, A, K, gf, w, dt, Rw, ste)
+    errs = {'state': _rel(_np(tA.grad), ref['state']), 'K': _rel(_np(tK.grad), ref['K']),
+            'gf': _rel(_np(tgf.grad), ref['gf_params']), 'w': _rel(_np(tw.grad), ref['weights']),
+            'dt': abs(float(tdt.grad) - ref['dt']) / max(abs(ref['dt']), 1e-30)}
+    print('  rel errors', {k: f'{v:.2e}' for k, v in errs.items()})
+    for k, v in errs.items():
+        assert v < 1e-4, (k, v)
+
+
+def test_update_with_field_and_potential_cotangents():
+    """Losses on the field and the potential outputs (keep_intermediary_data pipelines)."""
+    from leniax_amd import autograd as AG
+    C, gfs, w, tc, maxk, gf = _case('2c3k')
+    N, H, W = 1, 32, 32
+    A = _world(4, N, C, H, W, 16)
+    K = _asym_K(5, C, maxk, H, W)
+    K[0, 1, 1] = 0
+    rng = np.random.default_rng(6)
+    r_new, r_field = rng.normal(size=A.shape).astype(np.float32), rng.normal(size=A.shape).astype(np.float32)
+    r_pot = rng.normal(size=(N, 3, H, W)).astype(np.float32)
+    tA, tgf = _dev(A).requires_grad_(), _dev(gf).requires_grad_()
+    new, field, pot = AG.update(tA, _dev(K), tgf, _dev(w), 0.1, tc, gfs)
+    ((new * _dev(r_new)).sum() + (field * _dev(r_field)).sum() + (pot * _dev(r_pot)).sum()).backward()
+
+    # float64: the field and potential cotangents enter as extra terms of the oracle's chain
+    rule = L.Rule(gfs, tc, True, 'v1')
+    A64 = A.astype(np.float64)
+    ref = ADJ.step_vjp(rule, A64, K, gf, w, 0.1, r_new)
+    U = L.get_potential_fft(A64, K.astype(np.complex128), tc)
+    den = w.sum(1)
+    extra_U = np.zeros_like(U)
+    extra_gf = np.zeros(gf.shape)
+    for i, s in enumerate(gfs):
+        du, dm, ds = ADJ.growth_grad(s, gf[i].astype(np.float64), U[:, i])
+        dg = sum(r_field[:, c] * w[c, i] / den[c] for c in range(C))
+        extra_U[:, i] = dg * du + r_pot[:, i]
+        extra_gf[i] = [np.sum(dg * dm), np.sum(dg * ds)]
+    dA_extra, _ = ADJ.potential_vjp(A64, K.astype(np.complex128), extra_U, tc)
+    assert _rel(_np(tA.grad), ref['state'] + dA_extra) < 1e-4
+    assert _rel(_np(tgf.grad), ref['gf_params'] + extra_gf) < 1e-4
+
+
+@pytest.mark.parametrize('ste,ckpt', [(True, 8), (False, 7), (True, 24)])
+def test_checkpointed_run_matches_oracle(ste, ckpt):
+    from leniax_amd import autograd as AG
+    N, H, W, steps = 1, 64, 64, 24
+    A = _world(7, N, 1, H, W, 24)
+    K = _asym_K(8, 1, 1, H, W, R=6)
+    gf, w, dt = np.array([[0.25, 0.05]], np.float32), np.array([[1.0]], np.float32), 0.1
+    Rw = np.random.default_rng(9).normal(size=A.shape).astype(np.float32)
+    print(f'run steps={steps} ckpt={ckpt} ste={ste}', flush=True)
+    tA, tK, tgf = _dev(A).requires_grad_(), _dev(K).requires_grad_(), _dev(gf).requires_grad_()
+    tdt = torch.tensor(dt, device='cuda', requires_grad=True)
+    final = AG.run(tA, tK, tgf, _dev(w), tdt, steps, ste=ste, ckpt_every=ckpt)
+    (final * _dev(Rw)).sum().backward()
+    rule = L.Rule(['gaussian'])
+    o64 = ADJ.run_vjp(rule, A, K, gf, w, dt, steps, Rw, ste)
+    o32 = ADJ.run_vjp(rule, A, K, gf, w, dt, steps, Rw, ste, f=np.float32)
+    assert np.abs(_np(final) - o64['final']).max() <= max(1e-5, 4 * np.abs(o32['final'] - o64['final']).max())
+    for name, got in (('state', tA.grad), ('K', tK.grad), ('gf_params', tgf.grad), ('dt', tdt.grad)):
+        g, r64, r32 = _np(got), np.asarray(o64[name]), np.asarray(o32[name])
+        err, env = np.abs(g - r64).max(), np.abs(r32 - r64).max()
+        print(f'  {name}: |gpu-o64| {err:.3e}  |o32-o64| {env:.3e}  max|o64| {np.abs(r64).max():.3e}')
+        assert err <= 4 * env + 1e-4 * np.abs(r64).max(), name
+
+
+def test_checkpoint_interval_does_not_change_the_state_gradient():
+    from leniax_amd import autograd as AG
+    A = _world(10, 2, 1, 64, 64, 24)
+    K = _asym_K(11, 1, 1, 64, 64)
+    gf, w = _dev(np.array([[0.25, 0.05]], np.float32)), _dev(np.array([[1.0]], np.float32))
+    Rw = _dev(np.random.default_rng(12).normal(size=A.shape).astype(np.float32))
+    grads = []
+    for ckpt in (1, 5, 16):
+        tA = _dev(A).requires_grad_()
+        (AG.run(tA, _dev(K), gf, w, 0.1, 16, ckpt_every=ckpt) * Rw).sum().backward()
+        grads.append(_np(tA.grad))
+    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+
+
+def _c3_setup(H=256, patch=40, R=13, b=(1.0,), seed=0):
+    rng = np.random.default_rng(seed)
+    A = np.zeros((1, 1, H, H), np.float32)
+    y0 = (H - patch) // 2
+    A[0, 0, y0:y0 + patch, y0:y0 + patch] = rng.uniform(0, 1, (patch, patch))
+    return A, R, list(b)
+
+
+def _c3_loss_np(A0):
+    target = np.roll(A0, (8, 8), axis=(-2, -1)).astype(np.float64)
+    return lambda final: (final - target) / final.size  # d/dfinal of mean((final - target)^2) / 2
+
+
+@pytest.mark.parametrize('steps,ste', [(16, False), (64, True)])
+def test_gradient_pipeline_c3(steps, ste):
+    """Gradient, single world 256x256: MSE(final, roll(A0, (8, 8))) after `steps` updates,
+    w.r.t. A0, mu, sigma, ring weights b and dt, checkpointed every 16 steps, through
+    runner.make_pipeline_fn / make_gradient_fn (normalize=False) and autograd.Step."""
+    from leniax_amd import autograd as AG
+    from leniax_amd import runner
+    H, kf, kfp = 256, 'gauss_bump', [1.0]
+    A0, R, b = _c3_setup(H, b=(1.0, 0.5))
+    gf = np.array([[0.15, 0.015]], np.float32)
+    w = np.array([[1.0]], np.float32)
+    dt = 0.1
+    basis = AG.ring_basis(R, 1.0, len(b), kf, kfp, (H, H))
+    step = AG.Step(kernels_weight_per_channel=_dev(w), basis=basis, ste=ste)
+
+    def loss_fn(rng_key, preds, targets):
+        d = preds['cells'] - targets['cells']
+        return 0.5 * (d * d).mean(), None
+
+    pipeline = runner.make_pipeline_fn(steps, dt, step, loss_fn, keep_all_timesteps=False, ckpt_every=16)
+    grad_fn = runner.make_gradient_fn(pipeline, normalize=False)
+    params = {'b': _dev(np.array(b, np.float32)), 'gf_params': _dev(gf), 'dt': torch.tensor(dt, device='cuda')}
+    tA = _dev(A0).requires_grad_()
+    (loss, _), grads = grad_fn(None, params, {}, tA, {'cells': _dev(np.roll(A0, (8, 8), axis=(-2, -1)))})
+    print(f'c3 steps={steps} ste={ste} loss {float(loss):.6e}', flush=True)
+
+    rule = L.Rule(['gaussian'])
+    res = {}
+    for f in (np.float64, np.float32):
+        kappa = ADJ.kernel_from_b(R, 1.0, b, kf, kfp, (H, H))
+        K = np.fft.fftn(kappa)[None, None, None]
+        o = ADJ.run_vjp(rule, A0, K, gf, w, dt, steps, _c3_loss_np(A0), ste, f=f)
+        o['b'] = ADJ.b_vjp(R, 1.0, b, kf, kfp, (H, H), ADJ.dkappa_from_dK(np.asarray(o['K'][0, 0, 0], np.complex128),
+                                                                          (H, H)))
+        res[f] = o
+    for name, got in (('gf_params', grads['gf_params']), ('dt', grads['dt']), ('b', grads['b'])):
+        g, r64, r32 = _np(got), np.asarray(res[np.float64][name]), np.asarray(res[np.float32][name])
+        err, env = np.abs(g - r64).max(), np.abs(r32 - r64).max()
+        print(f'  {name}: gpu {np.ravel(g)[:3]} o64 {np.ravel(r64)[:3]} |gpu-o64| {err:.3e} |o32-o64| {env:.3e}')
+        assert err <= 4 * env + 1e-3 * np.abs(r64).max(), name
+
+
+def test_gradient_api_nonzero_like_reference():
+    """tests/test_core_gradient.py:17-56 on the orbium fixture: gradients of
+    sum((update(...) - 0.5)^2) w.r.t. cells, K, gf_params and weights are non-zero,
+    and equal the oracle's (torch convention for K)."""
+    from conftest import load_golden
+    from leniax_amd import helpers
+    meta, cells0, _ = load_golden('orbium')
+    config = {
+        'world_params': {'nb_dims': 2, 'nb_channels': 1, 'R': meta['R'], 'T': meta['T'],
+                         'weighted_average': True, 'get_state_fn_slug': 'v1', 'scale': 1.},
+        'render_params': {'world_size': meta['world_size']},
+        'kernels_params': [dict(k) for k in meta['kernels_params']],
+        'run_params': {'cells': cells0[0], 'max_run_iter': 4},
+    }
+    cells, K, mapping = helpers.init(config)
+    gf = mapping.get_gf_params()
+    w = mapping.get_kernels_weight_per_channel()
+    update_fn = helpers.build_update_fn(K.shape, mapping, 'v1')
+    dt = 1.0 / meta['T']
+    grads = {}
+    for i, name in enumerate(('cells', 'K', 'gf', 'w')):
+        args = [cells.clone(), K.clone(), gf.clone(), w.clone()]
+        args[i].requires_grad_()
+        out, _, _ = update_fn(None, args[0], args[1], args[2], args[3], dt)
+        ((out - 0.5) ** 2).sum().backward()
+        grads[name] = args[i].grad
+        val = grads[name].real.sum() if name == 'K' else grads[name].sum()
+        assert float(val) != 0.0, name
+    rule = L.Rule([g for sub in mapping.cin_gfs for g in sub])
+    A = _np(cells).astype(np.float64)
+    Kn = _np(K)
+    ref_new = rule.update(A, Kn.astype(np.complex128), _np(gf).astype(np.float64), _np(w).astype(np.float64), dt)[0]
+    ref = ADJ.step_vjp(rule, A, Kn, _np(gf), _np(w), dt, 2 * (ref_new - 0.5))
+    for name, key in (('cells', 'state'), ('K', 'K'), ('gf', 'gf_params'), ('w', 'weights')):
+        err = _rel(_np(grads[name]), ref[key])
+        print(f'  {name}: rel err {err:.2e}')
+        assert err < 1e-4, name
+
+
+def test_field_and_state_gradients_like_reference():
+    """tests/test_core_gradient.py:58-186: get_field / weighted_mean and get_state v1 / v2
+    gradients are non-zero and match their closed forms."""
+    from leniax_amd import core
+    from leniax_amd.growth_functions import register as gfr
+    rng = np.random.default_rng(13)
+    pot = rng.uniform(0, 0.4, (1, 3, 16, 16)).astype(np.float32)
+    gf = np.array([[0.15, 0.02], [0.2, 0.05], [0.25, 0.04]], np.float32)
+    w = np.array([[0.5, 0.3, 0.0], [0.1, 0.0, 0.9]], np.float32)
+    tp, tgf, tw = _dev(pot).requires_grad_(), _dev(gf).requires_grad_(), _dev(w).requires_grad_()
+    field = core.get_field(tp, tgf, tw, (gfr['gaussian'], gfr['poly_quad4'], gfr['gaussian']), core.weighted_mean)
+    field.sum().backward()
+    rule = L.Rule(['gaussian', 'poly_quad4', 'gaussian'])
+    # get_field's gradient = the step's with dt folded out: use the oracle's chain on dG directly
+    g = np.stack([L.GF[s](gf[i].astype(np.float64), pot[:, i].astype(np.float64), np.float64)
+                  for i, s in enumerate(rule.gf_slugs)], 1)
+    den = w.sum(1)
+    for i, s in enumerate(rule.gf_slugs):
+        du, dm, ds = ADJ.growth_grad(s, gf[i].astype(np.float64), pot[:, i].astype(np.float64))
+        coef = sum(w[c, i] / den[c] for c in range(2))
+        assert _rel(_np(tp.grad)[:, i], coef * du) < 1e-5
+        assert abs(float(_np(tgf.grad)[i, 0]) - np.sum(coef * dm)) <= 1e-4 * max(1, abs(np.sum(coef * dm)))
+    G = np.einsum('ck,nkhw->nchw', w / den[:, None], g)
+    for c in range(2):
+        for k in range(3):
+            ref = np.sum((g[:, k] - G[:, c]) / den[c])
+            assert abs(float(_np(tw.grad)[c, k]) - ref) <= 1e-4 * max(1, abs(ref))
+    # get_state: straight-through v1, v2
+    A = rng.uniform(0, 1, (1, 2, 8, 8)).astype(np.float32)
+    Gf = rng.uniform(-1, 1, (1, 2, 8, 8)).astype(np.float32)
+    for fn, dA_ref in ((core.get_state, 1.0), (core.get_state_v2, 0.9)):
+        tA, tG = _dev(A).requires_grad_(), _dev(Gf).requires_grad_()
+        fn(None, tA, tG, 0.1).sum().backward()
+        assert np.allclose(_np(tA.grad), dA_ref, atol=1e-7) and np.allclose(_np(tG.grad), 0.1, atol=1e-7)
