@@ -76,6 +76,56 @@ def cpu_baseline(sample_worlds=64, steps=384):
             'cell_updates_per_s': ws * N * N}
 
 
+def adjoint_rate(n=256, steps=64, ckpt=16, reps=5):
+    """Gradient, single world n x n (rule above, ring weights b=[1]): loss = MSE(final,
+    roll(A0, (8, 8))) after `steps` updates; gradients w.r.t. A0, (m, s), b and dt through
+    the checkpointed adjoint.  Returns forward and forward+backward times."""
+    import torch
+    from leniax_amd import autograd as AG
+    from leniax_amd import runner
+    dev = torch.device('cuda', 0)
+    rng = np.random.default_rng(0)
+    a0 = np.zeros((1, 1, n, n), np.float32)
+    p0 = (n - 40) // 2
+    a0[0, 0, p0:p0 + 40, p0:p0 + 40] = rng.uniform(0, 1, (40, 40))
+    A0 = torch.as_tensor(a0).to(dev)
+    target = torch.roll(A0, (8, 8), dims=(-2, -1))
+    basis = AG.ring_basis(R, 1.0, 1, 'gauss_bump', [1.0], (n, n))
+    step = AG.Step(kernels_weight_per_channel=torch.ones(1, 1, device=dev), basis=basis)
+
+    def loss_fn(rng_key, preds, targets):
+        d = preds['cells'] - targets
+        return 0.5 * (d * d).mean(), None
+
+    pipeline = runner.make_pipeline_fn(steps, np.float32(1) / np.float32(T), step, loss_fn,
+                                       keep_all_timesteps=False, ckpt_every=ckpt)
+    grad_fn = runner.make_gradient_fn(pipeline, normalize=False, wrt_state0=True)
+    params = {'b': torch.ones(1, device=dev), 'gf_params': torch.tensor([[M, S]], device=dev),
+              'dt': torch.tensor(np.float32(1) / np.float32(T), device=dev)}
+
+    def fwd():
+        with torch.no_grad():
+            return pipeline(None, params, {}, A0, target)[0]
+
+    def fwd_bwd():
+        (loss, _), g = grad_fn(None, params, {}, A0, target)
+        return loss
+
+    out = {}
+    for name, fn in (('fwd_ms', fwd), ('fwd_bwd_ms', fwd_bwd)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        out[name] = (time.perf_counter() - t0) / reps * 1e3
+    out['ratio'] = out['fwd_bwd_ms'] / out['fwd_ms']
+    out['fwd_bwd_steps_per_s'] = steps / (out['fwd_bwd_ms'] / 1e3)
+    out['config'] = f'{n}x{n}, 1 world, {steps} steps, checkpoints every {ckpt}, gradients of A0, m, s, b, dt'
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -160,6 +210,10 @@ def main():
     plain_batched = plain_rate(B, 20)
     log(f'plain step: single world {plain_single:.0f} world-steps/s, batched {plain_batched / 1e6:.2f} M world-steps/s')
 
+    adj = adjoint_rate()
+    log(f"gradient 256x256, 64 steps, checkpoints every 16: forward {adj['fwd_ms']:.2f} ms, "
+        f"forward+backward {adj['fwd_bwd_ms']:.2f} ms ({adj['ratio']:.2f}x forward)")
+
     flops_per_launch = F * steps_run.sum()
     achieved = flops_per_launch / (launch_ms / 1e3)
     result = {
@@ -183,6 +237,7 @@ def main():
         'cell_updates_per_s': value * N * N,
         'fused_full_1024_steps': {'world_steps_per_s': full_ws, 'cell_updates_per_s': full_ws * N * N,
                                   'ms': full_ms},
+        'gradient_256x256_64_steps': adj,
         'plain_single_world_steps_per_s': plain_single,
         'plain_batched_world_steps_per_s': plain_batched,
         'fused_vs_plain_single_world': full_ws / plain_single,

@@ -36,34 +36,36 @@ def _cdims(dims):
     return (ctypes.c_int32 * len(dims))(*dims)
 
 
-def rfft2(x):
-    """``numpy.fft.rfft2`` over the last two axes of a float32 device tensor (HIP)."""
+def rfftn(x, nd=2):
+    """``numpy.fft.rfftn`` over the last ``nd`` (2 or 3) axes of a float32 device tensor (HIP)."""
     x = x.contiguous()
-    dims = list(x.shape[-2:])
-    n = int(x.numel() // (dims[0] * dims[1]))
-    out = torch.empty(tuple(x.shape[:-1]) + (dims[1] // 2 + 1,), dtype=torch.complex64, device=x.device)
-    _lib.call('lx_rfft2', 2, _cdims(dims), n, engine.ptr(x), engine.ptr(out), engine.stream_ptr())
+    dims = list(x.shape[-nd:])
+    n = int(x.numel() // int(np.prod(dims)))
+    out = torch.empty(tuple(x.shape[:-1]) + (dims[-1] // 2 + 1,), dtype=torch.complex64, device=x.device)
+    _lib.call('lx_rfft2', nd, _cdims(dims), n, engine.ptr(x), engine.ptr(out), engine.stream_ptr())
     return out
 
 
-def irfft2(X, W):
-    """``numpy.fft.irfft2`` (last axis of length ``W``) of a complex64 device tensor (HIP)."""
+def irfftn(X, dims):
+    """``numpy.fft.irfftn`` with output shape ``dims`` (last 2 or 3 axes) of a complex64 device tensor (HIP)."""
     X = X.contiguous()
-    dims = [X.shape[-2], W]
-    n = int(X.numel() // (dims[0] * (W // 2 + 1)))
-    out = torch.empty(tuple(X.shape[:-1]) + (W,), dtype=torch.float32, device=X.device)
+    dims = list(dims)
+    nd = len(dims)
+    n = int(X.numel() // (int(np.prod(dims[:-1])) * (dims[-1] // 2 + 1)))
+    out = torch.empty(tuple(X.shape[:-nd]) + tuple(dims), dtype=torch.float32, device=X.device)
     work = torch.empty_like(X)
-    _lib.call('lx_irfft2', 2, _cdims(dims), n, engine.ptr(X), engine.ptr(out), engine.ptr(work), engine.stream_ptr())
+    _lib.call('lx_irfft2', nd, _cdims(dims), n, engine.ptr(X), engine.ptr(out), engine.ptr(work), engine.stream_ptr())
     return out
 
 
-def half_to_full(X, W, scale=1.0):
-    """Hermitian extension of half spectra ``[..., H, W/2+1]`` to ``[..., H, W]``, times ``scale``."""
+def half_to_full(X, dims, scale=1.0):
+    """Hermitian extension of half spectra ``[..., *dims[:-1], W/2+1]`` to ``[..., *dims]``, times ``scale``."""
     X = X.contiguous()
-    dims = [X.shape[-2], W]
-    n = int(X.numel() // (dims[0] * (W // 2 + 1)))
-    out = torch.empty(tuple(X.shape[:-1]) + (W,), dtype=torch.complex64, device=X.device)
-    _lib.call('lx_half_to_full', 2, _cdims(dims), n, engine.ptr(X), engine.ptr(out), float(scale),
+    dims = list(dims)
+    nd = len(dims)
+    n = int(X.numel() // (int(np.prod(dims[:-1])) * (dims[-1] // 2 + 1)))
+    out = torch.empty(tuple(X.shape[:-nd]) + tuple(dims), dtype=torch.complex64, device=X.device)
+    _lib.call('lx_half_to_full', nd, _cdims(dims), n, engine.ptr(X), engine.ptr(out), float(scale),
               engine.stream_ptr())
     return out
 
@@ -74,8 +76,8 @@ class _Ctx:
 
     def __init__(self, A, K, gf, w, dt, tc_indices, growth_fn_t, weighted_fn, state_slug):
         N, C = A.shape[:2]
-        if A.dim() != 4:
-            raise NotImplementedError('the adjoint is built for 2-D worlds')
+        if A.dim() not in (4, 5):
+            raise ValueError(f'state must be [N, C, *world] with a 2-D or 3-D world, got {tuple(A.shape)}')
         self.N, self.C = N, C
         self.dims = tuple(A.shape[2:])
         self.maxk = K.shape[2]
@@ -101,8 +103,8 @@ class _Ctx:
     def accumulators(self, need_K):
         acc = {'gf': torch.zeros_like(self.gf), 'w': torch.zeros_like(self.w), 'dt': torch.zeros_like(self.dt)}
         if need_K:
-            H, W = self.dims
-            acc['dspec'] = torch.zeros((self.N, self.Keff, H, W // 2 + 1), dtype=torch.complex64, device=self.Kh.device)
+            acc['dspec'] = torch.zeros((self.N, self.Keff) + self.dims[:-1] + (self.dims[-1] // 2 + 1,),
+                                       dtype=torch.complex64, device=self.Kh.device)
         return acc
 
     def adjoint(self, A, field, pot, lam, acc, ste, dfield=None, dpot_in=None):
@@ -115,7 +117,7 @@ class _Ctx:
                   engine.ptr(lam_out), engine.ptr(dpot), engine.ptr(acc['gf']), engine.ptr(acc['w']),
                   engine.ptr(acc['dt']), engine.stream_ptr())
         dspec = acc.get('dspec')
-        FA = rfft2(A) if dspec is not None else None
+        FA = rfftn(A, A.dim() - 2) if dspec is not None else None
         _lib.call('lx_adjoint_potential', ctypes.byref(self.rule), self.N, engine.ptr(self.Kh), self.kbs,
                   engine.ptr(dpot), engine.ptr(FA), engine.ptr(dspec), engine.ptr(lam_out),
                   engine.ptr(core._workspace(self.rule, self.N)), engine.stream_ptr())
@@ -123,12 +125,11 @@ class _Ctx:
 
     def grad_K(self, dspec, K):
         """dL/dK (full complex spectra, torch convention) from the accumulated dspec."""
-        H, W = self.dims
         if self.S == 1 and self.N > 1:
             dspec = dspec.sum(dim=0, keepdim=True)
-        full = half_to_full(dspec, W, 1.0 / (H * W))  # [S, Keff, H, W]
+        full = half_to_full(dspec, self.dims, 1.0 / float(np.prod(self.dims)))  # [S, Keff, *dims]
         g = torch.zeros_like(K)
-        g.view(self.S, -1, H, W)[:, self.slots] = full
+        g.view((self.S, -1) + self.dims)[:, self.slots] = full
         return g
 
     def grad_gf(self, g_pad, gf_orig):
@@ -272,33 +273,36 @@ def run(state0, K, gf_params, kernels_weight_per_channel, dt, steps, tc_indices=
 class _Spectrum(torch.autograd.Function):
 
     @staticmethod
-    def forward(ctx, kernel):
+    def forward(ctx, kernel, nd):
         k = kernel.detach().to(torch.float32).contiguous()
-        H, W = k.shape[-2:]
-        n = int(k.numel() // (H * W))
+        dims = list(k.shape[-nd:])
+        n = int(k.numel() // int(np.prod(dims)))
         out = torch.empty(k.shape, dtype=torch.complex64, device=k.device)
-        work = torch.empty((max(n, 1) * H * (W // 2 + 1),), dtype=torch.complex64, device=k.device)
-        _lib.call('lx_kernel_spectrum', 2, _cdims([H, W]), n, engine.ptr(k), engine.ptr(out), engine.ptr(work),
+        work = torch.empty((max(n, 1) * int(np.prod(dims[:-1])) * (dims[-1] // 2 + 1),), dtype=torch.complex64,
+                           device=k.device)
+        _lib.call('lx_kernel_spectrum', nd, _cdims(dims), n, engine.ptr(k), engine.ptr(out), engine.ptr(work),
                   engine.stream_ptr())
+        ctx.nd = nd
         return out
 
     @staticmethod
     def backward(ctx, gK):
-        # kernel real: dL/dk = Re(H W ifftn(gK)) = H W irfft2 of gK's Hermitian part
+        # kernel real: dL/dk = Re(n ifftn(gK)) = n irfftn of gK's Hermitian part
         gK = gK.contiguous()
-        H, W = gK.shape[-2:]
-        n = int(gK.numel() // (H * W))
-        rule = engine.make_rule([H, W], 1, [0], [0])
-        half = torch.empty((n, H, W // 2 + 1), dtype=torch.complex64, device=gK.device)
+        nd = ctx.nd
+        dims = list(gK.shape[-nd:])
+        n = int(gK.numel() // int(np.prod(dims)))
+        rule = engine.make_rule(dims, 1, [0], [0])
+        half = torch.empty((n,) + tuple(dims[:-1]) + (dims[-1] // 2 + 1,), dtype=torch.complex64, device=gK.device)
         slot = (ctypes.c_int32 * 1)(0)
         _lib.call('lx_prepare_spectra', ctypes.byref(rule), engine.ptr(gK), n, 1, slot, engine.ptr(half),
                   engine.stream_ptr())
-        return (irfft2(half, W) * float(H * W)).reshape(gK.shape)
+        return (irfftn(half, dims) * float(np.prod(dims))).reshape(gK.shape), None
 
 
-def kernel_spectrum(kernel):
-    """Differentiable ``fftn`` of real kernels already padded and fftshift-ed, ``[..., H, W]``."""
-    return _Spectrum.apply(kernel)
+def kernel_spectrum(kernel, nd=2):
+    """Differentiable ``fftn`` over the last ``nd`` axes of real kernels already padded and fftshift-ed."""
+    return _Spectrum.apply(kernel, nd)
 
 
 def ring_basis(R, r, nb_rings, kf_slug, kf_params, world_size):
@@ -388,7 +392,7 @@ class _Potential(torch.autograd.Function):
         acc = sc.accumulators(ctx.needs_input_grad[1])
         lam = torch.zeros_like(A)
         dspec = acc.get('dspec')
-        FA = rfft2(A) if dspec is not None else None
+        FA = rfftn(A, A.dim() - 2) if dspec is not None else None
         _lib.call('lx_adjoint_potential', ctypes.byref(sc.rule), N, engine.ptr(sc.Kh), sc.kbs,
                   engine.ptr(g.contiguous()), engine.ptr(FA), engine.ptr(dspec), engine.ptr(lam),
                   engine.ptr(core._workspace(sc.rule, N)), engine.stream_ptr())

@@ -19,7 +19,7 @@ int lx_fail(int code, const char* fmt, ...) {
 extern "C" const char* lx_last_error(void) { return g_err; }
 extern "C" int lx_version(void) { return 1; }
 
-static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
 
 int lx_check_rule(const lx_rule* rule) {
   if (!rule) return lx_fail(LX_EINVAL, "rule is NULL");

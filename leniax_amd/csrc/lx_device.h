@@ -124,11 +124,57 @@ LX_DEV void bitrev_perm(float2* x) {
   }
 }
 
+constexpr bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+template <int P, bool INV>
+LX_DEV void fft_reg(float2* x);
+
+// Radix-3 step of a length-P FFT (P = 3Q): X[k1 + Q k2] = sum_r W_3^(r k2) W_P^(r k1) Y_r[k1],
+// Y_r the length-Q transform of x[3 n + r].  Twiddles are compile-time constants.
+template <int P, bool INV, int K1>
+LX_DEV void radix3_body(const float2* y0, const float2* y1, const float2* y2, float2* x) {
+  constexpr int Q = P / 3;
+  if constexpr (K1 < Q) {
+    const float2 a = y0[K1];
+    float2 b = y1[K1], c = y2[K1];
+    if constexpr (K1 != 0) {
+      b = cmul(b, make_float2(Tw<P, K1, INV>::re, Tw<P, K1, INV>::im));
+      c = cmul(c, make_float2(Tw<P, 2 * K1, INV>::re, Tw<P, 2 * K1, INV>::im));
+    }
+    constexpr float r = 0.866025403784438646763723170752936183f;  // sqrt(3) / 2
+    const float2 s = cadd(b, c), d = csub(b, c);
+    const float2 m = make_float2(fmaf(-0.5f, s.x, a.x), fmaf(-0.5f, s.y, a.y));
+    // forward W_3 = -1/2 - i r: X1 = m - i r d, X2 = m + i r d (inverse: conjugate)
+    const float2 ird = INV ? make_float2(-r * d.y, r * d.x) : make_float2(r * d.y, -r * d.x);
+    x[K1] = cadd(a, s);
+    x[K1 + Q] = cadd(m, ird);
+    x[K1 + 2 * Q] = csub(m, ird);
+    radix3_body<P, INV, K1 + 1>(y0, y1, y2, x);
+  }
+}
+
 // Natural-order in, natural-order out, unnormalised; INV uses exp(+2*pi*i*jk/P).
+// P a power of two (radix 2) or three times one (one radix-3 step on top).
 template <int P, bool INV>
 LX_DEV void fft_reg(float2* x) {
-  bitrev_perm<P, 0>(x);
-  fft_stages<P, INV, 2>(x);
+  if constexpr (is_pow2(P)) {
+    bitrev_perm<P, 0>(x);
+    fft_stages<P, INV, 2>(x);
+  } else {
+    static_assert(P % 3 == 0 && is_pow2(P / 3), "in-register FFT lengths are 2^k and 3 * 2^k");
+    constexpr int Q = P / 3;
+    float2 y0[Q], y1[Q], y2[Q];
+#pragma unroll
+    for (int n = 0; n < Q; ++n) {
+      y0[n] = x[3 * n];
+      y1[n] = x[3 * n + 1];
+      y2[n] = x[3 * n + 2];
+    }
+    fft_reg<Q, INV>(y0);
+    fft_reg<Q, INV>(y1);
+    fft_reg<Q, INV>(y2);
+    radix3_body<P, INV, 0>(y0, y1, y2, x);
+  }
 }
 
 // ----------------------------------------------------------------- DPP quad exchange

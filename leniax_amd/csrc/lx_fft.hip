@@ -1,0 +1,661 @@
+// The leniax step on the HBM-resident path, for 2-D and 3-D worlds, as separable FFT passes
+// over the Hermitian half spectrum, plus the plain transforms and the adjoint of the potential.
+//
+//   rows_fwd:  real rows (last axis, length W) -> half spectra along W, two rows per complex FFT
+//   lines:     the FFT of every line of the half spectrum along one leading axis.  mode 0 (the
+//              convolution axis): forward, x every kernel's half spectrum, inverse; mode 1:
+//              forward in place; mode 2: inverse (in place or into another buffer)
+//   rows_inv:  per channel c: inverse row FFT of every kernel feeding c (two rows at once),
+//              growth g_k, weighted sum into G_c, then the update
+//
+//   2-D [H][W]:    rows_fwd, lines(H, conv), rows_inv                            (3 passes)
+//   3-D [D][H][W]: rows_fwd, lines(H, fwd), lines(D, conv), lines(H, inv), rows_inv  (5 passes)
+//
+// Transform lengths: 2^k in [16, 512] and 3 * 2^k in [48, 384].  A length-L transform is
+// spread over a group of TPR lanes of one wave (P = L / TPR points per lane): an in-register
+// FFT of P points, a twiddle multiply and a radix-2 network across the group through DPP.
+// Workspace of a step: S [B][C][half] and U [B][K][half] complex64, half = prod(dims[:-1]) (W/2+1).
+#include <algorithm>
+#include <type_traits>
+
+#include "lx_internal.h"
+
+namespace {
+
+template <int L>
+struct Lanes {
+  static constexpr int TPR = L <= 128 ? 4 : (L % 3 == 0 ? L / 24 : L / 32);  // lanes per transform
+  static constexpr int P = L / TPR;                                          // points per lane
+  static_assert(TPR == 4 || TPR == 8 || TPR == 16, "lane groups of 4, 8 or 16");
+};
+
+template <int V>
+using IntC = std::integral_constant<int, V>;
+
+bool len_ok(int n) {
+  switch (n) {
+    case 16: case 32: case 48: case 64: case 96: case 128: case 192: case 256: case 384: case 512: return true;
+    default: return false;
+  }
+}
+
+// calls f(IntC<L>) for a supported transform length
+template <typename F>
+int dispatch_len(int L, F&& f) {
+  switch (L) {
+    case 16: return f(IntC<16>{});
+    case 32: return f(IntC<32>{});
+    case 48: return f(IntC<48>{});
+    case 64: return f(IntC<64>{});
+    case 96: return f(IntC<96>{});
+    case 128: return f(IntC<128>{});
+    case 192: return f(IntC<192>{});
+    case 256: return f(IntC<256>{});
+    case 384: return f(IntC<384>{});
+    case 512: return f(IntC<512>{});
+    default: return lx_fail(LX_ENOTSUP, "transform length %d (supported: 2^k in [16, 512], 3*2^k in [48, 384])", L);
+  }
+}
+
+// twiddle table exp(-2 pi i j / L) in global memory, one per length and device
+template <int L>
+__device__ float2 g_tw[L];
+
+template <int L>
+__global__ void init_tw_kernel() {
+  for (int j = threadIdx.x; j < L; j += blockDim.x) {
+    double sn, cs;
+    sincospi(2.0 * j / L, &sn, &cs);
+    g_tw<L>[j] = make_float2((float)cs, (float)-sn);
+  }
+}
+
+constexpr int kMaxDevices = 64;
+
+template <int L>
+int ensure_tw(hipStream_t stream) {
+  static bool ready[kMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return lx_fail(LX_EDEVICE, "hipGetDevice failed or device index out of range");
+  if (!ready[dev]) {
+    hipLaunchKernelGGL(init_tw_kernel<L>, dim3(1), dim3(256), 0, stream);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return lx_fail(LX_EDEVICE, "twiddle init: %s", hipGetErrorString(e));
+    ready[dev] = true;
+  }
+  return LX_OK;
+}
+
+int launched(const char* what) {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LX_OK : lx_fail(LX_EDEVICE, "%s launch: %s", what, hipGetErrorString(e));
+}
+
+unsigned blocks_for(int64_t threads) { return unsigned((threads + 255) / 256); }
+
+// partner lane holding block (TPR - s) mod TPR, s = bitrev(t)
+template <int TPR>
+LX_DEV int neg_block_lane(int t) {
+  const int s = bitrev_n(t, TPR);
+  return bitrev_n((TPR - s) & (TPR - 1), TPR);
+}
+
+// value of lane t ^ (TPR-1) in the same lane group
+template <int TPR>
+LX_DEV float2 mirror_fetch(float2 v) {
+  if constexpr (TPR == 4) return dpp2<LX_QP(3, 2, 1, 0)>(v);
+  else if constexpr (TPR == 8) return dpp2<0x141>(v);   // row_half_mirror
+  else return dpp2<0x140>(v);                           // row_mirror (TPR == 16)
+}
+
+// ---------------------------------------------------------------- geometry
+struct Geom {
+  int ndim, W, WH;
+  int64_t rows;   // rows of length W per image: prod(dims[:-1])
+  int64_t half;   // complex elements of one image's half spectrum: rows * WH
+  int naxes;      // leading axes: 1 (2-D) or 2 (3-D); axis 0 is the convolution axis
+  int L[2], A[2];
+  int64_t a_stride[2], st[2];  // line base = img * half + (a * a_stride + j), element stride st
+};
+
+int make_geom(int ndim, const int32_t* dims, Geom& g) {
+  if (ndim != 2 && ndim != 3) return lx_fail(LX_EINVAL, "ndim %d", ndim);
+  for (int i = 0; i < ndim; ++i)
+    if (!len_ok(dims[i]))
+      return lx_fail(LX_ENOTSUP, "world size %d on axis %d (supported: 2^k in [16, 512], 3*2^k in [48, 384])",
+                     dims[i], i);
+  g.ndim = ndim;
+  g.W = dims[ndim - 1];
+  g.WH = g.W / 2 + 1;
+  if (ndim == 2) {
+    const int H = dims[0];
+    g.rows = H;
+    g.naxes = 1;
+    g.L[0] = H; g.A[0] = 1; g.a_stride[0] = 0; g.st[0] = g.WH;
+  } else {
+    const int D = dims[0], H = dims[1];
+    g.rows = int64_t(D) * H;
+    g.naxes = 2;
+    g.L[0] = D; g.A[0] = H; g.a_stride[0] = g.WH; g.st[0] = int64_t(H) * g.WH;
+    g.L[1] = H; g.A[1] = D; g.a_stride[1] = int64_t(H) * g.WH; g.st[1] = g.WH;
+  }
+  g.half = g.rows * g.WH;
+  return LX_OK;
+}
+
+// ---------------------------------------------------------------- rows forward
+template <int W>
+__global__ __launch_bounds__(256) void rows_fwd_kernel(const float* __restrict__ A, float2* __restrict__ S,
+                                                       int64_t npairs, int64_t rows) {
+  using LC = Lanes<W>;
+  constexpr int TPR = LC::TPR, P = LC::P, WH = W / 2 + 1;
+  const int lane = threadIdx.x & 63;
+  const int t = threadIdx.x % TPR;
+  const int64_t pr = (int64_t(blockIdx.x) * 256 + threadIdx.x) / TPR;
+  const bool live = pr < npairs;
+  const int64_t row0 = live ? pr * 2 : 0;  // rows pair within an image: rows is even
+  const float* a0 = A + row0 * W;
+  const float* a1 = a0 + W;
+  float2 x[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) x[m] = make_float2(a0[TPR * m + t], a1[TPR * m + t]);
+  fft_lanes<P, TPR, false>(x, t, lane, g_tw<W>);
+  const int s = bitrev_n(t, TPR);
+  // Z[W - k]: k1 >= 1 from lane t^(TPR-1) at index P - k1; k1 == 0 from block (TPR - s) at index 0
+  const float2 z0n = make_float2(__shfl(x[0].x, (lane & ~(TPR - 1)) | neg_block_lane<TPR>(t), 64),
+                                 __shfl(x[0].y, (lane & ~(TPR - 1)) | neg_block_lane<TPR>(t), 64));
+  float2* s0 = S + row0 * int64_t(WH);
+  float2* s1 = s0 + WH;
+#pragma unroll
+  for (int k1 = 0; k1 < P; ++k1) {
+    const float2 zn = k1 == 0 ? z0n : mirror_fetch<TPR>(x[P - k1 > 0 && k1 > 0 ? P - k1 : 0]);
+    const float2 z = x[k1];
+    const int k = k1 + P * s;
+    // X_a = (z + conj zn) / 2, X_b = (z - conj zn) / 2i
+    const float2 xa = make_float2(0.5f * (z.x + zn.x), 0.5f * (z.y - zn.y));
+    const float2 xb = make_float2(0.5f * (z.y + zn.y), 0.5f * (zn.x - z.x));
+    if (live && k <= W / 2) {
+      s0[k] = xa;
+      s1[k] = xb;
+    }
+  }
+  (void)rows;
+}
+
+// ---------------------------------------------------------------- lines along a leading axis
+struct LineArgs {
+  const float2* S;
+  float2* U;
+  const float2* Kh;
+  int64_t k_bstride;
+  int64_t nlines;      // images * A * WH
+  int WH, A;
+  int64_t a_stride, st, img_elems;
+  int C, K;            // images per world (channels) and kernels (mode 0)
+  int cin[LX_MAX_K];   // mode 0: kernel k reads channel cin[k]
+  int mode;            // 0: conv, 1: forward in place, 2: inverse S -> U
+  int conjk;           // mode 0: multiply by conj(K) (adjoint of the potential)
+  // mode 0, optional: dspec[b][k] += X_k * conj(FAin[b][acc_cin[k]]) (kernel-spectrum gradient)
+  const float2* FAin;
+  float2* dspec;
+  int C_fa;
+  int acc_cin[LX_MAX_K];
+};
+
+LineArgs line_args() {
+  LineArgs a;
+  a.S = nullptr; a.U = nullptr; a.Kh = nullptr; a.k_bstride = 0; a.nlines = 0;
+  a.WH = 0; a.A = 1; a.a_stride = 0; a.st = 0; a.img_elems = 0; a.C = 1; a.K = 0;
+  for (int k = 0; k < LX_MAX_K; ++k) { a.cin[k] = -1; a.acc_cin[k] = 0; }
+  a.mode = 0; a.conjk = 0; a.FAin = nullptr; a.dspec = nullptr; a.C_fa = 1;
+  return a;
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
+  using LC = Lanes<L>;
+  constexpr int TPR = LC::TPR, P = LC::P;
+  const int lane = threadIdx.x & 63;
+  const int t = threadIdx.x % TPR;
+  const int64_t lid = (int64_t(blockIdx.x) * 256 + threadIdx.x) / TPR;
+  const bool live = lid < a.nlines;
+  const int64_t per_img = int64_t(a.A) * a.WH;
+  const int64_t img = live ? lid / per_img : 0;
+  const int64_t rem = live ? lid - img * per_img : 0;
+  const int64_t aa = rem / a.WH;
+  const int64_t off = aa * a.a_stride + (rem - aa * a.WH);  // line base inside its image
+  const int64_t b = img / a.C;
+  const int c = int(img - b * a.C);
+  const int64_t st = a.st;
+  const float2* src = a.S + img * a.img_elems + off;
+  const int s = bitrev_n(t, TPR);
+  float2 x[P], y[P];
+  if (a.mode == 2) {  // inverse only; input in frequency order
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) y[k1] = src[int64_t(k1 + P * s) * st];
+    fft_lanes<P, TPR, true>(y, t, lane, g_tw<L>);
+    float2* dst = a.U + img * a.img_elems + off;
+    if (live) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) dst[int64_t(TPR * m + t) * st] = y[m];
+    }
+    return;
+  }
+#pragma unroll
+  for (int m = 0; m < P; ++m) x[m] = src[int64_t(TPR * m + t) * st];
+  fft_lanes<P, TPR, false>(x, t, lane, g_tw<L>);
+  if (a.mode == 1) {  // forward only, in place
+    float2* dst = const_cast<float2*>(src);
+    if (live) {
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) dst[int64_t(k1 + P * s) * st] = x[k1];
+    }
+    return;
+  }
+  for (int k = 0; k < a.K; ++k) {
+    if (a.cin[k] != c) continue;
+    if (a.dspec && live) {
+      const float2* fa = a.FAin + (b * a.C_fa + a.acc_cin[k]) * a.img_elems + off;
+      float2* ds = a.dspec + (b * a.K + k) * a.img_elems + off;
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) {
+        const int64_t o = int64_t(k1 + P * s) * st;
+        ds[o] = cadd(ds[o], cmulc(x[k1], fa[o]));
+      }
+    }
+    const float2* kk = a.Kh + b * a.k_bstride + int64_t(k) * a.img_elems + off;
+    if (a.conjk) {
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) y[k1] = cmulc(x[k1], kk[int64_t(k1 + P * s) * st]);
+    } else {
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) y[k1] = cmul(x[k1], kk[int64_t(k1 + P * s) * st]);
+    }
+    fft_lanes<P, TPR, true>(y, t, lane, g_tw<L>);
+    float2* dst = a.U + (b * a.K + k) * a.img_elems + off;
+    if (live) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) dst[int64_t(TPR * m + t) * st] = y[m];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- rows inverse + field + update
+struct RowArgs {
+  const float2* U;
+  const float* A;
+  float* A_out;
+  float* field;
+  float* pot;
+  const float* gf;
+  int64_t gf_bs;
+  const float* w;
+  int64_t w_bs;
+  const float* dt;
+  int64_t dt_bs;
+  int64_t npairs;  // images * rows / 2
+  int64_t rows;    // rows per image
+  int C, K;
+  int gfk[LX_MAX_K];
+  int state_fn, average;
+  int update;      // 0: potential only
+  // selection mode: weight of kernel k in channel c is (cin[k] == c), growth is the identity and the
+  // update is A_out = A + G (no gf / w / dt reads).  Used for the potential alone, the adjoint of the
+  // potential (lambda += correlations) and the inverse real FFT.
+  int sel;
+  int cin[LX_MAX_K];
+};
+
+template <int W>
+__global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
+  using LC = Lanes<W>;
+  constexpr int TPR = LC::TPR, P = LC::P, WH = W / 2 + 1;
+  const int lane = threadIdx.x & 63;
+  const int t = threadIdx.x % TPR;
+  const int64_t pr = (int64_t(blockIdx.x) * 256 + threadIdx.x) / TPR;
+  const bool live = pr < a.npairs;
+  const int64_t half_rows = a.rows / 2;
+  const int64_t img = live ? pr / half_rows : 0;
+  const int64_t r2 = live ? (pr - img * half_rows) * 2 : 0;
+  const int64_t b = img / a.C;
+  const int c = int(img - b * a.C);
+  const float inv_n = 1.f / float(a.rows * W);
+  const float* pw = a.sel ? nullptr : a.w + b * a.w_bs;
+  const float* pg = a.sel ? nullptr : a.gf + b * a.gf_bs;
+  const int s = bitrev_n(t, TPR);
+  const bool mirror = s >= TPR / 2;
+  const float sg = mirror ? -1.f : 1.f;
+  float G0[P], G1[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) G0[m] = G1[m] = 0.f;
+  float den = 0.f;
+  float2 x[P];
+  for (int k = 0; k < a.K; ++k) {
+    const float wk = a.sel ? (a.cin[k] == c ? 1.f : 0.f) : pw[c * a.K + k];
+    den = add_nc(den, wk);
+    int owner = 0;
+    if (a.sel) {
+      owner = a.cin[k];
+    } else {
+      for (int cc = a.C - 1; cc >= 0; --cc)
+        if (pw[cc * a.K + k] != 0.f) owner = cc;
+    }
+    const bool want_pot = a.pot != nullptr && owner == c;
+    if (wk == 0.f && !want_pot) continue;
+    const float2* u0 = a.U + ((b * a.K + k) * a.rows + r2) * WH;
+    const float2* u1 = u0 + WH;
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) {
+      const int kk = k1 + P * s;
+      const int col = mirror ? W - kk : kk;  // W - kk <= W/2 when mirrored
+      const float2 ya = u0[col], yb = u1[col];
+      float2 z = make_float2(fmaf(-sg, yb.y, ya.x), fmaf(sg, ya.y, yb.x));
+      if (k1 == 0 && (s == 0 || s == TPR / 2)) z = make_float2(ya.x, yb.x);  // DC and Nyquist are real
+      x[k1] = z;
+    }
+    fft_lanes<P, TPR, true>(x, t, lane, g_tw<W>);
+    const float gm = a.sel ? 0.f : pg[2 * k], gs = a.sel ? 0.f : pg[2 * k + 1];
+    float* po0 = want_pot ? a.pot + ((b * a.K + k) * a.rows + r2) * W : nullptr;
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      const float U0 = x[m].x * inv_n, U1 = x[m].y * inv_n;
+      if (want_pot && live) {
+        po0[TPR * m + t] = U0;
+        po0[W + TPR * m + t] = U1;
+      }
+      if (a.update) {
+        G0[m] = add_nc(G0[m], mul_nc(wk, a.sel ? U0 : growth(a.gfk[k], gm, gs, U0)));
+        G1[m] = add_nc(G1[m], mul_nc(wk, a.sel ? U1 : growth(a.gfk[k], gm, gs, U1)));
+      }
+    }
+  }
+  if (!a.update || !live) return;
+  const float dtw = a.sel ? 1.f : a.dt[b * a.dt_bs];
+  const int64_t off = (img * a.rows + r2) * int64_t(W);
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const int col = TPR * m + t;
+    float g0 = G0[m], g1 = G1[m];
+    if (a.average) { g0 = (g0 / den); g1 = (g1 / den); }
+    if (a.field) { a.field[off + col] = g0; a.field[off + W + col] = g1; }
+    const float v0 = a.A[off + col], v1 = a.A[off + W + col];
+    float n0, n1;
+    if (a.state_fn == LX_STATE_V2) {
+      n0 = add_nc(mul_nc(v0, sub_nc(1.f, dtw)), mul_nc(dtw, g0));
+      n1 = add_nc(mul_nc(v1, sub_nc(1.f, dtw)), mul_nc(dtw, g1));
+    } else {
+      n0 = add_nc(v0, mul_nc(dtw, g0));
+      n1 = add_nc(v1, mul_nc(dtw, g1));
+      if (a.state_fn == LX_STATE_V1) {
+        n0 = n0 < 0.f ? 0.f : (n0 > 1.f ? 1.f : n0);
+        n1 = n1 < 0.f ? 0.f : (n1 > 1.f ? 1.f : n1);
+      }
+    }
+    a.A_out[off + col] = n0;
+    a.A_out[off + W + col] = n1;
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+int launch_rows_fwd(const Geom& g, const float* A, float2* S, int64_t nimg, hipStream_t st) {
+  const int64_t npairs = nimg * (g.rows / 2);
+  return dispatch_len(g.W, [&](auto w) {
+    constexpr int W = decltype(w)::value;
+    if (int rc = ensure_tw<W>(st)) return rc;
+    hipLaunchKernelGGL(rows_fwd_kernel<W>, dim3(blocks_for(npairs * Lanes<W>::TPR)), dim3(256), 0, st, A, S,
+                       npairs, g.rows);
+    return launched("rows_fwd");
+  });
+}
+
+// la: mode-specific fields filled by the caller; geometry of `axis` and the line count set here
+int launch_lines(const Geom& g, int axis, LineArgs la, int64_t nimg, hipStream_t st) {
+  la.WH = g.WH; la.A = g.A[axis]; la.a_stride = g.a_stride[axis]; la.st = g.st[axis]; la.img_elems = g.half;
+  la.nlines = nimg * g.A[axis] * int64_t(g.WH);
+  return dispatch_len(g.L[axis], [&](auto l) {
+    constexpr int L = decltype(l)::value;
+    if (int rc = ensure_tw<L>(st)) return rc;
+    hipLaunchKernelGGL(lines_kernel<L>, dim3(blocks_for(la.nlines * Lanes<L>::TPR)), dim3(256), 0, st, la);
+    return launched("lines");
+  });
+}
+
+int launch_rows_inv(const Geom& g, RowArgs ra, int64_t nimg, hipStream_t st) {
+  ra.rows = g.rows;
+  ra.npairs = nimg * (g.rows / 2);
+  return dispatch_len(g.W, [&](auto w) {
+    constexpr int W = decltype(w)::value;
+    if (int rc = ensure_tw<W>(st)) return rc;
+    hipLaunchKernelGGL(rows_inv_kernel<W>, dim3(blocks_for(ra.npairs * Lanes<W>::TPR)), dim3(256), 0, st, ra);
+    return launched("rows_inv");
+  });
+}
+
+RowArgs row_args(const lx_rule* rule) {
+  RowArgs ra;
+  ra.U = nullptr; ra.A = nullptr; ra.A_out = nullptr; ra.field = nullptr; ra.pot = nullptr;
+  ra.gf = nullptr; ra.gf_bs = 0; ra.w = nullptr; ra.w_bs = 0; ra.dt = nullptr; ra.dt_bs = 0;
+  ra.npairs = 0; ra.rows = 0; ra.C = rule->C; ra.K = rule->K;
+  for (int k = 0; k < LX_MAX_K; ++k) {
+    ra.gfk[k] = k < rule->K ? rule->gf[k] : 0;
+    ra.cin[k] = k < rule->K ? rule->cin[k] : -1;
+  }
+  ra.state_fn = rule->state_fn; ra.average = rule->average; ra.update = 0; ra.sel = 0;
+  return ra;
+}
+
+// forward transform of every image along the leading axes (after rows_fwd), in place
+int fwd_leading(const Geom& g, float2* X, int64_t nimg, hipStream_t st) {
+  for (int ax = g.naxes - 1; ax >= 0; --ax) {
+    LineArgs la = line_args();
+    la.S = X; la.mode = 1;
+    if (int rc = launch_lines(g, ax, la, nimg, st)) return rc;
+  }
+  return LX_OK;
+}
+
+// One leniax step (or, with sel, the potential alone).  A: [B][C][dims]; work: S then U.
+int plain_step(const lx_rule* rule, const float* A, int64_t B, const void* Kh, int64_t k_bs, const float* gf,
+               int64_t gf_bs, const float* w, int64_t w_bs, const float* dt, int64_t dt_bs, float* A_out,
+               float* field, float* pot, void* work, hipStream_t st, int sel) {
+  Geom g;
+  if (int rc = make_geom(rule->ndim, rule->dims, g)) return rc;
+  const int C = rule->C, K = rule->K;
+  float2* S = static_cast<float2*>(work);
+  float2* U = S + B * C * g.half;
+  int rc = launch_rows_fwd(g, A, S, B * C, st);
+  if (!rc && g.naxes == 2) {  // forward along H
+    LineArgs la = line_args();
+    la.S = S; la.mode = 1;
+    rc = launch_lines(g, 1, la, B * C, st);
+  }
+  if (!rc) {  // the convolution axis
+    LineArgs la = line_args();
+    la.S = S; la.U = U; la.Kh = static_cast<const float2*>(Kh); la.k_bstride = k_bs; la.C = C; la.K = K;
+    for (int k = 0; k < LX_MAX_K; ++k) la.cin[k] = k < K ? rule->cin[k] : -1;
+    rc = launch_lines(g, 0, la, B * C, st);
+  }
+  if (!rc && g.naxes == 2) {  // inverse along H, in place
+    LineArgs la = line_args();
+    la.S = U; la.U = U; la.mode = 2;
+    rc = launch_lines(g, 1, la, B * K, st);
+  }
+  if (rc) return rc;
+  RowArgs ra = row_args(rule);
+  ra.U = U; ra.A = A; ra.A_out = A_out; ra.field = field; ra.pot = pot;
+  ra.gf = gf; ra.gf_bs = gf_bs; ra.w = w; ra.w_bs = w_bs; ra.dt = dt; ra.dt_bs = dt_bs;
+  ra.update = A_out != nullptr; ra.sel = sel;
+  return launch_rows_inv(g, ra, B * C, st);
+}
+
+// full spectrum from the half spectrum of a real array (Hermitian extension), times scale
+__global__ void expand_half_kernel(const float2* __restrict__ half, float2* __restrict__ full, int64_t n, int D,
+                                   int H, int W, float scale) {
+  const int WH = W / 2 + 1;
+  const int64_t total = n * D * int64_t(H) * W;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x) {
+    const int kw = int(i % W);
+    const int kh = int((i / W) % H);
+    const int kd = int((i / (int64_t(W) * H)) % D);
+    const int64_t img = i / (int64_t(D) * H * W);
+    const float2* hb = half + img * int64_t(D) * H * WH;
+    const float2 v = kw <= W / 2 ? hb[(int64_t(kd) * H + kh) * WH + kw]
+                                 : cconj(hb[(int64_t((D - kd) % D) * H + (H - kh) % H) * WH + (W - kw)]);
+    full[i] = cscale(v, scale);
+  }
+}
+
+int half_to_full(const Geom& g, int64_t n, const float2* half, float2* full, float scale, hipStream_t st) {
+  const int D = g.ndim == 3 ? g.L[0] : 1;
+  const int H = g.ndim == 3 ? g.L[1] : g.L[0];
+  const int64_t total = n * D * int64_t(H) * g.W;
+  const unsigned grid = unsigned(std::min<int64_t>((total + 255) / 256, 65536));
+  hipLaunchKernelGGL(expand_half_kernel, dim3(grid), dim3(256), 0, st, half, full, n, D, H, g.W, scale);
+  return launched("half to full");
+}
+
+// numpy.fft.irfftn (scaled by 1 / prod(dims)) of n half spectra; X untouched; work: n * half
+int irfft(const Geom& g, int64_t n, const float2* X, float* x, float2* work, hipStream_t st) {
+  for (int ax = 0; ax < g.naxes; ++ax) {
+    LineArgs la = line_args();
+    la.S = ax == 0 ? X : work; la.U = work; la.mode = 2;
+    if (int rc = launch_lines(g, ax, la, n, st)) return rc;
+  }
+  lx_rule one{};
+  one.ndim = g.ndim; one.C = 1; one.K = 1; one.cin[0] = 0;
+  RowArgs ra = row_args(&one);
+  ra.U = work; ra.pot = x; ra.sel = 1;
+  return launch_rows_inv(g, ra, n, st);
+}
+
+}  // namespace
+
+// ====================================================================== entry points
+extern "C" int lx_workspace_size(const lx_rule* rule, int64_t B, size_t* bytes) {
+  int rc = lx_check_rule(rule);
+  if (rc) return rc;
+  if (!bytes || B < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  int64_t half = 1;
+  for (int i = 0; i < rule->ndim - 1; ++i) half *= rule->dims[i];
+  half *= rule->dims[rule->ndim - 1] / 2 + 1;
+  *bytes = size_t(B) * size_t(rule->C + rule->K) * size_t(half) * sizeof(float2);
+  return LX_OK;
+}
+
+extern "C" int lx_update(const lx_rule* rule, const float* state_d, int64_t B, const void* K_half_d,
+                         int64_t k_bstride, const float* gf_d, int64_t gf_bstride, const float* w_d, int64_t w_bstride,
+                         const float* dt_d, int64_t dt_bstride, float* state_out_d, float* field_d,
+                         float* potential_d, void* work_d, void* stream) {
+  int rc = lx_check_rule(rule);
+  if (rc) return rc;
+  if (!state_d || !K_half_d || !gf_d || !w_d || !dt_d || !state_out_d || !work_d)
+    return lx_fail(LX_EINVAL, "NULL pointer");
+  if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
+  if (B == 0) return LX_OK;
+  if (state_out_d == state_d) return lx_fail(LX_EINVAL, "lx_update: state_out must not alias state");
+  return plain_step(rule, state_d, B, K_half_d, k_bstride, gf_d, gf_bstride, w_d, w_bstride, dt_d, dt_bstride,
+                    state_out_d, field_d, potential_d, work_d, (hipStream_t)stream, 0);
+}
+
+extern "C" int lx_potential(const lx_rule* rule, const float* state_d, int64_t B, const void* K_half_d,
+                            int64_t k_bstride, float* potential_d, void* work_d, void* stream) {
+  int rc = lx_check_rule(rule);
+  if (rc) return rc;
+  if (!state_d || !K_half_d || !potential_d || !work_d) return lx_fail(LX_EINVAL, "NULL pointer");
+  if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
+  if (B == 0) return LX_OK;
+  // selection mode: kernel k's potential is written by the row pass of its input channel; no update
+  return plain_step(rule, state_d, B, K_half_d, k_bstride, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, nullptr,
+                    potential_d, work_d, (hipStream_t)stream, 1);
+}
+
+extern "C" int lx_rfft2(int32_t ndim, const int32_t* dims, int64_t n, const float* x_d, void* X_d, void* stream) {
+  if (!dims || !x_d || !X_d || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  Geom g;
+  if (int rc = make_geom(ndim, dims, g)) return rc;
+  if (n == 0) return LX_OK;
+  hipStream_t st = (hipStream_t)stream;
+  float2* X = static_cast<float2*>(X_d);
+  if (int rc = launch_rows_fwd(g, x_d, X, n, st)) return rc;
+  return fwd_leading(g, X, n, st);
+}
+
+extern "C" int lx_irfft2(int32_t ndim, const int32_t* dims, int64_t n, const void* X_d, float* x_d, void* work_d,
+                         void* stream) {
+  if (!dims || !x_d || !X_d || !work_d || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  Geom g;
+  if (int rc = make_geom(ndim, dims, g)) return rc;
+  if (n == 0) return LX_OK;
+  return irfft(g, n, static_cast<const float2*>(X_d), x_d, static_cast<float2*>(work_d), (hipStream_t)stream);
+}
+
+extern "C" int lx_half_to_full(int32_t ndim, const int32_t* dims, int64_t n, const void* half_d, void* full_d,
+                               float scale, void* stream) {
+  if (!dims || !half_d || !full_d || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  Geom g;
+  if (int rc = make_geom(ndim, dims, g)) return rc;
+  if (n == 0) return LX_OK;
+  return half_to_full(g, n, static_cast<const float2*>(half_d), static_cast<float2*>(full_d), scale,
+                      (hipStream_t)stream);
+}
+
+extern "C" int lx_kernel_spectrum(int32_t ndim, const int32_t* dims, int64_t n, const float* k_d, void* K_full_d,
+                                  void* work_d, void* stream) {
+  if (!dims || !k_d || !K_full_d || !work_d || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  Geom g;
+  if (int rc = make_geom(ndim, dims, g)) return rc;
+  if (n == 0) return LX_OK;
+  hipStream_t st = (hipStream_t)stream;
+  float2* work = static_cast<float2*>(work_d);
+  if (int rc = launch_rows_fwd(g, k_d, work, n, st)) return rc;
+  if (int rc = fwd_leading(g, work, n, st)) return rc;
+  return half_to_full(g, n, work, static_cast<float2*>(K_full_d), 1.f, st);
+}
+
+// lambda[b][cin k] += correlation of dU_k with kernel k; dspec[b][k] += fft(dU_k) conj(FA[b][cin k])
+extern "C" int lx_adjoint_potential(const lx_rule* rule, int64_t B, const void* K_half_d, int64_t k_bstride,
+                                    const float* dpot_d, const void* state_spec_d, void* dspec_d, float* lambda_d,
+                                    void* work_d, void* stream) {
+  int rc = lx_check_rule(rule);
+  if (rc) return rc;
+  if (!K_half_d || !dpot_d || !lambda_d || !work_d) return lx_fail(LX_EINVAL, "NULL pointer");
+  if ((state_spec_d == nullptr) != (dspec_d == nullptr))
+    return lx_fail(LX_EINVAL, "state_spec and dspec are given together or not at all");
+  if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
+  if (B == 0) return LX_OK;
+  Geom g;
+  if ((rc = make_geom(rule->ndim, rule->dims, g))) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int K = rule->K;
+  float2* X = static_cast<float2*>(work_d);
+  // the K gradient images of a world are transformed as K "channels", kernel k on image k
+  if ((rc = launch_rows_fwd(g, dpot_d, X, B * K, st))) return rc;
+  if (g.naxes == 2) {
+    LineArgs la = line_args();
+    la.S = X; la.mode = 1;
+    if ((rc = launch_lines(g, 1, la, B * K, st))) return rc;
+  }
+  {
+    LineArgs la = line_args();
+    la.S = X; la.U = X;  // each line is read whole before it is written back
+    la.Kh = static_cast<const float2*>(K_half_d); la.k_bstride = k_bstride; la.C = K; la.K = K; la.conjk = 1;
+    for (int k = 0; k < LX_MAX_K; ++k) {
+      la.cin[k] = k < K ? k : -1;
+      la.acc_cin[k] = k < K ? rule->cin[k] : 0;
+    }
+    la.FAin = static_cast<const float2*>(state_spec_d);
+    la.dspec = static_cast<float2*>(dspec_d);
+    la.C_fa = rule->C;
+    if ((rc = launch_lines(g, 0, la, B * K, st))) return rc;
+  }
+  if (g.naxes == 2) {
+    LineArgs la = line_args();
+    la.S = X; la.U = X; la.mode = 2;
+    if ((rc = launch_lines(g, 1, la, B * K, st))) return rc;
+  }
+  RowArgs ra = row_args(rule);
+  ra.U = X; ra.A = lambda_d; ra.A_out = lambda_d;  // lambda_c += sum over kernels fed by c (in place, per element)
+  ra.update = 1; ra.sel = 1; ra.average = 0; ra.state_fn = LX_STATE_SIMPLE;
+  return launch_rows_inv(g, ra, B * rule->C, st);
+}

@@ -286,9 +286,11 @@ def _tree_set(tree, path, value):
     return {k: (_tree_set(v, path[1:], value) if k == path[0] else v) for k, v in tree.items()}
 
 
-def make_gradient_fn(pipeline_fn: Callable, normalize: bool = True) -> Callable:
+def make_gradient_fn(pipeline_fn: Callable, normalize: bool = True, wrt_state0: bool = False) -> Callable:
     """leniax/runner.py:426-444: ``fn(...) -> ((loss, aux), grads)`` with grads w.r.t. ``params``
-    (same tree), each leaf divided by its norm + 1e-8 when ``normalize``."""
+    (same tree), each leaf divided by its norm + 1e-8 when ``normalize``.  With
+    ``wrt_state0`` (not in leniax, whose state0 is not differentiated) ``grads['state0']``
+    holds the gradient of the initial state as well."""
 
     def fn(rng_key, params: Dict, variables: Dict, state0, targets):
         leaves = _tree_leaves(params)
@@ -298,12 +300,18 @@ def make_gradient_fn(pipeline_fn: Callable, normalize: bool = True) -> Callable:
             t = leaf.detach().clone().requires_grad_(True)
             ins.append(t)
             tracked = _tree_set(tracked, path, t)
+        if wrt_state0:
+            state0 = engine.as_device(state0).detach().clone().requires_grad_(True)
+            ins.append(state0)
         loss, aux = pipeline_fn(rng_key, tracked, variables, state0, targets)
-        gs = torch.autograd.grad(loss, ins, allow_unused=True)
+        gs = list(torch.autograd.grad(loss, ins, allow_unused=True))
         grads = params
         for (path, leaf), t, g in zip(leaves, ins, gs):
             g = torch.zeros_like(t) if g is None else g
             grads = _tree_set(grads, path, normalize_fn(g) if normalize else g)
+        if wrt_state0:
+            g = gs[-1] if gs[-1] is not None else torch.zeros_like(state0)
+            grads = dict(grads, state0=normalize_fn(g) if normalize else g)
         return (loss.detach(), aux), grads
 
     return fn

@@ -54,17 +54,22 @@ def _asym_K(seed, C, maxk, H, W, R=5):
     return K
 
 
-def test_device_rfft2_irfft2_half_to_full():
+@pytest.mark.parametrize('shape', [(3, 64, 32), (2, 96, 48), (1, 48, 96, 32), (2, 32, 96, 96)])
+def test_device_rfftn_irfftn_half_to_full(shape):
+    """2-D and 3-D transforms, radix-2 and mixed radix-3 lengths, against numpy."""
     from leniax_amd import autograd as AG
+    nd = len(shape) - 1
+    print('shape', shape, flush=True)
     rng = np.random.default_rng(0)
-    x = rng.normal(size=(3, 64, 32)).astype(np.float32)
-    X = AG.rfft2(_dev(x))
-    ref = np.fft.rfft2(x.astype(np.float64))
+    x = rng.normal(size=shape).astype(np.float32)
+    axes = tuple(range(-nd, 0))
+    X = AG.rfftn(_dev(x), nd)
+    ref = np.fft.rfftn(x.astype(np.float64), axes=axes)
     assert _rel(_np(X), ref) < 2e-6
-    back = AG.irfft2(X, 32)
+    back = AG.irfftn(X, shape[1:])
     assert np.abs(_np(back) - x).max() < 1e-5
-    full = AG.half_to_full(X, 32, 0.5)
-    assert _rel(_np(full), 0.5 * np.fft.fft2(x.astype(np.float64))) < 2e-6
+    full = AG.half_to_full(X, shape[1:], 0.5)
+    assert _rel(_np(full), 0.5 * np.fft.fftn(x.astype(np.float64), axes=axes)) < 2e-6
 
 
 CASES = [('1c1k', 'v1', True, True), ('1c1k', 'v1', True, False), ('1c1k', 'v2', True, False),
@@ -211,10 +216,9 @@ def test_gradient_pipeline_c3(steps, ste):
         return 0.5 * (d * d).mean(), None
 
     pipeline = runner.make_pipeline_fn(steps, dt, step, loss_fn, keep_all_timesteps=False, ckpt_every=16)
-    grad_fn = runner.make_gradient_fn(pipeline, normalize=False)
+    grad_fn = runner.make_gradient_fn(pipeline, normalize=False, wrt_state0=True)
     params = {'b': _dev(np.array(b, np.float32)), 'gf_params': _dev(gf), 'dt': torch.tensor(dt, device='cuda')}
-    tA = _dev(A0).requires_grad_()
-    (loss, _), grads = grad_fn(None, params, {}, tA, {'cells': _dev(np.roll(A0, (8, 8), axis=(-2, -1)))})
+    (loss, _), grads = grad_fn(None, params, {}, _dev(A0), {'cells': _dev(np.roll(A0, (8, 8), axis=(-2, -1)))})
     print(f'c3 steps={steps} ste={ste} loss {float(loss):.6e}', flush=True)
 
     rule = L.Rule(['gaussian'])
@@ -226,7 +230,8 @@ def test_gradient_pipeline_c3(steps, ste):
         o['b'] = ADJ.b_vjp(R, 1.0, b, kf, kfp, (H, H), ADJ.dkappa_from_dK(np.asarray(o['K'][0, 0, 0], np.complex128),
                                                                           (H, H)))
         res[f] = o
-    for name, got in (('gf_params', grads['gf_params']), ('dt', grads['dt']), ('b', grads['b'])):
+    for name, got in (('gf_params', grads['gf_params']), ('dt', grads['dt']), ('b', grads['b']),
+                      ('state', grads['state0'])):
         g, r64, r32 = _np(got), np.asarray(res[np.float64][name]), np.asarray(res[np.float32][name])
         err, env = np.abs(g - r64).max(), np.abs(r32 - r64).max()
         print(f'  {name}: gpu {np.ravel(g)[:3]} o64 {np.ravel(r64)[:3]} |gpu-o64| {err:.3e} |o32-o64| {env:.3e}')
