@@ -12,6 +12,7 @@ baseline (the float32 oracle on this host's cores, bounded sample).
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -126,12 +127,140 @@ def adjoint_rate(n=256, steps=64, ckpt=16, reps=5):
     return out
 
 
+def c2_rate(worlds=256, n=256, steps=8, seed=0):
+    """Batched 3-channel 6-kernel parameter search on the plain HBM path: `worlds` worlds of
+    n x n, per-world kernels (kernel k reads channel k // 2, writes a random channel; r_k =
+    R_k / 15 with R_k ~ U[10, 15]; 1-3 rings, b ~ U[0, 1]; gaussian growth mu ~ U[0.1, 0.4],
+    sigma ~ U[0.01, 0.08]; weight h ~ U[0.1, 1]; weighted sum), per-channel U[0, 1) 64x64
+    centred patches; one step = lx_update + lx_stats over the batch."""
+    import torch
+    from leniax_amd import _lib, engine, kernels
+    dev = torch.device('cuda', 0)
+    rng = np.random.default_rng(seed)
+    Ks, gfs, ws = [], [], []
+    for _ in range(worlds):
+        kp = []
+        for k in range(6):
+            nb = int(rng.integers(1, 4))
+            b = rng.uniform(0, 1, nb)
+            while not b.any():
+                b = rng.uniform(0, 1, nb)
+            kp.append({'k_slug': 'circle_2d', 'k_params': [float(rng.uniform(10, 15)) / 15., [float(v) for v in b]],
+                       'kf_slug': 'gauss_bump', 'kf_params': [1.], 'gf_slug': 'gaussian',
+                       'gf_params': [float(rng.uniform(0.1, 0.4)), float(rng.uniform(0.01, 0.08))],
+                       'h': float(rng.uniform(0.1, 1.)), 'c_in': k // 2, 'c_out': int(rng.integers(0, 3))})
+        K, mp = kernels.get_kernels_and_mapping(kp, [n, n], 3, 15.)
+        Ks.append(K)
+        gfs.append(mp.get_gf_params())
+        ws.append(mp.get_kernels_weight_per_channel())
+    K = torch.cat(Ks)  # [worlds, 3, 2, n, n]
+    rule = engine.make_rule([n, n], 3, [0, 0, 1, 1, 2, 2], [_lib.GROWTH_IDS['gaussian']] * 6, 0, 0)
+    Kh = engine.prepare_spectra(rule, K, list(range(6)))
+    del K, Ks
+    gf = torch.stack(gfs).contiguous()
+    w = torch.stack(ws).contiguous()
+    cells = np.zeros((worlds, 3, n, n), np.float32)
+    o = (n - 64) // 2
+    cells[:, :, o:o + 64, o:o + 64] = rng.random((worlds, 3, 64, 64), dtype=np.float32)
+    a = torch.as_tensor(cells).to(dev)
+    bbuf = torch.empty_like(a)
+    field = torch.empty_like(a)
+    pot = torch.empty((worlds, 6, n, n), device=dev)
+    dt = torch.tensor([np.float32(0.1)], device=dev)
+    nbytes = ctypes.c_size_t(0)
+    _lib.call('lx_workspace_size', ctypes.byref(rule), worlds, ctypes.byref(nbytes))
+    work = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
+    carry = engine.carry_init(worlds)
+    stats = torch.empty((worlds, 11 + 3), device=dev)
+    kbs = int(Kh[0].numel())
+    st = engine.stream_ptr()
+
+    def step(x, y):
+        _lib.call('lx_update', ctypes.byref(rule), engine.ptr(x), worlds, engine.ptr(Kh), kbs, engine.ptr(gf),
+                  12, engine.ptr(w), 18, engine.ptr(dt), 0, engine.ptr(y), engine.ptr(field), engine.ptr(pot),
+                  engine.ptr(work), st)
+        _lib.call('lx_stats', ctypes.byref(rule), engine.ptr(x), engine.ptr(field), engine.ptr(pot), worlds,
+                  ctypes.c_float(15.), ctypes.c_float(10.), engine.ptr(carry), engine.ptr(stats), st)
+
+    step(a, bbuf)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step(a, bbuf)
+        a, bbuf = bbuf, a
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    F = 2.5 * n * n * math.log2(n * n) * 9 + 6 * 6 * n * (n // 2 + 1) + 10 * 6 * n * n + 9 * n * n
+    return {'config': f'{worlds} worlds of {n}x{n}, 3 channels, 6 kernels, per-world spectra, step + statistics',
+            'ms_per_step': sec * 1e3, 'world_steps_per_s': worlds / sec,
+            'cell_updates_per_s': worlds * 3 * n * n / sec,
+            'flops_per_world_step': F, 'achieved_tflops': worlds * F / sec / 1e12}
+
+
+def c4_adjoint_rate(worlds=16, n=96, steps=128, ckpt=32, reps=2):
+    """Batched 3-D adjoint: `worlds` worlds of n^3 (R=10, b=[1, 0.5], gaussian 0.2 / 0.03,
+    dt=0.1, U[0,1) 24^3 cubes), `steps` steps, loss = sum over worlds of the squared change
+    of total mass, gradients w.r.t. the initial states and (m, s), checkpoints every `ckpt`."""
+    import torch
+    from leniax_amd import autograd as AG
+    from leniax_amd import kernels
+    dev = torch.device('cuda', 0)
+    rng = np.random.default_rng(0)
+    k = kernels.sphere_nd(10, 1.0, [1.0, 0.5], 'gauss_bump', [1.0], 3)[0]
+    pads = [((n - d) // 2, n - d - (n - d) // 2) for d in k.shape]
+    kp = np.fft.fftshift(np.pad(k, pads)).astype(np.float32)
+    K = AG.kernel_spectrum(torch.as_tensor(kp).to(dev), 3)[None, None, None]
+    a0 = np.zeros((worlds, 1, n, n, n), np.float32)
+    o = (n - 24) // 2
+    a0[:, 0, o:o + 24, o:o + 24, o:o + 24] = rng.random((worlds, 24, 24, 24), dtype=np.float32)
+    A0 = torch.as_tensor(a0).to(dev)
+    mass0 = A0.sum(dim=(1, 2, 3, 4))
+    w = torch.ones(1, 1, device=dev)
+
+    def fwd_bwd(backward=True):
+        A = A0.clone().requires_grad_(backward)
+        gf = torch.tensor([[0.2, 0.03]], device=dev, requires_grad=backward)
+        fin = AG.run(A, K, gf, w, 0.1, steps, ckpt_every=ckpt)
+        loss = ((fin.sum(dim=(1, 2, 3, 4)) - mass0) ** 2).sum()
+        if backward:
+            loss.backward()
+        return loss
+
+    out = {}
+    for name, bw in (('fwd_ms', False), ('fwd_bwd_ms', True)):
+        with torch.set_grad_enabled(bw):
+            fwd_bwd(bw)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fwd_bwd(bw)
+            torch.cuda.synchronize()
+        out[name] = (time.perf_counter() - t0) / reps * 1e3
+    out['ratio'] = out['fwd_bwd_ms'] / out['fwd_ms']
+    out['fwd_bwd_world_steps_per_s'] = worlds * steps / (out['fwd_bwd_ms'] / 1e3)
+    out['config'] = f'{worlds} worlds of {n}^3, {steps} steps, checkpoints every {ckpt}, gradients of A0 and (m, s)'
+    return out
+
+
+def measured_traffic():
+    """HBM bytes per fused launch from the committed rocprofv3 PMC summary (FETCH_SIZE +
+    WRITE_SIZE, each scaled by the counter's reading for a copy of known size), or None."""
+    path = os.path.join(ROOT, 'profiles', 'fused2d_kernel_128_traffic.json')
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get('hbm_bytes_per_launch')
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--only-fused', action='store_true', help='time the fused launch only (profiling runs)')
+    ap.add_argument('--copy-mb', type=int, default=0,
+                    help='after timing, copy this many MiB device-to-device (a known-size kernel for PMC scaling)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: every world batch fits one MI355X and worlds are independent '
@@ -188,6 +317,17 @@ def main():
     full_ws = B * MAX_STEPS / (full_ms / 1e3)
     log(f'fused full run: {full_ms:.1f} ms for {B * MAX_STEPS} world-steps -> {full_ws / 1e6:.2f} M world-steps/s')
 
+    if args.copy_mb:
+        src = torch.ones(args.copy_mb * 2 ** 18, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f'copied {args.copy_mb} MiB device to device')
+    if args.only_fused:
+        print(json.dumps({'fused_launch_ms': launch_ms, 'world_steps_per_launch': int(steps_run.sum()),
+                          'fused_full_ms': full_ms}))
+        return
+
     # plain multi-pass step (lx_update), one world per call and the whole batch per call
     from leniax_amd.growth_functions import register as gfr
     gfn = (gfr['gaussian'],)
@@ -213,6 +353,12 @@ def main():
     adj = adjoint_rate()
     log(f"gradient 256x256, 64 steps, checkpoints every 16: forward {adj['fwd_ms']:.2f} ms, "
         f"forward+backward {adj['fwd_bwd_ms']:.2f} ms ({adj['ratio']:.2f}x forward)")
+    c4 = c4_adjoint_rate()
+    log(f"3-D adjoint 16 x 96^3, 128 steps: forward {c4['fwd_ms']:.1f} ms, forward+backward "
+        f"{c4['fwd_bwd_ms']:.1f} ms ({c4['ratio']:.2f}x)")
+    c2 = c2_rate()
+    log(f"3C6K 256 x 256^2 plain step + statistics: {c2['ms_per_step']:.2f} ms/step, "
+        f"{c2['world_steps_per_s']:.0f} world-steps/s")
 
     flops_per_launch = F * steps_run.sum()
     achieved = flops_per_launch / (launch_ms / 1e3)
@@ -238,13 +384,15 @@ def main():
         'fused_full_1024_steps': {'world_steps_per_s': full_ws, 'cell_updates_per_s': full_ws * N * N,
                                   'ms': full_ms},
         'gradient_256x256_64_steps': adj,
+        'adjoint_3d_16x96cubed_128_steps': c4,
+        'plain_3c6k_256_worlds_256x256': c2,
         'plain_single_world_steps_per_s': plain_single,
         'plain_batched_world_steps_per_s': plain_batched,
         'fused_vs_plain_single_world': full_ws / plain_single,
         'fused_vs_plain_batched': full_ws / plain_batched,
         'roofline': {'bound': 'mfma', 'compute': 'fp32 VALU (no MFMA: no dense contraction on the path)',
                      'kernel': 'fused2d_kernel<128>', 'achieved': achieved / 1e12, 'peak': PEAK_FP32 / 1e12,
-                     'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32, 'traffic': None,
+                     'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32, 'traffic': measured_traffic(),
                      'flops_per_world_step': F, 'launch_ms': launch_ms},
     }
     if not args.no_cpu:

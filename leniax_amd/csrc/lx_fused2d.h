@@ -290,15 +290,18 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
       }
       fft_lanes<P, TPR, true, SPREAD>(x, fresh(q), lane, tw);  // x[m] = (U[2r][TPR m+q], U[2r+1][TPR m+q]) * N^2
 
-      // ---- growth, field, update, statistics sums
-      float sA = 0.f, sG = 0.f;
-      uint32_t nA = 0, nG = 0, nU = 0;  // per-lane counts
-      float sAy0 = 0.f, sAy1 = 0.f, sGy0 = 0.f, sGy1 = 0.f, sAyy0 = 0.f, sAyy1 = 0.f;
+      // ---- growth, field, update, statistics sums.  A lane owns rows 2r, 2r+1 at columns
+      //      TPR m + q: row moments factor into per-row sums, column moments into per-column
+      //      sums of the two rows.
+      float sa0 = 0.f, sa1 = 0.f, sg0 = 0.f, sg1 = 0.f;  // per-row sums of A and G+
+      float sAy1 = 0.f, sGy1 = 0.f, sAyy1 = 0.f;         // column (x) moments
+      uint32_t nA = 0, nG = 0, nU = 0;                    // per-lane counts
       const int64_t hist_off = (int64_t(t) * p.B + world) * cells_per_world;
 #pragma unroll
       for (int m = 0; m < P; ++m) {
         const int col = TPR * m + q;
         const float xc = float(((col - sh1) & (N - 1)) - N / 2);
+        float am = 0.f, gpm = 0.f;
 #pragma unroll
         for (int part = 0; part < 2; ++part) {
           const int row = 2 * r + part;
@@ -307,18 +310,12 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
           const float g = growth_pre<GF>(gm, gs, inv_s, inv_9s2, U);
           const float Gf = mul_nc(wk, g) * inv_wden;
           const float gp = maxnan(Gf, 0.f);
-          const float yc = float(((row - sh0) & (N - 1)) - N / 2);
-          sA += a;
-          sG += gp;
+          if (part) { sa1 += a; sg1 += gp; } else { sa0 += a; sg0 += gp; }
+          am += a;
+          gpm += gp;
           nA += (a > LX_EPS) ? 1u : 0u;
           nG += (gp > LX_EPS) ? 1u : 0u;
           nU += (U > LX_EPS) ? 1u : 0u;
-          sAy0 = fmaf(a, yc, sAy0);
-          sAy1 = fmaf(a, xc, sAy1);
-          sGy0 = fmaf(gp, yc, sGy0);
-          sGy1 = fmaf(gp, xc, sGy1);
-          sAyy0 = fmaf(a * yc, yc, sAyy0);
-          sAyy1 = fmaf(a * xc, xc, sAyy1);
           if constexpr (HIST) {
             if (p.cells_hist) p.cells_hist[hist_off + row * N + col] = a;
             if (p.field_hist) p.field_hist[hist_off + row * N + col] = Gf;
@@ -327,8 +324,17 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
           const float an = state_update(p.state_fn, a, dtw, Gf);
           if (part) a1[m] = an; else a0[m] = an;
         }
+        sAy1 = fmaf(am, xc, sAy1);
+        sGy1 = fmaf(gpm, xc, sGy1);
+        sAyy1 = fmaf(am * xc, xc, sAyy1);
         if ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
       }
+      const float yc0 = float(((2 * r - sh0) & (N - 1)) - N / 2);
+      const float yc1 = float(((2 * r + 1 - sh0) & (N - 1)) - N / 2);
+      const float sA = sa0 + sa1, sG = sg0 + sg1;
+      const float sAy0 = fmaf(sa0, yc0, sa1 * yc1);
+      const float sGy0 = fmaf(sg0, yc0, sg1 * yc1);
+      const float sAyy0 = fmaf(sa0 * yc0, yc0, sa1 * yc1 * yc1);
       const float vals[8] = {sA, sG, sAy0, sAy1, sGy0, sGy1, sAyy0, sAyy1};
       constexpr int slot[8] = {0, 1, 5, 6, 7, 8, 9, 10};
 #pragma unroll
