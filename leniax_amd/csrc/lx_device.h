@@ -352,6 +352,90 @@ LX_DEV void fft_lanes(float2* x, int t, int lane, const float2* tw) {
   }
 }
 
+// Length-(TPR*P) FFT over a group of TPR lanes whose cross-lane step is one transpose through
+// the group's LDS scratch instead of a DPP network (P a multiple of TPR, J = P / TPR):
+//   X[k1 + P k2] = sum_t W_TPR^(t k2) W_L^(t k1) Y_t[k1],  Y_t = the in-register FFT of lane t.
+// Lane t writes W_L^(t k1) Y_t[k1] to row t of the scratch, then owns J values of k1
+// (k1 = J t + j) and transforms them over the TPR rows in registers.
+//  forward: in  x[m] = data[TPR m + t];  out x[j TPR + k2] = DATA[(J t + j) + P k2]
+//  inverse: in  x[j TPR + k2] = DATA[(J t + j) + P k2];  out x[m] = data[TPR m + t] (x TPR P)
+// ``tw`` = exp(-2 pi i j / (TPR P)); ``xch``: TPR rows of XRS(P) complex, private to the
+// group.  The group lives in one wave, whose LDS accesses complete in issue order; the
+// wavefront-scope fences keep the compiler from moving them across the exchange.
+constexpr int xch_row(int P) { return P + 2; }  // row stride: 16-byte aligned pairs, rotated banks
+
+template <int P, int TPR, bool INV>
+LX_DEV void fft_xlds(float2* x, int t, const float2* tw, float2* xch) {
+  constexpr int J = P / TPR, RS = xch_row(P);
+  static_assert(P % TPR == 0 && J >= 1, "fft_xlds needs P >= TPR");
+  if constexpr (!INV) {
+    fft_reg<P, false>(x);
+    int j = 0;
+#pragma unroll
+    for (int k = 1; k < P; ++k) {
+      j += t;  // t * k < TPR * P
+      x[k] = cmul(x[k], tw[j]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+#pragma unroll
+    for (int k = 0; k < P; k += 2) {
+      const float4 v = make_float4(x[k].x, x[k].y, x[k + 1].x, x[k + 1].y);
+      *reinterpret_cast<float4*>(&xch[t * RS + k]) = v;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+    for (int r = 0; r < TPR; ++r) {
+      if constexpr (J % 2 == 0) {
+#pragma unroll
+        for (int j2 = 0; j2 < J; j2 += 2) {
+          const float4 v = *reinterpret_cast<const float4*>(&xch[r * RS + J * t + j2]);
+          x[j2 * TPR + r] = make_float2(v.x, v.y);
+          x[(j2 + 1) * TPR + r] = make_float2(v.z, v.w);
+        }
+      } else {
+#pragma unroll
+        for (int j2 = 0; j2 < J; ++j2) x[j2 * TPR + r] = xch[r * RS + J * t + j2];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int j2 = 0; j2 < J; ++j2) fft_reg<TPR, false>(x + j2 * TPR);
+  } else {
+#pragma unroll
+    for (int j2 = 0; j2 < J; ++j2) fft_reg<TPR, true>(x + j2 * TPR);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+#pragma unroll
+    for (int r = 0; r < TPR; ++r) {
+      if constexpr (J % 2 == 0) {
+#pragma unroll
+        for (int j2 = 0; j2 < J; j2 += 2) {
+          const float4 v = make_float4(x[j2 * TPR + r].x, x[j2 * TPR + r].y, x[(j2 + 1) * TPR + r].x,
+                                       x[(j2 + 1) * TPR + r].y);
+          *reinterpret_cast<float4*>(&xch[r * RS + J * t + j2]) = v;
+        }
+      } else {
+#pragma unroll
+        for (int j2 = 0; j2 < J; ++j2) xch[r * RS + J * t + j2] = x[j2 * TPR + r];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+    for (int k = 0; k < P; k += 2) {
+      const float4 v = *reinterpret_cast<const float4*>(&xch[t * RS + k]);
+      x[k] = make_float2(v.x, v.y);
+      x[k + 1] = make_float2(v.z, v.w);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int j = 0;
+#pragma unroll
+    for (int k = 1; k < P; ++k) {
+      j += t;
+      x[k] = cmulc(x[k], tw[j]);
+    }
+    fft_reg<P, true>(x);
+  }
+}
+
 // ----------------------------------------------------------------- growth functions
 // Ids follow the registry order of leniax/growth_functions.py:256-264.
 enum : int {

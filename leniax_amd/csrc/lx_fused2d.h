@@ -7,8 +7,9 @@
 // queue.  The state never leaves the chip between steps.
 //
 // Lane groups of TPR lanes own one length-N transform (P = N / TPR points per
-// lane, fft_lanes: in-register FFT + a radix-2 network across the group through
-// DPP).  Thread (r, t) -- r = tid / TPR, t = tid % TPR -- holds rows 2r and 2r+1
+// lane, fft_xlds: in-register FFT, twiddles, one transpose through the group's LDS
+// scratch, in-register DFTs across the group; spectra come out with lane t holding
+// frequencies (J t + j) + P k2, J = P / TPR).  Thread (r, t) holds rows 2r and 2r+1
 // of the state at columns TPR*m + t.  A step:
 //   rows fwd  two real rows packed as one complex row -> Z (LDS)
 //   cols fwd  unpack Z into the half spectrum column c = r (columns 0 and N/2
@@ -16,7 +17,8 @@
 //   cols inv  inverse FFT -> Y (LDS)
 //   rows inv  repack rows 2r, 2r+1 from their half spectra, inverse FFT -> U;
 //             growth, weighted field, clip update, statistics sums in registers.
-// LDS: one complex buffer of N*N/2 (+padding) elements reused by both transposes.
+// LDS: one complex buffer of N*N/2 (+padding) elements reused by both transposes, and
+// one TPR x (P + 2) exchange scratch per lane group.
 #pragma once
 #include "lx_internal.h"
 
@@ -35,6 +37,9 @@ struct F2 {
   static constexpr int YSZ = N * YRS;
   static constexpr int BUF = ZSZ > YSZ ? ZSZ : YSZ;
   static constexpr int NRED = 12;
+  static constexpr int J = P / TPR;              // frequencies k1 per lane after the exchange
+  static constexpr int XRS = xch_row(P);         // exchange scratch row stride
+  static constexpr int XG = TPR * XRS;           // exchange scratch per lane group
   LX_DEV static int zidx(int hp, int k) { return hp * ZRS + (k / P) * (P + 1) + (k % P); }
   LX_DEV static int ycol(int c) { return c + (c >= N / 4 ? 1 : 0); }
 };
@@ -104,6 +109,7 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
   constexpr int KS = HALF + 1;  // half-spectrum row length
   constexpr bool SPREAD = (TPR == 8);
   __shared__ float2 buf[G::BUF];
+  __shared__ __align__(16) float2 xch_all[(NT / TPR) * G::XG];
   __shared__ float2 tw[N];
   __shared__ float red[NW][G::NRED];
   __shared__ int ctrl[4];
@@ -114,8 +120,9 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
   // {0..3, 8..11} + 4*(r & 1) + 16*(r >> 1), so that t ^ 4 is lane ^ 8 (one DPP rotate)
   const int t0 = SPREAD ? ((tid & 3) | (((tid >> 3) & 1) << 2)) : (tid % TPR);
   const int r0 = SPREAD ? (((tid >> 2) & 1) | ((tid >> 4) << 1)) : (tid / TPR);
-  const int s0 = bitrev_n(t0, TPR);
   const int lane = tid & 63, wave = tid >> 6;
+  constexpr int J = G::J;
+  float2* const xch = xch_all + r0 * G::XG;
   const float inv_n2 = 1.f / float(N * N);
   const int64_t cells_per_world = int64_t(N) * N;
 
@@ -163,16 +170,18 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
       // The lane indices are laundered through an empty asm each step so that the
       // per-lane LDS/global addresses of the transposes are recomputed inside the
       // loop instead of being hoisted out of it and pinned in registers.
-      int r = r0, q = t0, s = s0;
-      __asm__ volatile("" : "+v"(r), "+v"(q), "+v"(s));
+      int r = r0, q = t0;
+      __asm__ volatile("" : "+v"(r), "+v"(q));
       const int sh0 = ctrl[2], sh1 = ctrl[3];
 
       // ---- rows forward: packed row r = A[2r] + i A[2r+1]
 #pragma unroll
       for (int m = 0; m < P; ++m) x[m] = make_float2(a0[m], a1[m]);
-      fft_lanes<P, TPR, false, SPREAD>(x, fresh(q), lane, tw);  // x[k1] = Z_r[k1 + P s]
+      fft_xlds<P, TPR, false>(x, fresh(q), tw, xch);  // x[j TPR + k2] = Z_r[(J q + j) + P k2]
 #pragma unroll
-      for (int k1 = 0; k1 < P; ++k1) buf[r * G::ZRS + s * (P + 1) + k1] = x[k1];
+      for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int k2 = 0; k2 < TPR; ++k2) buf[r * G::ZRS + k2 * (P + 1) + J * q + j] = x[j * TPR + k2];
       __syncthreads();
 
       // ---- columns forward: c = r, rows h = TPR m + q; unpack the packed rows:
@@ -205,90 +214,77 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
           }
         }
       }
-      fft_lanes<P, TPR, false, SPREAD>(x, fresh(q), lane, tw);  // x[k1] = F[kh = k1 + P s][c]
+      fft_xlds<P, TPR, false>(x, fresh(q), tw, xch);  // x[j TPR + k2] = F[kh = (J q + j) + P k2][c]
 
       // ---- multiply by the half spectrum; column 0 carries columns 0 and N/2:
-      //      Q[f] = X[f] a[f] + conj(X[-f]) b[f], a = (K0 + KN)/2, b = (K0 - KN)/2
+      //      Q[f] = X[f] a[f] + conj(X[-f]) b[f], a = (K0 + KN)/2, b = (K0 - KN)/2.
+      //      X[-f] lives in another lane of the group: column 0 goes through its scratch.
       {
         const float2* kcol = kh + r;
-        if (wave != 0) {
+        if (wave == 0 && r == 0) {
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 #pragma unroll
-          for (int k1 = 0; k1 < P; ++k1) {
-            x[k1] = cmul(x[k1], kcol[(k1 + P * s) * KS]);
-            if ((k1 & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-          }
-        } else {
-          const bool c0 = (r == 0);
-          {
-            // X[-f] for k1 = 0 sits in block (TPR - s) mod TPR, register 0
-            const int src = group_lane<TPR>(lane, bitrev_n((TPR - s) & (TPR - 1), TPR));
-            const float2 pm = make_float2(__shfl(x[0].x, src, 64), __shfl(x[0].y, src, 64));
-            const int row = P * s;
-            const float2 kv = kcol[row * KS];
-            const float2 kn = kh[row * KS + HALF];
-            if (c0) {
+          for (int j = 0; j < J; ++j)
+#pragma unroll
+            for (int k2 = 0; k2 < TPR; ++k2) xch[J * q + j + P * k2] = x[j * TPR + k2];
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+          for (int j = 0; j < J; ++j) {
+#pragma unroll
+            for (int k2 = 0; k2 < TPR; ++k2) {
+              const int f = J * q + j + P * k2;
+              const float2 pm = xch[(N - f) & (N - 1)];
+              const float2 kv = kh[f * KS], kn = kh[f * KS + HALF];
               const float2 a = cscale(cadd(kv, kn), 0.5f), b = cscale(csub(kv, kn), 0.5f);
-              x[0] = cadd(cmul(x[0], a), cmul(cconj(pm), b));
-            } else {
-              x[0] = cmul(x[0], kv);
+              x[j * TPR + k2] = cadd(cmul(x[j * TPR + k2], a), cmul(cconj(pm), b));
             }
           }
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
 #pragma unroll
-          for (int k1 = 1; k1 <= P / 2; ++k1) {
-            // X[-f] for k1 >= 1 sits in lane t ^ (TPR-1), register P - k1
-            const int k2 = P - k1;
-            const float2 p1 = group_mirror<TPR>(x[k2]);
-            const float2 p2 = group_mirror<TPR>(x[k1]);
-            const int row1 = k1 + P * s, row2 = k2 + P * s;
-            const float2 kv1 = kcol[row1 * KS], kn1 = kh[row1 * KS + HALF];
-            const float2 kv2 = kcol[row2 * KS], kn2 = kh[row2 * KS + HALF];
-            float2 q1, q2;
-            if (c0) {
-              q1 = cadd(cmul(x[k1], cscale(cadd(kv1, kn1), 0.5f)), cmul(cconj(p1), cscale(csub(kv1, kn1), 0.5f)));
-              q2 = cadd(cmul(x[k2], cscale(cadd(kv2, kn2), 0.5f)), cmul(cconj(p2), cscale(csub(kv2, kn2), 0.5f)));
-            } else {
-              q1 = cmul(x[k1], kv1);
-              q2 = cmul(x[k2], kv2);
-            }
-            x[k1] = q1;
-            if (k2 != k1) x[k2] = q2;
+          for (int j = 0; j < J; ++j) {
+#pragma unroll
+            for (int k2 = 0; k2 < TPR; ++k2) x[j * TPR + k2] = cmul(x[j * TPR + k2], kcol[(J * q + j + P * k2) * KS]);
+            if ((j & 1) == 1) __builtin_amdgcn_sched_barrier(0);
           }
         }
       }
 
       // ---- columns inverse -> Y
-      fft_lanes<P, TPR, true, SPREAD>(x, fresh(q), lane, tw);  // x[m] = column c at row h = TPR m + q (x N)
+      fft_xlds<P, TPR, true>(x, fresh(q), tw, xch);  // x[m] = column c at row h = TPR m + q (x N)
       __syncthreads();  // every column gather of Z is done before Y overwrites it
 #pragma unroll
       for (int m = 0; m < P; ++m) buf[(TPR * m + q) * G::YRS + G::ycol(r)] = x[m];
       __syncthreads();
 
-      // ---- rows inverse: repack rows 2r, 2r+1 from their half spectra, k = k1 + P s:
+      // ---- rows inverse: repack rows 2r, 2r+1 from their half spectra, k = (J q + j) + P k2:
       //      k < N/2 read directly, k > N/2 through conj(Y[N-k]); k = 0 and N/2 are real
       {
-        const bool mirror = s >= TPR / 2;
-        const float sg = mirror ? -1.f : 1.f;
-        const int dir = mirror ? -1 : 1;
-        const int kbase = mirror ? N - P * s : P * s;
         const int rowa = (2 * r) * G::YRS;
 #pragma unroll
-        for (int k1 = 0; k1 < P; ++k1) {
-          int col = kbase + dir * k1;
-          if (k1 == 0 && (s == 0 || s == TPR / 2)) col = 0;
-          const int ia = rowa + G::ycol(col);
-          const float2 ya = buf[ia];
-          const float2 yb = buf[ia + G::YRS];
-          float2 z = make_float2(fmaf(-sg, yb.y, ya.x), fmaf(sg, ya.y, yb.x));
-          if (k1 == 0) {
-            const bool d0 = s == 0, dn = s == TPR / 2;
-            z.x = d0 ? ya.x : (dn ? ya.y : z.x);
-            z.y = d0 ? yb.x : (dn ? yb.y : z.y);
+        for (int j = 0; j < J; ++j) {
+#pragma unroll
+          for (int k2 = 0; k2 < TPR; ++k2) {
+            const bool mirror = k2 >= TPR / 2;
+            const float sg = mirror ? -1.f : 1.f;
+            const int k = J * q + j + P * k2;
+            int col = mirror ? N - k : k;
+            const bool edge = (j == 0) && (k2 == 0 || k2 == TPR / 2) && q == 0;  // k = 0 or N/2
+            if (j == 0 && k2 == TPR / 2) col = q == 0 ? 0 : col;
+            const int ia = rowa + G::ycol(col);
+            const float2 ya = buf[ia];
+            const float2 yb = buf[ia + G::YRS];
+            float2 z = make_float2(fmaf(-sg, yb.y, ya.x), fmaf(sg, ya.y, yb.x));
+            if (j == 0 && (k2 == 0 || k2 == TPR / 2)) {
+              const float2 zr = k2 == 0 ? make_float2(ya.x, yb.x) : make_float2(ya.y, yb.y);
+              z = edge ? zr : z;
+            }
+            x[j * TPR + k2] = z;
           }
-          x[k1] = z;
-          if ((k1 & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+          if ((j & 1) == 1) __builtin_amdgcn_sched_barrier(0);
         }
       }
-      fft_lanes<P, TPR, true, SPREAD>(x, fresh(q), lane, tw);  // x[m] = (U[2r][TPR m+q], U[2r+1][TPR m+q]) * N^2
+      fft_xlds<P, TPR, true>(x, fresh(q), tw, xch);  // x[m] = (U[2r][TPR m+q], U[2r+1][TPR m+q]) * N^2
 
       // ---- growth, field, update, statistics sums.  A lane owns rows 2r, 2r+1 at columns
       //      TPR m + q: row moments factor into per-row sums, column moments into per-column
