@@ -265,16 +265,19 @@ def test_gradient_api_nonzero_like_reference():
         ((out - 0.5) ** 2).sum().backward()
         grads[name] = args[i].grad
         val = grads[name].real.sum() if name == 'K' else grads[name].sum()
-        assert float(val) != 0.0, name
+        if name != 'w':  # one kernel, weighted mean: G = w g / w, so dL/dw is exactly 0
+            assert float(val) != 0.0, name
     rule = L.Rule([g for sub in mapping.cin_gfs for g in sub])
     A = _np(cells).astype(np.float64)
     Kn = _np(K)
     ref_new = rule.update(A, Kn.astype(np.complex128), _np(gf).astype(np.float64), _np(w).astype(np.float64), dt)[0]
     ref = ADJ.step_vjp(rule, A, Kn, _np(gf), _np(w), dt, 2 * (ref_new - 0.5))
-    for name, key in (('cells', 'state'), ('K', 'K'), ('gf', 'gf_params'), ('w', 'weights')):
+    for name, key in (('cells', 'state'), ('K', 'K'), ('gf', 'gf_params')):
         err = _rel(_np(grads[name]), ref[key])
         print(f'  {name}: rel err {err:.2e}')
         assert err < 1e-4, name
+    # leniax's test sees a non-zero sum here only through float32 rounding of two equal terms
+    assert abs(float(_np(grads['w']).sum()) - float(ref['weights'].sum())) <= 1e-6 * np.abs(ref['gf_params']).max()
 
 
 def test_field_and_state_gradients_like_reference():
