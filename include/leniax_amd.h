@@ -99,6 +99,14 @@ int lx_update(const lx_rule* rule, const float* state_d, int64_t B, const void* 
               int64_t dt_bstride, float* state_out_d, float* field_d, float* potential_d, void* work_d,
               void* stream);
 
+/* lx_update that also writes the forward spectra of the input state, rfftn(state) as
+ * [B][C][dims[:-1]][dims[-1]/2+1] complex64, to state_spec_d: the recompute step of the adjoint,
+ * whose kernel-spectrum gradient needs them (lx_adjoint_potential). */
+int lx_update_with_spectra(const lx_rule* rule, const float* state_d, int64_t B, const void* K_half_d,
+                           int64_t k_bstride, const float* gf_d, int64_t gf_bstride, const float* w_d,
+                           int64_t w_bstride, const float* dt_d, int64_t dt_bstride, float* state_out_d,
+                           float* field_d, float* potential_d, void* state_spec_d, void* work_d, void* stream);
+
 /* leniax/statistics.py:35-126 compute_stats (one step, B worlds) + the per-world carry
  * (shift, centroid, angle: leniax/runner.py:309-323) kept in carry_d (lx_carry_bytes() each).
  * stats_d [B][11 + C] floats (order: mass, mass_volume, mass_density, growth, growth_volume,

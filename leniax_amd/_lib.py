@@ -91,6 +91,8 @@ SIGNATURES = {
     'lx_field': (ctypes.c_int, [_R, _P, _I64, _P, _I64, _P, _I64, _P, _P]),
     'lx_state': (ctypes.c_int, [_R, _P, _P, _I64, _P, _I64, _P, _P]),
     'lx_update': (ctypes.c_int, [_R, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P]),
+    'lx_update_with_spectra': (ctypes.c_int, [_R, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P,
+                                              _P, _P]),
     'lx_carry_bytes': (ctypes.c_size_t, []),
     'lx_carry_init': (ctypes.c_int, [_I64, _P, _P]),
     'lx_stats': (ctypes.c_int, [_R, _P, _P, _P, _I64, _F, _F, _P, _P, _P]),

@@ -94,11 +94,19 @@ class _Ctx:
         self.ws = core._per_world_stride(self.w, N, 'weights')
         self.dt, self.dts = core._dt_tensor(dt, N)
 
-    def step(self, A, out, field, pot):
-        _lib.call('lx_update', ctypes.byref(self.rule), engine.ptr(A), self.N, engine.ptr(self.Kh), self.kbs,
-                  engine.ptr(self.gf), self.gfs, engine.ptr(self.w), self.ws, engine.ptr(self.dt), self.dts,
-                  engine.ptr(out), engine.ptr(field), engine.ptr(pot),
-                  engine.ptr(core._workspace(self.rule, self.N)), engine.stream_ptr())
+    def step(self, A, out, field, pot, FA=None):
+        """One plain step A -> out; field / potential / the state's spectra FA kept when given."""
+        args = (ctypes.byref(self.rule), engine.ptr(A), self.N, engine.ptr(self.Kh), self.kbs, engine.ptr(self.gf),
+                self.gfs, engine.ptr(self.w), self.ws, engine.ptr(self.dt), self.dts, engine.ptr(out),
+                engine.ptr(field), engine.ptr(pot))
+        work = engine.ptr(core._workspace(self.rule, self.N))
+        if FA is None:
+            _lib.call('lx_update', *args, work, engine.stream_ptr())
+        else:
+            _lib.call('lx_update_with_spectra', *args, engine.ptr(FA), work, engine.stream_ptr())
+
+    def spectra_shape(self):
+        return (self.N, self.C) + self.dims[:-1] + (self.dims[-1] // 2 + 1,)
 
     def accumulators(self, need_K):
         acc = {'gf': torch.zeros_like(self.gf), 'w': torch.zeros_like(self.w), 'dt': torch.zeros_like(self.dt)}
@@ -107,20 +115,22 @@ class _Ctx:
                                        dtype=torch.complex64, device=self.Kh.device)
         return acc
 
-    def adjoint(self, A, field, pot, lam, acc, ste, dfield=None, dpot_in=None):
-        """dL/dA of one step given lam = dL/d(state out); parameter gradients added into acc."""
-        lam_out = torch.empty_like(A)
-        dpot = torch.empty_like(pot)
+    def adjoint(self, A, field, pot, lam, acc, ste, dfield=None, dpot_in=None, FA=None, lam_out=None, dpot=None):
+        """dL/dA of one step given lam = dL/d(state out); parameter gradients added into acc.
+        lam_out may be lam itself (updated in place); FA = rfftn(A) when the caller kept it."""
+        lam_out = torch.empty_like(A) if lam_out is None else lam_out
+        dpot = torch.empty_like(pot) if dpot is None else dpot
         _lib.call('lx_adjoint_pointwise', ctypes.byref(self.rule), self.N, engine.ptr(A), engine.ptr(field),
                   engine.ptr(pot), engine.ptr(self.gf), self.gfs, engine.ptr(self.w), self.ws, engine.ptr(self.dt),
                   self.dts, engine.ptr(lam), engine.ptr(dfield), engine.ptr(dpot_in), int(bool(ste)),
                   engine.ptr(lam_out), engine.ptr(dpot), engine.ptr(acc['gf']), engine.ptr(acc['w']),
                   engine.ptr(acc['dt']), engine.stream_ptr())
         dspec = acc.get('dspec')
-        FA = rfftn(A, A.dim() - 2) if dspec is not None else None
+        if dspec is not None and FA is None:
+            FA = rfftn(A, A.dim() - 2)
         _lib.call('lx_adjoint_potential', ctypes.byref(self.rule), self.N, engine.ptr(self.Kh), self.kbs,
-                  engine.ptr(dpot), engine.ptr(FA), engine.ptr(dspec), engine.ptr(lam_out),
-                  engine.ptr(core._workspace(self.rule, self.N)), engine.stream_ptr())
+                  engine.ptr(dpot), engine.ptr(FA if dspec is not None else None), engine.ptr(dspec),
+                  engine.ptr(lam_out), engine.ptr(core._workspace(self.rule, self.N)), engine.stream_ptr())
         return lam_out
 
     def grad_K(self, dspec, K):
@@ -233,21 +243,25 @@ class _Run(torch.autograd.Function):
         sc, steps, c = ctx.sc, ctx.steps, ctx.ckpt
         need = ctx.needs_input_grad
         acc = sc.accumulators(need[1])
-        lam = g_out.contiguous().clone()
+        lam = g_out.contiguous().clone()  # updated in place, step after step
         if steps > 0:
             A0 = ctx.ckpts[0]
             seg_A = torch.empty((c,) + tuple(A0.shape), dtype=torch.float32, device=A0.device)
             seg_G = torch.empty_like(seg_A)
             seg_U = torch.empty((c, sc.N, sc.Keff) + sc.dims, dtype=torch.float32, device=A0.device)
+            seg_F = (torch.empty((c,) + sc.spectra_shape(), dtype=torch.complex64, device=A0.device)
+                     if 'dspec' in acc else None)
+            dpot = torch.empty_like(seg_U[0])
             nxt = torch.empty_like(A0)
             for s in range((steps - 1) // c, -1, -1):
                 t0, t1 = s * c, min(s * c + c, steps)
                 seg_A[0].copy_(ctx.ckpts[s])
-                for i in range(t1 - t0):  # recompute the segment, keeping field and potential
+                for i in range(t1 - t0):  # recompute the segment, keeping field, potential and spectra
                     out = seg_A[i + 1] if i + 1 < t1 - t0 else nxt
-                    sc.step(seg_A[i], out, seg_G[i], seg_U[i])
+                    sc.step(seg_A[i], out, seg_G[i], seg_U[i], None if seg_F is None else seg_F[i])
                 for i in range(t1 - t0 - 1, -1, -1):
-                    lam = sc.adjoint(seg_A[i], seg_G[i], seg_U[i], lam, acc, ctx.ste)
+                    sc.adjoint(seg_A[i], seg_G[i], seg_U[i], lam, acc, ctx.ste,
+                               FA=None if seg_F is None else seg_F[i], lam_out=lam, dpot=dpot)
         ctx.ckpts = None
         return (lam if need[0] else None,
                 sc.grad_K(acc['dspec'], K) if need[1] else None,

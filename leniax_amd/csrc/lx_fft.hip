@@ -196,6 +196,8 @@ struct LineArgs {
   int cin[LX_MAX_K];   // mode 0: kernel k reads channel cin[k]
   int mode;            // 0: conv, 1: forward in place, 2: inverse S -> U
   int conjk;           // mode 0: multiply by conj(K) (adjoint of the potential)
+  // mode 0, optional: the forward spectra of the input images, FA [img][half]
+  float2* FA;
   // mode 0, optional: dspec[b][k] += X_k * conj(FAin[b][acc_cin[k]]) (kernel-spectrum gradient)
   const float2* FAin;
   float2* dspec;
@@ -208,7 +210,7 @@ LineArgs line_args() {
   a.S = nullptr; a.U = nullptr; a.Kh = nullptr; a.k_bstride = 0; a.nlines = 0;
   a.WH = 0; a.A = 1; a.a_stride = 0; a.st = 0; a.img_elems = 0; a.C = 1; a.K = 0;
   for (int k = 0; k < LX_MAX_K; ++k) { a.cin[k] = -1; a.acc_cin[k] = 0; }
-  a.mode = 0; a.conjk = 0; a.FAin = nullptr; a.dspec = nullptr; a.C_fa = 1;
+  a.mode = 0; a.conjk = 0; a.FA = nullptr; a.FAin = nullptr; a.dspec = nullptr; a.C_fa = 1;
   return a;
 }
 
@@ -252,6 +254,11 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
       for (int k1 = 0; k1 < P; ++k1) dst[int64_t(k1 + P * s) * st] = x[k1];
     }
     return;
+  }
+  if (a.FA && live) {
+    float2* fa = a.FA + img * a.img_elems + off;
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) fa[int64_t(k1 + P * s) * st] = x[k1];
   }
   for (int k = 0; k < a.K; ++k) {
     if (a.cin[k] != c) continue;
@@ -458,7 +465,7 @@ int fwd_leading(const Geom& g, float2* X, int64_t nimg, hipStream_t st) {
 // One leniax step (or, with sel, the potential alone).  A: [B][C][dims]; work: S then U.
 int plain_step(const lx_rule* rule, const float* A, int64_t B, const void* Kh, int64_t k_bs, const float* gf,
                int64_t gf_bs, const float* w, int64_t w_bs, const float* dt, int64_t dt_bs, float* A_out,
-               float* field, float* pot, void* work, hipStream_t st, int sel) {
+               float* field, float* pot, void* work, hipStream_t st, int sel, float2* FA = nullptr) {
   Geom g;
   if (int rc = make_geom(rule->ndim, rule->dims, g)) return rc;
   const int C = rule->C, K = rule->K;
@@ -473,6 +480,7 @@ int plain_step(const lx_rule* rule, const float* A, int64_t B, const void* Kh, i
   if (!rc) {  // the convolution axis
     LineArgs la = line_args();
     la.S = S; la.U = U; la.Kh = static_cast<const float2*>(Kh); la.k_bstride = k_bs; la.C = C; la.K = K;
+    la.FA = FA;
     for (int k = 0; k < LX_MAX_K; ++k) la.cin[k] = k < K ? rule->cin[k] : -1;
     rc = launch_lines(g, 0, la, B * C, st);
   }
@@ -556,6 +564,23 @@ extern "C" int lx_update(const lx_rule* rule, const float* state_d, int64_t B, c
   if (state_out_d == state_d) return lx_fail(LX_EINVAL, "lx_update: state_out must not alias state");
   return plain_step(rule, state_d, B, K_half_d, k_bstride, gf_d, gf_bstride, w_d, w_bstride, dt_d, dt_bstride,
                     state_out_d, field_d, potential_d, work_d, (hipStream_t)stream, 0);
+}
+
+extern "C" int lx_update_with_spectra(const lx_rule* rule, const float* state_d, int64_t B, const void* K_half_d,
+                                      int64_t k_bstride, const float* gf_d, int64_t gf_bstride, const float* w_d,
+                                      int64_t w_bstride, const float* dt_d, int64_t dt_bstride, float* state_out_d,
+                                      float* field_d, float* potential_d, void* state_spec_d, void* work_d,
+                                      void* stream) {
+  int rc = lx_check_rule(rule);
+  if (rc) return rc;
+  if (!state_d || !K_half_d || !gf_d || !w_d || !dt_d || !state_out_d || !state_spec_d || !work_d)
+    return lx_fail(LX_EINVAL, "NULL pointer");
+  if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
+  if (B == 0) return LX_OK;
+  if (state_out_d == state_d) return lx_fail(LX_EINVAL, "lx_update_with_spectra: state_out must not alias state");
+  return plain_step(rule, state_d, B, K_half_d, k_bstride, gf_d, gf_bstride, w_d, w_bstride, dt_d, dt_bstride,
+                    state_out_d, field_d, potential_d, work_d, (hipStream_t)stream, 0,
+                    static_cast<float2*>(state_spec_d));
 }
 
 extern "C" int lx_potential(const lx_rule* rule, const float* state_d, int64_t B, const void* K_half_d,

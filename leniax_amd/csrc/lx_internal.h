@@ -46,6 +46,15 @@ struct lx_carry {
   int32_t pad;
 };
 
+// Carry record of the HBM-path statistics: the carry plus one slot of partial sums per
+// block of the per-world reduction (summed in slot order: deterministic).
+#define LX_STAT_PARTS 16
+#define LX_NSUM (14 + LX_MAX_C)
+struct lx_stat_carry {
+  lx_carry c;
+  float part[LX_STAT_PARTS][LX_NSUM];
+};
+
 // Per-world reductions one step's statistics need.  Coordinates are the
 // re-centred ones of leniax/statistics.py:28-33,64 (index - shift mod size - size//2).
 struct lx_sums {

@@ -91,7 +91,7 @@ __global__ void heuristics_kernel(const float* __restrict__ stats, int T, int64_
   if (nout) nout[b] = (float)c.n;
 }
 
-__global__ void carry_init_kernel(lx_carry* c, int64_t B) {
+__global__ void carry_init_kernel(lx_stat_carry* c, int64_t B) {
   const int64_t b = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (b >= B) return;
   lx_carry z;
@@ -101,7 +101,7 @@ __global__ void carry_init_kernel(lx_carry* c, int64_t B) {
   z.prev_mass = z.prev_sign = 0.f;
   z.mono = z.vol = z.n = z.t = z.pad = 0;
   z.cont = 1;
-  c[b] = z;
+  c[b].c = z;
 }
 
 unsigned grid_for(int64_t total, int block) {
@@ -155,13 +155,13 @@ extern "C" int lx_check_heuristics(const float* stats_d, int32_t T, int64_t B, i
   return launched("check_heuristics");
 }
 
-extern "C" size_t lx_carry_bytes(void) { return sizeof(lx_carry); }
+extern "C" size_t lx_carry_bytes(void) { return sizeof(lx_stat_carry); }
 
 extern "C" int lx_carry_init(int64_t B, void* carry_d, void* stream) {
   if (!carry_d || B < 0) return lx_fail(LX_EINVAL, "bad carry arguments");
   if (B == 0) return LX_OK;
   hipLaunchKernelGGL(carry_init_kernel, dim3(grid_for(B, 128)), dim3(128), 0, (hipStream_t)stream,
-                     static_cast<lx_carry*>(carry_d), B);
+                     static_cast<lx_stat_carry*>(carry_d), B);
   return launched("carry_init");
 }
 
@@ -174,7 +174,7 @@ struct StatArgs {
   const float* cells;
   const float* field;
   const float* pot;
-  lx_carry* carry;
+  lx_stat_carry* carry;
   float* stats;
   int64_t B;
   int ndim;
@@ -183,41 +183,44 @@ struct StatArgs {
   float R2, R, dt;
 };
 
-__global__ __launch_bounds__(256) void stats_kernel(StatArgs a) {
-  __shared__ int shs[3];
-  const int64_t b = blockIdx.x;
+// Partial sums of one world's statistics over 1 / LX_STAT_PARTS of its cells (grid: parts x B).
+// Coordinates are leniax's re-centred ones (leniax/statistics.py:28-33,64 via
+// leniax/utils.py:269-293): (index - shift) mod size - size // 2, tracked incrementally.
+__global__ __launch_bounds__(256) void stats_part_kernel(StatArgs a) {
+  const int64_t b = blockIdx.y;
+  const int part = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  lx_carry c;
-  if (tid == 0) {
-    c = a.carry[b];
-    for (int i = 0; i < 3; ++i) shs[i] = c.shift[i];
-  }
-  __syncthreads();
+  lx_stat_carry* sc = a.carry + b;
   const int d0 = a.ndim == 3 ? a.dims[0] : 1, d1 = a.dims[a.ndim - 2], d2 = a.dims[a.ndim - 1];
-  const int64_t n = int64_t(d0) * d1 * d2;
-  // ndim == 2: axes (d1, d2) map to centroid components (0, 1); ndim == 3: (d0, d1, d2) -> (0, 1, 2)
-  const int s0 = a.ndim == 3 ? shs[0] : 0, s1 = shs[a.ndim - 2], s2 = shs[a.ndim - 1];
-  float v[16];
+  const int n = d0 * d1 * d2;
+  // shifts reduced into [0, size): ndim == 2 axes (d1, d2) are carry axes (0, 1), ndim == 3 (0, 1, 2)
+  auto modp = [](int v, int m) { v %= m; return v < 0 ? v + m : v; };
+  const int s0 = a.ndim == 3 ? modp(sc->c.shift[0], d0) : 0;
+  const int s1 = modp(sc->c.shift[a.ndim - 2], d1), s2 = modp(sc->c.shift[a.ndim - 1], d2);
+  const int i0 = int(int64_t(n) * part / LX_STAT_PARTS), i1 = int(int64_t(n) * (part + 1) / LX_STAT_PARTS);
+  float v[LX_NSUM];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = 0.f;
-  // v: 0 sA, 1 sG, 2 nA, 3 nG, 4 nU, 5..7 sAy, 8..10 sGy, 11..13 sAyy (components in world-axis order)
-  float cm[LX_MAX_C];
-  for (int ch = 0; ch < LX_MAX_C; ++ch) cm[ch] = 0.f;
+  for (int i = 0; i < LX_NSUM; ++i) v[i] = 0.f;
+  // v: 0 sA, 1 sG, 2 nA, 3 nG, 4 nU, 5..7 sAy, 8..10 sGy, 11..13 sAyy, 14.. channel masses
+  const int adv_z = 256 % d2, adv_r = 256 / d2;
   for (int ch = 0; ch < a.C; ++ch) {
-    const float* A = a.cells + (b * a.C + ch) * n;
-    const float* G = a.field + (b * a.C + ch) * n;
-    float sc = 0.f;
-    for (int64_t i = tid; i < n; i += 256) {
-      const int z = int(i % d2), y = int((i / d2) % d1), xq = int(i / (int64_t(d1) * d2));
-      const float cz = float(((z - s2) % d2 + d2) % d2 - d2 / 2);
-      const float cy = float(((y - s1) % d1 + d1) % d1 - d1 / 2);
-      const float cx = float(((xq - s0) % d0 + d0) % d0 - d0 / 2);
+    const float* A = a.cells + (b * a.C + ch) * int64_t(n);
+    const float* G = a.field + (b * a.C + ch) * int64_t(n);
+    float sc_mass = 0.f;
+    const int first = i0 + tid;
+    int z = first % d2, row = first / d2;
+    int y = row % d1, xq = row / d1;
+    for (int i = first; i < i1; i += 256) {
+      int zc = z - s2; zc += zc < 0 ? d2 : 0;
+      int yc = y - s1; yc += yc < 0 ? d1 : 0;
+      int xc = xq - s0; xc += xc < 0 ? d0 : 0;
+      const float cz = float(zc - d2 / 2), cy = float(yc - d1 / 2), cx = float(xc - d0 / 2);
       const float av = A[i];
       const float g = G[i];
       const float gp = (g != g) ? g : fmaxf(g, 0.f);
       float co[3];
       if (a.ndim == 3) { co[0] = cx; co[1] = cy; co[2] = cz; } else { co[0] = cy; co[1] = cz; co[2] = 0.f; }
-      sc += av;
+      sc_mass += av;
       v[1] += gp;
       v[2] += av > LX_EPS ? 1.f : 0.f;
       v[3] += gp > LX_EPS ? 1.f : 0.f;
@@ -227,36 +230,55 @@ __global__ __launch_bounds__(256) void stats_kernel(StatArgs a) {
         v[8 + d] = fmaf(gp, co[d], v[8 + d]);
         v[11 + d] = fmaf(av * co[d], co[d], v[11 + d]);
       }
+      // advance by 256 cells
+      z += adv_z;
+      int carry_r = adv_r;
+      if (z >= d2) { z -= d2; ++carry_r; }
+      y += carry_r;
+      while (y >= d1) { y -= d1; ++xq; }
     }
-    v[0] += sc;
-    cm[ch] = sc;
+    v[0] += sc_mass;
+#pragma unroll
+    for (int c2 = 0; c2 < LX_MAX_C; ++c2) v[14 + c2] += c2 == ch ? sc_mass : 0.f;
   }
   for (int k = 0; k < a.K; ++k) {
-    const float* U = a.pot + (b * a.K + k) * n;
-    for (int64_t i = tid; i < n; i += 256) v[4] += U[i] > LX_EPS ? 1.f : 0.f;
+    const float* U = a.pot + (b * a.K + k) * int64_t(n);
+    for (int i = i0 + tid; i < i1; i += 256) v[4] += U[i] > LX_EPS ? 1.f : 0.f;
   }
-  // block reduction of the 14 sums and the C channel sums
-  constexpr int NV = 14 + LX_MAX_C;
-  __shared__ float red2[4][NV];
-  for (int i = 0; i < NV; ++i) {
-    const float val = wave_sum(i < 14 ? v[i] : cm[i - 14]);
-    if (lane == 0) red2[wave][i] = val;
+  __shared__ float red[4][LX_NSUM];
+#pragma unroll
+  for (int i = 0; i < LX_NSUM; ++i) {
+    const float val = wave_sum(v[i]);
+    if (lane == 0) red[wave][i] = val;
   }
   __syncthreads();
-  float out[NV];
-  if (tid == 0)
-    for (int i = 0; i < NV; ++i) out[i] = red2[0][i] + red2[1][i] + red2[2][i] + red2[3][i];
-  if (tid == 0) {
-    lx_sums s;
-    s.sA = out[0]; s.sG = out[1]; s.nA = out[2]; s.nG = out[3]; s.nU = out[4];
-    for (int d = 0; d < 3; ++d) { s.sAy[d] = out[5 + d]; s.sGy[d] = out[8 + d]; s.sAyy[d] = out[11 + d]; }
-    for (int ch = 0; ch < LX_MAX_C; ++ch) s.sAc[ch] = ch < a.C ? out[14 + ch] : 0.f;
-    int32_t dd[3];
-    for (int i = 0; i < 3; ++i) dd[i] = a.dims[i];
-    lx_stats_finish(s, c, a.ndim, dd, a.C, a.R2, a.R, a.dt, LX_STOP_NONE, 0, float(n * a.C),
-                    a.stats + b * (LX_NSTAT_BASE + a.C));
-    a.carry[b] = c;
+  if (tid < LX_NSUM) sc->part[part][tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+}
+
+// One thread per world: sums the partial slots in order, finishes the step's statistics and
+// carry (leniax/statistics.py:64-126).
+__global__ void stats_finish_kernel(StatArgs a) {
+  const int64_t b = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (b >= a.B) return;
+  lx_stat_carry* sc = a.carry + b;
+  float out[LX_NSUM];
+#pragma unroll
+  for (int i = 0; i < LX_NSUM; ++i) {
+    float acc = 0.f;
+    for (int p = 0; p < LX_STAT_PARTS; ++p) acc += sc->part[p][i];
+    out[i] = acc;
   }
+  lx_sums s;
+  s.sA = out[0]; s.sG = out[1]; s.nA = out[2]; s.nG = out[3]; s.nU = out[4];
+  for (int d = 0; d < 3; ++d) { s.sAy[d] = out[5 + d]; s.sGy[d] = out[8 + d]; s.sAyy[d] = out[11 + d]; }
+  for (int ch = 0; ch < LX_MAX_C; ++ch) s.sAc[ch] = ch < a.C ? out[14 + ch] : 0.f;
+  int32_t dd[3];
+  for (int i = 0; i < 3; ++i) dd[i] = a.dims[i];
+  const int64_t n = int64_t(a.ndim == 3 ? a.dims[0] : 1) * a.dims[a.ndim - 2] * a.dims[a.ndim - 1];
+  lx_carry c = sc->c;
+  lx_stats_finish(s, c, a.ndim, dd, a.C, a.R2, a.R, a.dt, LX_STOP_NONE, 0, float(n * a.C),
+                  a.stats + b * (LX_NSTAT_BASE + a.C));
+  sc->c = c;
 }
 
 // ---------------------------------------------------------------- adjoint of the step, pointwise part
@@ -436,11 +458,14 @@ extern "C" int lx_stats(const lx_rule* rule, const float* cells_d, const float* 
   if (B == 0) return LX_OK;
   StatArgs sa;
   sa.cells = cells_d; sa.field = field_d; sa.pot = potential_d;
-  sa.carry = static_cast<lx_carry*>(carry_d);
+  sa.carry = static_cast<lx_stat_carry*>(carry_d);
   sa.stats = stats_d; sa.B = B; sa.ndim = rule->ndim;
   for (int i = 0; i < 3; ++i) sa.dims[i] = i < rule->ndim ? rule->dims[i] : 1;
   sa.C = rule->C; sa.K = rule->K;
   sa.R2 = (float)((double)R * (double)R); sa.R = R; sa.dt = (float)(1.0 / (double)T);
-  hipLaunchKernelGGL(stats_kernel, dim3((unsigned)B), dim3(256), 0, (hipStream_t)stream, sa);
+  if (B > 65535) return lx_fail(LX_ENOTSUP, "lx_stats: more than 65535 worlds per call");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(stats_part_kernel, dim3(LX_STAT_PARTS, (unsigned)B), dim3(256), 0, st, sa);
+  hipLaunchKernelGGL(stats_finish_kernel, dim3(grid_for(B, 64)), dim3(64), 0, st, sa);
   return launched("stats");
 }
