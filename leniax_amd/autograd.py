@@ -21,6 +21,7 @@ wrapped as ``torch.autograd.Function``s:
 (what central finite differences of the forward measure).  Complex gradients
 use torch's convention (``dL/dRe + i dL/dIm``), the conjugate of JAX's.
 """
+import copy
 import ctypes
 import math
 
@@ -93,13 +94,17 @@ class _Ctx:
         self.w = w.contiguous()
         self.ws = core._per_world_stride(self.w, N, 'weights')
         self.dt, self.dts = core._dt_tensor(dt, N)
+        self.work = None  # a workspace of its own (graph-captured segments), else the shared one
+
+    def workspace(self):
+        return self.work if self.work is not None else core._workspace(self.rule, self.N)
 
     def step(self, A, out, field, pot, FA=None):
         """One plain step A -> out; field / potential / the state's spectra FA kept when given."""
         args = (ctypes.byref(self.rule), engine.ptr(A), self.N, engine.ptr(self.Kh), self.kbs, engine.ptr(self.gf),
                 self.gfs, engine.ptr(self.w), self.ws, engine.ptr(self.dt), self.dts, engine.ptr(out),
                 engine.ptr(field), engine.ptr(pot))
-        work = engine.ptr(core._workspace(self.rule, self.N))
+        work = engine.ptr(self.workspace())
         if FA is None:
             _lib.call('lx_update', *args, work, engine.stream_ptr())
         else:
@@ -130,7 +135,7 @@ class _Ctx:
             FA = rfftn(A, A.dim() - 2)
         _lib.call('lx_adjoint_potential', ctypes.byref(self.rule), self.N, engine.ptr(self.Kh), self.kbs,
                   engine.ptr(dpot), engine.ptr(FA if dspec is not None else None), engine.ptr(dspec),
-                  engine.ptr(lam_out), engine.ptr(core._workspace(self.rule, self.N)), engine.stream_ptr())
+                  engine.ptr(lam_out), engine.ptr(self.workspace()), engine.stream_ptr())
         return lam_out
 
     def grad_K(self, dspec, K):
@@ -216,24 +221,124 @@ def update(state, K, gf_params, kernels_weight_per_channel, dt, tc_indices=None,
 
 
 # ------------------------------------------------------------------ checkpointed run
+class _RunGraphs:
+    """HIP graphs of autograd.run's segments for one rule and shape: the forward's c steps,
+    the backward's recompute of a segment and its reverse walk.  Single small worlds are
+    launch-bound (3 launches per forward step, 10 per backward step); a replay issues a
+    segment at once.  The graphs read their own copies of the spectra and parameters
+    (``load``) and work in their own buffers; checkpoints stay with the caller."""
+
+    def __init__(self, sc, c, ste, need_K):
+        self.sc = copy.copy(sc)
+        self.sc.Kh, self.sc.gf, self.sc.w, self.sc.dt = (sc.Kh.clone(), sc.gf.clone(), sc.w.clone(), sc.dt.clone())
+        nbytes = ctypes.c_size_t(0)
+        _lib.call('lx_workspace_size', ctypes.byref(sc.rule), sc.N, ctypes.byref(nbytes))
+        self.sc.work = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device=sc.Kh.device)
+        self.c, self.ste = c, ste
+        shape = (sc.N, sc.C) + sc.dims
+        dev = sc.Kh.device
+        self.seg_A = torch.empty((c,) + shape, device=dev)
+        self.seg_G = torch.empty_like(self.seg_A)
+        self.seg_U = torch.empty((c, sc.N, sc.Keff) + sc.dims, device=dev)
+        self.seg_F = torch.empty((c,) + sc.spectra_shape(), dtype=torch.complex64, device=dev) if need_K else None
+        self.nxt = torch.empty(shape, device=dev)
+        self.fb = [torch.empty(shape, device=dev), torch.empty(shape, device=dev)]
+        self.lam = torch.empty(shape, device=dev)
+        self.dpot = torch.empty_like(self.seg_U[0])
+        self.acc = self.sc.accumulators(need_K)
+        self.graphs = {}
+
+    def load(self, sc):
+        for dst, src in ((self.sc.Kh, sc.Kh), (self.sc.gf, sc.gf), (self.sc.w, sc.w), (self.sc.dt, sc.dt)):
+            dst.copy_(src)
+
+    def _graph(self, key, fn):
+        g = self.graphs.get(key)
+        if g is None:
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                fn()  # warm-up outside the capture
+            torch.cuda.current_stream().wait_stream(side)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                fn()
+            self.graphs[key] = g
+        return g
+
+    def forward_segment(self, L):
+        """Replays L steps from seg_A[0]; returns the static buffer holding the result."""
+        def fn():
+            cur = self.seg_A[0]
+            for i in range(L):
+                self.sc.step(cur, self.fb[i % 2], None, None)
+                cur = self.fb[i % 2]
+        self._graph(('fwd', L), fn).replay()
+        return self.fb[(L - 1) % 2]
+
+    def backward_segment(self, L):
+        def rec():
+            for i in range(L):
+                out = self.seg_A[i + 1] if i + 1 < L else self.nxt
+                self.sc.step(self.seg_A[i], out, self.seg_G[i], self.seg_U[i],
+                             None if self.seg_F is None else self.seg_F[i])
+
+        def adj():
+            for i in range(L - 1, -1, -1):
+                self.sc.adjoint(self.seg_A[i], self.seg_G[i], self.seg_U[i], self.lam, self.acc, self.ste,
+                                FA=None if self.seg_F is None else self.seg_F[i], lam_out=self.lam, dpot=self.dpot)
+        g_rec = self._graph(('rec', L), rec)
+        g_adj = self._graph(('adj', L), adj)
+        return g_rec, g_adj
+
+
+_GRAPHS = {}
+GRAPH_MAX_CELLS = 1 << 22  # graphs pay off where launches, not HBM, bound a step
+
+
+def _graphs_for(sc, c, ste, need_K):
+    r = sc.rule
+    key = (torch.cuda.current_device(), sc.N, sc.C, sc.Keff, sc.dims, tuple(r.cin[:sc.Keff]), tuple(r.gf[:sc.Keff]),
+           r.state_fn, r.average, tuple(sc.Kh.shape), sc.kbs, tuple(sc.gf.shape), sc.gfs, tuple(sc.w.shape), sc.ws,
+           sc.dt.numel(), sc.dts, c, bool(ste), bool(need_K))
+    g = _GRAPHS.get(key)
+    if g is None:
+        g = _GRAPHS[key] = _RunGraphs(sc, c, ste, need_K)
+    return g
+
+
 class _Run(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, state0, K, gf, w, dt, meta):
-        tc_indices, growth_fn_t, weighted_fn, slug, ste, steps, ckpt = meta
+        tc_indices, growth_fn_t, weighted_fn, slug, ste, steps, ckpt, graphs = meta
         A, K_, gf_, w_ = _as_tensors(state0, K, gf, w)
         sc = _Ctx(A, K_, gf_, w_, dt, tc_indices, growth_fn_t, weighted_fn, slug)
+        if graphs is None:
+            graphs = sc.N * sc.C * int(np.prod(sc.dims)) <= GRAPH_MAX_CELLS
         ckpts = [A]
-        bufs = [torch.empty_like(A), torch.empty_like(A)]
-        cur = A
-        for t in range(steps):
-            if t > 0 and t % ckpt == 0:
-                ckpts.append(cur.clone())
-            nxt = bufs[t % 2]
-            sc.step(cur, nxt, None, None)
-            cur = nxt
-        out = cur.clone() if steps > 0 else A.clone()
+        if graphs and steps > 0:
+            G = _graphs_for(sc, ckpt, ste, True)
+            G.load(sc)
+            cur = A
+            for s in range((steps + ckpt - 1) // ckpt):
+                if s > 0:
+                    ckpts.append(cur.clone())
+                G.seg_A[0].copy_(cur)
+                cur = G.forward_segment(min(ckpt, steps - s * ckpt))
+            out = cur.clone()
+        else:
+            bufs = [torch.empty_like(A), torch.empty_like(A)]
+            cur = A
+            for t in range(steps):
+                if t > 0 and t % ckpt == 0:
+                    ckpts.append(cur.clone())
+                nxt = bufs[t % 2]
+                sc.step(cur, nxt, None, None)
+                cur = nxt
+            out = cur.clone() if steps > 0 else A.clone()
         ctx.sc, ctx.ste, ctx.steps, ctx.ckpt, ctx.ckpts, ctx.dt_orig = sc, ste, steps, ckpt, ckpts, dt
+        ctx.graphs = graphs
         ctx.save_for_backward(K_, gf_)
         return out
 
@@ -242,29 +347,45 @@ class _Run(torch.autograd.Function):
         K, gf = ctx.saved_tensors
         sc, steps, c = ctx.sc, ctx.steps, ctx.ckpt
         need = ctx.needs_input_grad
-        acc = sc.accumulators(need[1])
-        lam = g_out.contiguous().clone()  # updated in place, step after step
-        if steps > 0:
-            A0 = ctx.ckpts[0]
-            seg_A = torch.empty((c,) + tuple(A0.shape), dtype=torch.float32, device=A0.device)
-            seg_G = torch.empty_like(seg_A)
-            seg_U = torch.empty((c, sc.N, sc.Keff) + sc.dims, dtype=torch.float32, device=A0.device)
-            seg_F = (torch.empty((c,) + sc.spectra_shape(), dtype=torch.complex64, device=A0.device)
-                     if 'dspec' in acc else None)
-            dpot = torch.empty_like(seg_U[0])
-            nxt = torch.empty_like(A0)
+        if ctx.graphs and steps > 0:
+            G = _graphs_for(sc, c, ctx.ste, True)
+            G.load(sc)
+            lengths = sorted({min(c, steps - s * c) for s in range((steps - 1) // c + 1)})
+            segs = {L: G.backward_segment(L) for L in lengths}  # captures (and their warm-ups) first
+            G.lam.copy_(g_out)
+            for v in G.acc.values():
+                v.zero_()
             for s in range((steps - 1) // c, -1, -1):
-                t0, t1 = s * c, min(s * c + c, steps)
-                seg_A[0].copy_(ctx.ckpts[s])
-                for i in range(t1 - t0):  # recompute the segment, keeping field, potential and spectra
-                    out = seg_A[i + 1] if i + 1 < t1 - t0 else nxt
-                    sc.step(seg_A[i], out, seg_G[i], seg_U[i], None if seg_F is None else seg_F[i])
-                for i in range(t1 - t0 - 1, -1, -1):
-                    sc.adjoint(seg_A[i], seg_G[i], seg_U[i], lam, acc, ctx.ste,
-                               FA=None if seg_F is None else seg_F[i], lam_out=lam, dpot=dpot)
+                g_rec, g_adj = segs[min(c, steps - s * c)]
+                G.seg_A[0].copy_(ctx.ckpts[s])
+                g_rec.replay()
+                g_adj.replay()
+            lam = G.lam.clone()
+            acc = {k: v.clone() for k, v in G.acc.items()}
+        else:
+            acc = sc.accumulators(need[1])
+            lam = g_out.contiguous().clone()  # updated in place, step after step
+            if steps > 0:
+                A0 = ctx.ckpts[0]
+                seg_A = torch.empty((c,) + tuple(A0.shape), dtype=torch.float32, device=A0.device)
+                seg_G = torch.empty_like(seg_A)
+                seg_U = torch.empty((c, sc.N, sc.Keff) + sc.dims, dtype=torch.float32, device=A0.device)
+                seg_F = (torch.empty((c,) + sc.spectra_shape(), dtype=torch.complex64, device=A0.device)
+                         if 'dspec' in acc else None)
+                dpot = torch.empty_like(seg_U[0])
+                nxt = torch.empty_like(A0)
+                for s in range((steps - 1) // c, -1, -1):
+                    t0, t1 = s * c, min(s * c + c, steps)
+                    seg_A[0].copy_(ctx.ckpts[s])
+                    for i in range(t1 - t0):  # recompute the segment, keeping field, potential and spectra
+                        out = seg_A[i + 1] if i + 1 < t1 - t0 else nxt
+                        sc.step(seg_A[i], out, seg_G[i], seg_U[i], None if seg_F is None else seg_F[i])
+                    for i in range(t1 - t0 - 1, -1, -1):
+                        sc.adjoint(seg_A[i], seg_G[i], seg_U[i], lam, acc, ctx.ste,
+                                   FA=None if seg_F is None else seg_F[i], lam_out=lam, dpot=dpot)
         ctx.ckpts = None
         return (lam if need[0] else None,
-                sc.grad_K(acc['dspec'], K) if need[1] else None,
+                sc.grad_K(acc['dspec'], K) if need[1] and 'dspec' in acc else None,
                 sc.grad_gf(acc['gf'], gf) if need[2] else None,
                 acc['w'] if need[3] else None,
                 sc.grad_dt(acc['dt'], ctx.dt_orig) if need[4] else None,
@@ -272,15 +393,17 @@ class _Run(torch.autograd.Function):
 
 
 def run(state0, K, gf_params, kernels_weight_per_channel, dt, steps, tc_indices=None, growth_fn_t=('gaussian',),
-        weighted_fn=None, state_slug='v1', ste=True, ckpt_every=16):
-    """Final state after ``steps`` updates, differentiable with checkpointed recompute."""
+        weighted_fn=None, state_slug='v1', ste=True, ckpt_every=16, graphs=None):
+    """Final state after ``steps`` updates, differentiable with checkpointed recompute.
+    ``graphs``: replay the segments as HIP graphs (default: for batches of at most
+    GRAPH_MAX_CELLS cells, where launches bound a step)."""
     if weighted_fn is None:
         weighted_fn = core.weighted_mean
     if ckpt_every < 1:
         raise ValueError('ckpt_every must be >= 1')
     return _Run.apply(state0, K, gf_params, kernels_weight_per_channel, dt,
                       (tc_indices, tuple(growth_fn_t), weighted_fn, state_slug, ste, int(steps),
-                       int(min(ckpt_every, max(steps, 1)))))
+                       int(min(ckpt_every, max(steps, 1))), graphs))
 
 
 # ------------------------------------------------------------------ kernels

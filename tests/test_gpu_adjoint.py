@@ -314,3 +314,33 @@ def test_field_and_state_gradients_like_reference():
         tA, tG = _dev(A).requires_grad_(), _dev(Gf).requires_grad_()
         fn(None, tA, tG, 0.1).sum().backward()
         assert np.allclose(_np(tA.grad), dA_ref, atol=1e-7) and np.allclose(_np(tG.grad), 0.1, atol=1e-7)
+
+
+def test_graph_replays_match_eager():
+    """autograd.run with its segments replayed as HIP graphs equals the eager run on three
+    calls in a row (new inputs, same shapes, a tail segment), with eager work in between."""
+    from leniax_amd import autograd as AG
+    from leniax_amd import core
+    K = _dev(_asym_K(30, 1, 1, 64, 64))
+    w = _dev(np.array([[1.0]], np.float32))
+    for rep in range(3):
+        print('replay', rep, flush=True)
+        A = _world(31 + rep, 2, 1, 64, 64, 24)
+        gf = np.array([[0.25 + 0.01 * rep, 0.05]], np.float32)
+        Rw = _dev(np.random.default_rng(40 + rep).normal(size=A.shape).astype(np.float32))
+        res = []
+        for graphs in (False, True):
+            tA, tgf = _dev(A).requires_grad_(), _dev(gf).requires_grad_()
+            tdt = torch.tensor(0.1, device='cuda', requires_grad=True)
+            fin = AG.run(tA, K, tgf, w, tdt, 20, ckpt_every=8, graphs=graphs)
+            (fin * Rw).sum().backward()
+            torch.cuda.synchronize()
+            res.append((_np(fin), _np(tA.grad), _np(tgf.grad), float(tdt.grad)))
+            # unrelated eager work on the same stream between graph replays
+            core.fused_update(_dev(A), K, _dev(gf), w, 0.1, None, ('gaussian',), core.weighted_mean, 'v1')
+            torch.cuda.synchronize()
+        (f0, a0, g0, d0), (f1, a1, g1, d1) = res
+        assert np.array_equal(f0, f1)
+        assert np.array_equal(a0, a1)
+        assert np.abs(g0 - g1).max() <= 1e-5 * np.abs(g0).max()
+        assert abs(d0 - d1) <= 1e-5 * abs(d0)
