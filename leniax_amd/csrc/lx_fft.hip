@@ -24,9 +24,13 @@ namespace {
 
 template <int L>
 struct Lanes {
-  static constexpr int TPR = L <= 128 ? 4 : (L % 3 == 0 ? L / 24 : L / 32);  // lanes per transform
-  static constexpr int P = L / TPR;                                          // points per lane
+  static constexpr int TPR = L <= 96 ? 4 : (L <= 192 ? 8 : 16);  // lanes per transform
+  static constexpr int P = L / TPR;                                // points per lane
+  static constexpr bool XL = P % TPR == 0;                         // cross-lane step through LDS
+  static constexpr int XRS = xch_row(P);                           // group scratch row stride
+  static constexpr int XCH = 256 * XRS;                            // scratch of a 256-thread block
   static_assert(TPR == 4 || TPR == 8 || TPR == 16, "lane groups of 4, 8 or 16");
+  static_assert(TPR * XRS >= L + 2, "a group's scratch holds one transform plus two");
 };
 
 template <int V>
@@ -94,20 +98,29 @@ int launched(const char* what) {
 
 unsigned blocks_for(int64_t threads) { return unsigned((threads + 255) / 256); }
 
-// partner lane holding block (TPR - s) mod TPR, s = bitrev(t)
-template <int TPR>
-LX_DEV int neg_block_lane(int t) {
-  const int s = bitrev_n(t, TPR);
-  return bitrev_n((TPR - s) & (TPR - 1), TPR);
+// Frequency held by register e of lane t after a forward group transform: with the LDS
+// exchange e = j TPR + k2 holds (J t + j) + P k2; with the DPP network e holds e + P bitrev(t).
+template <int L>
+LX_DEV int freq_of(int t, int e) {
+  using LC = Lanes<L>;
+  if constexpr (LC::XL) {
+    constexpr int J = LC::P / LC::TPR;
+    return J * t + e / LC::TPR + LC::P * (e % LC::TPR);
+  } else {
+    return e + LC::P * bitrev_n(t, LC::TPR);
+  }
 }
 
-// value of lane t ^ (TPR-1) in the same lane group
-template <int TPR>
-LX_DEV float2 mirror_fetch(float2 v) {
-  if constexpr (TPR == 4) return dpp2<LX_QP(3, 2, 1, 0)>(v);
-  else if constexpr (TPR == 8) return dpp2<0x141>(v);   // row_half_mirror
-  else return dpp2<0x140>(v);                           // row_mirror (TPR == 16)
+// Length-L transform over a lane group: natural order (data[TPR m + t] in x[m]) <-> the
+// frequency layout of freq_of; xch is the group's LDS scratch.
+template <int L, bool INV>
+LX_DEV void fft_group(float2* x, int t, int lane, float2* xch) {
+  using LC = Lanes<L>;
+  if constexpr (LC::XL) fft_xlds<LC::P, LC::TPR, INV>(x, t, g_tw<L>, xch);
+  else fft_lanes<LC::P, LC::TPR, INV>(x, t, lane, g_tw<L>);
 }
+
+LX_DEV void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
 
 // ---------------------------------------------------------------- geometry
 struct Geom {
@@ -145,42 +158,45 @@ int make_geom(int ndim, const int32_t* dims, Geom& g) {
 }
 
 // ---------------------------------------------------------------- rows forward
+// Group = one pair of rows packed as z = a + i b; after the transform Z goes through the group's
+// scratch in frequency order and comes out as the two half spectra, A[k] = (Z[k] + conj Z[W-k])/2,
+// B[k] = (Z[k] - conj Z[W-k])/2i, stored with consecutive lanes on consecutive k.
 template <int W>
 __global__ __launch_bounds__(256) void rows_fwd_kernel(const float* __restrict__ A, float2* __restrict__ S,
-                                                       int64_t npairs, int64_t rows) {
+                                                       int64_t npairs) {
   using LC = Lanes<W>;
   constexpr int TPR = LC::TPR, P = LC::P, WH = W / 2 + 1;
+  __shared__ __align__(16) float2 xch_all[LC::XCH];
   const int lane = threadIdx.x & 63;
   const int t = threadIdx.x % TPR;
+  float2* xch = xch_all + (threadIdx.x / TPR) * (TPR * LC::XRS);
   const int64_t pr = (int64_t(blockIdx.x) * 256 + threadIdx.x) / TPR;
   const bool live = pr < npairs;
-  const int64_t row0 = live ? pr * 2 : 0;  // rows pair within an image: rows is even
+  const int64_t row0 = live ? pr * 2 : 0;  // row pairs never straddle images: rows per image are even
   const float* a0 = A + row0 * W;
   const float* a1 = a0 + W;
   float2 x[P];
 #pragma unroll
   for (int m = 0; m < P; ++m) x[m] = make_float2(a0[TPR * m + t], a1[TPR * m + t]);
-  fft_lanes<P, TPR, false>(x, t, lane, g_tw<W>);
-  const int s = bitrev_n(t, TPR);
-  // Z[W - k]: k1 >= 1 from lane t^(TPR-1) at index P - k1; k1 == 0 from block (TPR - s) at index 0
-  const float2 z0n = make_float2(__shfl(x[0].x, (lane & ~(TPR - 1)) | neg_block_lane<TPR>(t), 64),
-                                 __shfl(x[0].y, (lane & ~(TPR - 1)) | neg_block_lane<TPR>(t), 64));
+  fft_group<W, false>(x, t, lane, xch);
+  wave_lds_fence();
+#pragma unroll
+  for (int e = 0; e < P; ++e) xch[freq_of<W>(t, e)] = x[e];
+  wave_lds_fence();
   float2* s0 = S + row0 * int64_t(WH);
   float2* s1 = s0 + WH;
 #pragma unroll
-  for (int k1 = 0; k1 < P; ++k1) {
-    const float2 zn = k1 == 0 ? z0n : mirror_fetch<TPR>(x[P - k1 > 0 && k1 > 0 ? P - k1 : 0]);
-    const float2 z = x[k1];
-    const int k = k1 + P * s;
-    // X_a = (z + conj zn) / 2, X_b = (z - conj zn) / 2i
-    const float2 xa = make_float2(0.5f * (z.x + zn.x), 0.5f * (z.y - zn.y));
-    const float2 xb = make_float2(0.5f * (z.y + zn.y), 0.5f * (zn.x - z.x));
-    if (live && k <= W / 2) {
-      s0[k] = xa;
-      s1[k] = xb;
+  for (int i = 0; i < (WH + TPR - 1) / TPR; ++i) {
+    const int k = t + TPR * i;
+    if (k < WH) {
+      const float2 z = xch[k];
+      const float2 zn = xch[k == 0 ? 0 : W - k];
+      if (live) {
+        s0[k] = make_float2(0.5f * (z.x + zn.x), 0.5f * (z.y - zn.y));
+        s1[k] = make_float2(0.5f * (z.y + zn.y), 0.5f * (zn.x - z.x));
+      }
     }
   }
-  (void)rows;
 }
 
 // ---------------------------------------------------------------- lines along a leading axis
@@ -214,76 +230,91 @@ LineArgs line_args() {
   return a;
 }
 
-template <int L>
+// One launch per mode (MODE 0: conv, 1: forward in place, 2: inverse), so that each keeps
+// only its own registers.  Offsets inside an image are 32-bit; loads are issued in batches
+// of 8 (sched_barrier) so that addresses and values are not all held at once.
+template <int L, int MODE>
 __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
   using LC = Lanes<L>;
   constexpr int TPR = LC::TPR, P = LC::P;
+  __shared__ __align__(16) float2 xch_all[LC::XL ? LC::XCH : 2];
   const int lane = threadIdx.x & 63;
   const int t = threadIdx.x % TPR;
+  float2* xch = xch_all + (LC::XL ? (threadIdx.x / TPR) * (TPR * LC::XRS) : 0);
   const int64_t lid = (int64_t(blockIdx.x) * 256 + threadIdx.x) / TPR;
   const bool live = lid < a.nlines;
   const int64_t per_img = int64_t(a.A) * a.WH;
   const int64_t img = live ? lid / per_img : 0;
-  const int64_t rem = live ? lid - img * per_img : 0;
-  const int64_t aa = rem / a.WH;
-  const int64_t off = aa * a.a_stride + (rem - aa * a.WH);  // line base inside its image
+  const int rem = live ? int(lid - img * per_img) : 0;
+  const int aa = rem / a.WH;
+  const int off = aa * int(a.a_stride) + (rem - aa * a.WH);  // line base inside its image
   const int64_t b = img / a.C;
   const int c = int(img - b * a.C);
-  const int64_t st = a.st;
+  const int st = int(a.st);
   const float2* src = a.S + img * a.img_elems + off;
-  const int s = bitrev_n(t, TPR);
   float2 x[P], y[P];
-  if (a.mode == 2) {  // inverse only; input in frequency order
+  if constexpr (MODE == 2) {  // inverse only; input in frequency order
 #pragma unroll
-    for (int k1 = 0; k1 < P; ++k1) y[k1] = src[int64_t(k1 + P * s) * st];
-    fft_lanes<P, TPR, true>(y, t, lane, g_tw<L>);
+    for (int e = 0; e < P; ++e) {
+      y[e] = src[freq_of<L>(t, e) * st];
+      if ((e & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+    fft_group<L, true>(y, t, lane, xch);
     float2* dst = a.U + img * a.img_elems + off;
     if (live) {
 #pragma unroll
-      for (int m = 0; m < P; ++m) dst[int64_t(TPR * m + t) * st] = y[m];
+      for (int m = 0; m < P; ++m) dst[(TPR * m + t) * st] = y[m];
     }
     return;
   }
 #pragma unroll
-  for (int m = 0; m < P; ++m) x[m] = src[int64_t(TPR * m + t) * st];
-  fft_lanes<P, TPR, false>(x, t, lane, g_tw<L>);
-  if (a.mode == 1) {  // forward only, in place
+  for (int m = 0; m < P; ++m) {
+    x[m] = src[(TPR * m + t) * st];
+    if ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+  }
+  fft_group<L, false>(x, t, lane, xch);
+  if constexpr (MODE == 1) {  // forward only, in place
     float2* dst = const_cast<float2*>(src);
     if (live) {
 #pragma unroll
-      for (int k1 = 0; k1 < P; ++k1) dst[int64_t(k1 + P * s) * st] = x[k1];
+      for (int e = 0; e < P; ++e) dst[freq_of<L>(t, e) * st] = x[e];
     }
     return;
   }
   if (a.FA && live) {
     float2* fa = a.FA + img * a.img_elems + off;
 #pragma unroll
-    for (int k1 = 0; k1 < P; ++k1) fa[int64_t(k1 + P * s) * st] = x[k1];
+    for (int e = 0; e < P; ++e) fa[freq_of<L>(t, e) * st] = x[e];
   }
+  const float csign = a.conjk ? -1.f : 1.f;  // multiply by K or by conj(K)
   for (int k = 0; k < a.K; ++k) {
     if (a.cin[k] != c) continue;
+    // an opaque copy of the lane index: the per-kernel addresses and twiddle loads are then
+    // recomputed in each iteration instead of being hoisted out of the loop and kept live
+    int t = threadIdx.x % TPR;
+    __asm__ volatile("" : "+v"(t));
     if (a.dspec && live) {
       const float2* fa = a.FAin + (b * a.C_fa + a.acc_cin[k]) * a.img_elems + off;
       float2* ds = a.dspec + (b * a.K + k) * a.img_elems + off;
 #pragma unroll
-      for (int k1 = 0; k1 < P; ++k1) {
-        const int64_t o = int64_t(k1 + P * s) * st;
-        ds[o] = cadd(ds[o], cmulc(x[k1], fa[o]));
+      for (int e = 0; e < P; ++e) {
+        const int o = freq_of<L>(t, e) * st;
+        ds[o] = cadd(ds[o], cmulc(x[e], fa[o]));
+        if ((e & 7) == 7) __builtin_amdgcn_sched_barrier(0);
       }
     }
     const float2* kk = a.Kh + b * a.k_bstride + int64_t(k) * a.img_elems + off;
-    if (a.conjk) {
 #pragma unroll
-      for (int k1 = 0; k1 < P; ++k1) y[k1] = cmulc(x[k1], kk[int64_t(k1 + P * s) * st]);
-    } else {
-#pragma unroll
-      for (int k1 = 0; k1 < P; ++k1) y[k1] = cmul(x[k1], kk[int64_t(k1 + P * s) * st]);
+    for (int e = 0; e < P; ++e) {
+      const float2 kv = kk[freq_of<L>(t, e) * st];
+      y[e] = cmul(x[e], make_float2(kv.x, csign * kv.y));
+      if ((e & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     }
-    fft_lanes<P, TPR, true>(y, t, lane, g_tw<L>);
+    fft_group<L, true>(y, t, lane, xch);
     float2* dst = a.U + (b * a.K + k) * a.img_elems + off;
     if (live) {
 #pragma unroll
-      for (int m = 0; m < P; ++m) dst[int64_t(TPR * m + t) * st] = y[m];
+      for (int m = 0; m < P; ++m) dst[(TPR * m + t) * st] = y[m];
     }
   }
 }
@@ -318,8 +349,10 @@ template <int W>
 __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
   using LC = Lanes<W>;
   constexpr int TPR = LC::TPR, P = LC::P, WH = W / 2 + 1;
+  __shared__ __align__(16) float2 xch_all[LC::XCH];
   const int lane = threadIdx.x & 63;
   const int t = threadIdx.x % TPR;
+  float2* xch = xch_all + (threadIdx.x / TPR) * (TPR * LC::XRS);
   const int64_t pr = (int64_t(blockIdx.x) * 256 + threadIdx.x) / TPR;
   const bool live = pr < a.npairs;
   const int64_t half_rows = a.rows / 2;
@@ -330,9 +363,6 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
   const float inv_n = 1.f / float(a.rows * W);
   const float* pw = a.sel ? nullptr : a.w + b * a.w_bs;
   const float* pg = a.sel ? nullptr : a.gf + b * a.gf_bs;
-  const int s = bitrev_n(t, TPR);
-  const bool mirror = s >= TPR / 2;
-  const float sg = mirror ? -1.f : 1.f;
   float G0[P], G1[P];
 #pragma unroll
   for (int m = 0; m < P; ++m) G0[m] = G1[m] = 0.f;
@@ -350,18 +380,35 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
     }
     const bool want_pot = a.pot != nullptr && owner == c;
     if (wk == 0.f && !want_pot) continue;
+    int t = threadIdx.x % TPR;  // opaque per iteration: addresses and twiddles are not hoisted
+    __asm__ volatile("" : "+v"(t));
+    // stage the two rows' half spectra (coalesced), then repack them as one complex row:
+    // Z[k] = A[k] + i B[k], A[k] = conj(A[W-k]) above W/2; DC and Nyquist are real
     const float2* u0 = a.U + ((b * a.K + k) * a.rows + r2) * WH;
     const float2* u1 = u0 + WH;
+    wave_lds_fence();
 #pragma unroll
-    for (int k1 = 0; k1 < P; ++k1) {
-      const int kk = k1 + P * s;
-      const int col = mirror ? W - kk : kk;  // W - kk <= W/2 when mirrored
-      const float2 ya = u0[col], yb = u1[col];
-      float2 z = make_float2(fmaf(-sg, yb.y, ya.x), fmaf(sg, ya.y, yb.x));
-      if (k1 == 0 && (s == 0 || s == TPR / 2)) z = make_float2(ya.x, yb.x);  // DC and Nyquist are real
-      x[k1] = z;
+    for (int i = 0; i < (WH + TPR - 1) / TPR; ++i) {
+      const int col = t + TPR * i;
+      if (col < WH) {
+        xch[col] = u0[col];
+        xch[WH + col] = u1[col];
+      }
     }
-    fft_lanes<P, TPR, true>(x, t, lane, g_tw<W>);
+    wave_lds_fence();
+#pragma unroll
+    for (int e = 0; e < P; ++e) {
+      const int kk = freq_of<W>(t, e);
+      const bool mirror = kk >= W / 2;
+      const float sg = mirror ? -1.f : 1.f;
+      const int col = mirror ? W - kk : kk;
+      const float2 ya = xch[col], yb = xch[WH + col];
+      float2 z = make_float2(fmaf(-sg, yb.y, ya.x), fmaf(sg, ya.y, yb.x));
+      if (kk == 0 || kk == W / 2) z = make_float2(ya.x, yb.x);
+      x[e] = z;
+    }
+    wave_lds_fence();
+    fft_group<W, true>(x, t, lane, xch);
     const float gm = a.sel ? 0.f : pg[2 * k], gs = a.sel ? 0.f : pg[2 * k + 1];
     float* po0 = want_pot ? a.pot + ((b * a.K + k) * a.rows + r2) * W : nullptr;
 #pragma unroll
@@ -411,7 +458,7 @@ int launch_rows_fwd(const Geom& g, const float* A, float2* S, int64_t nimg, hipS
     constexpr int W = decltype(w)::value;
     if (int rc = ensure_tw<W>(st)) return rc;
     hipLaunchKernelGGL(rows_fwd_kernel<W>, dim3(blocks_for(npairs * Lanes<W>::TPR)), dim3(256), 0, st, A, S,
-                       npairs, g.rows);
+                       npairs);
     return launched("rows_fwd");
   });
 }
@@ -423,7 +470,10 @@ int launch_lines(const Geom& g, int axis, LineArgs la, int64_t nimg, hipStream_t
   return dispatch_len(g.L[axis], [&](auto l) {
     constexpr int L = decltype(l)::value;
     if (int rc = ensure_tw<L>(st)) return rc;
-    hipLaunchKernelGGL(lines_kernel<L>, dim3(blocks_for(la.nlines * Lanes<L>::TPR)), dim3(256), 0, st, la);
+    const dim3 grid(blocks_for(la.nlines * Lanes<L>::TPR));
+    if (la.mode == 0) hipLaunchKernelGGL((lines_kernel<L, 0>), grid, dim3(256), 0, st, la);
+    else if (la.mode == 1) hipLaunchKernelGGL((lines_kernel<L, 1>), grid, dim3(256), 0, st, la);
+    else hipLaunchKernelGGL((lines_kernel<L, 2>), grid, dim3(256), 0, st, la);
     return launched("lines");
   });
 }
