@@ -367,9 +367,8 @@ __global__ __launch_bounds__(256) void adj_growth_kernel(AdjArgs a) {
   float* dUk = a.dU + bk * a.cells;
   for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < a.cells; p += int64_t(gridDim.x) * blockDim.x) {
     const float u = Uk[p];
-    const float g = growth(kind, gm, gs, u);
     float dgdu, dgdm, dgds;
-    growth_grad(kind, gm, gs, u, dgdu, dgdm, dgds);
+    const float g = growth_val_grad(kind, gm, gs, u, dgdu, dgdm, dgds);
     float dg = 0.f;
 #pragma unroll
     for (int c = 0; c < LX_MAX_C; ++c) {
@@ -392,6 +391,43 @@ __global__ __launch_bounds__(256) void adj_growth_kernel(AdjArgs a) {
       a.g_w && 2 < a.C ? a.g_w + b * a.w_bs + 2 * a.K + k : nullptr,
       a.g_w && 3 < a.C ? a.g_w + b * a.w_bs + 3 * a.K + k : nullptr};
   block_atomic_sum<2 + LX_MAX_C>(acc, dst);
+}
+
+// One channel, one kernel: the whole pointwise reverse in one pass (grid: blocks per image x B).
+// Reads A, G, U and lambda once; lambda_out may alias lambda (each thread reads before it writes).
+__global__ __launch_bounds__(256) void adj_11_kernel(AdjArgs a) {
+  const int64_t b = blockIdx.y;
+  const float w = a.w[b * a.w_bs];
+  const float gm = a.gf[b * a.gf_bs], gs = a.gf[b * a.gf_bs + 1];
+  const float dtw = a.dt[b * a.dt_bs];
+  const int kind = a.gfk[0];
+  const float coef = a.average ? w / w : w;      // dG -> dg (weighted mean of one kernel: w / w)
+  const float inv_den = a.average ? 1.f / w : 1.f;
+  const int64_t base = b * a.cells;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};  // dm, ds, dw, ddt
+  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < a.cells; p += int64_t(gridDim.x) * blockDim.x) {
+    const float A = a.A[base + p], G = a.G[base + p], U = a.U[base + p];
+    const float dz = adj_dz(a, A, G, a.lam[base + p], dtw);
+    float dG = dtw * dz;
+    if (a.dfield) dG += a.dfield[base + p];
+    float dgdu, dgdm, dgds;
+    const float g = growth_val_grad(kind, gm, gs, U, dgdu, dgdm, dgds);
+    const float dg = dG * coef;
+    a.dU[base + p] = a.dpot_in ? fmaf(dg, dgdu, a.dpot_in[base + p]) : dg * dgdu;
+    acc[0] += dg * dgdm;
+    acc[1] += dg * dgds;
+    acc[2] += dG * (a.average ? (g - G) * inv_den : g);
+    if (a.state_fn == LX_STATE_V2) {
+      a.lam_out[base + p] = dz * (1.f - dtw);
+      acc[3] += dz * (G - A);
+    } else {
+      a.lam_out[base + p] = dz;
+      acc[3] += dz * G;
+    }
+  }
+  float* const dst[4] = {a.g_gf ? a.g_gf + b * a.gf_bs : nullptr, a.g_gf ? a.g_gf + b * a.gf_bs + 1 : nullptr,
+                         a.g_w ? a.g_w + b * a.w_bs : nullptr, a.g_dt ? a.g_dt + b * a.dt_bs : nullptr};
+  block_atomic_sum<4>(acc, dst);
 }
 
 // grid (blocks per image, B * C): direct part of dL/dA and dL/ddt
@@ -443,9 +479,13 @@ extern "C" int lx_adjoint_pointwise(const lx_rule* rule, int64_t B, const float*
   a.state_fn = rule->state_fn; a.average = rule->average; a.ste = ste != 0;
   const unsigned per_img = unsigned(std::min<int64_t>((a.cells + 1023) / 1024, 256));
   hipStream_t st = (hipStream_t)stream;
-  // growth first: it reads lambda, which the state kernel may overwrite in place
-  hipLaunchKernelGGL(adj_growth_kernel, dim3(per_img, unsigned(B * rule->K)), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(adj_state_kernel, dim3(per_img, unsigned(B * rule->C)), dim3(256), 0, st, a);
+  if (rule->C == 1 && rule->K == 1) {
+    hipLaunchKernelGGL(adj_11_kernel, dim3(per_img, unsigned(B)), dim3(256), 0, st, a);
+  } else {
+    // growth first: it reads lambda, which the state kernel may overwrite in place
+    hipLaunchKernelGGL(adj_growth_kernel, dim3(per_img, unsigned(B * rule->K)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(adj_state_kernel, dim3(per_img, unsigned(B * rule->C)), dim3(256), 0, st, a);
+  }
   return launched("adjoint pointwise");
 }
 
