@@ -102,7 +102,26 @@ LX_DEV int group_lane(int lane, int t) {
   else return (lane & ~11) | (t & 3) | ((t & 4) << 1);
 }
 
-template <int N, int GF, bool HIST>
+// Weighted field of the one kernel, w g_k(U) / w (leniax/core.py:163-242), per growth kind.
+template <int GF>
+LX_DEV float field_of(float U, float m, float s, float gk, float inv_9s2, float wk, float inv_wden, float c2,
+                      float c1) {
+  if constexpr (GF == LX_GF_GAUSSIAN || GF == LX_GF_GAUSSIAN_TARGET) {
+    const float z = (U - m) * gk;
+    const float e = __builtin_amdgcn_exp2f(-(z * z));
+    return GF == LX_GF_GAUSSIAN ? fmaf(c2, e, -c1) : c1 * e;
+  } else if constexpr (GF == LX_GF_POLY_QUAD4) {
+    const float d = U - m;
+    float o = fmaxf(0.f, 1.f - (d * d) * inv_9s2);
+    o = o * o;
+    o = o * o;
+    return fmaf(c2, o, -c1);
+  } else {
+    return mul_nc(wk, growth_k<GF>(m, s, U)) * inv_wden;
+  }
+}
+
+template <int N, int GF, bool HIST, bool V1>
 __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
   using G = F2<N>;
   constexpr int P = G::P, TPR = G::TPR, NT = G::NT, NW = G::NW, HALF = G::HALF;
@@ -153,6 +172,10 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
     const float wk = p.w[world * p.w_bstride];
     const float inv_wden = 1.f / (p.average ? wk : 1.f);
     const float inv_s = 1.f / gs, inv_9s2 = 1.f / (9.f * (gs * gs));
+    // field of the single kernel: G = w g(U) / w; the growths 2 e - 1 and 2 o^4 - 1 become one
+    // FMA c2 e - c1, and the Gaussian's exp(-z^2 / 2) one exp2 of the pre-scaled z
+    const float c1 = wk * inv_wden, c2 = 2.f * c1;
+    const float gk = inv_s * 0.84932180028801904272f;  // sqrt(log2(e) / 2)
     const float dtw = p.dt[world * p.dt_bstride];
     if (tid == 0) {
       for (int i = 0; i < 3; ++i) { carry.shift[i] = 0; carry.centroid[i] = 0.f; }
@@ -160,6 +183,7 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
       carry.t = 0;
       carry.cont = 1;
       carry.n = 0;
+      ctrl[1] = 0;
       ctrl[2] = 0;
       ctrl[3] = 0;
     }
@@ -172,7 +196,6 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
       // loop instead of being hoisted out of it and pinned in registers.
       int r = r0, q = t0;
       __asm__ volatile("" : "+v"(r), "+v"(q));
-      const int sh0 = ctrl[2], sh1 = ctrl[3];
 
       // ---- rows forward: packed row r = A[2r] + i A[2r+1]
 #pragma unroll
@@ -183,6 +206,10 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
 #pragma unroll
         for (int k2 = 0; k2 < TPR; ++k2) buf[r * G::ZRS + k2 * (P + 1) + J * q + j] = x[j * TPR + k2];
       __syncthreads();
+      // The previous step's statistics were finished by thread 0 while the other waves ran
+      // this step's row transforms; this barrier publishes its stop decision and shifts.
+      if (ctrl[1]) break;
+      const int sh0 = ctrl[2], sh1 = ctrl[3];
 
       // ---- columns forward: c = r, rows h = TPR m + q; unpack the packed rows:
       //      row 2h' = (za + conj zb)/2, row 2h'+1 = (za - conj zb)/2i; column 0 packs
@@ -303,8 +330,7 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
           const int row = 2 * r + part;
           const float U = (part ? x[m].y : x[m].x) * inv_n2;
           const float a = part ? a1[m] : a0[m];
-          const float g = growth_pre<GF>(gm, gs, inv_s, inv_9s2, U);
-          const float Gf = mul_nc(wk, g) * inv_wden;
+          const float Gf = field_of<GF>(U, gm, gs, gk, inv_9s2, wk, inv_wden, c2, c1);
           const float gp = maxnan(Gf, 0.f);
           if (part) { sa1 += a; sg1 += gp; } else { sa0 += a; sg0 += gp; }
           am += a;
@@ -317,7 +343,13 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
             if (p.field_hist) p.field_hist[hist_off + row * N + col] = Gf;
             if (p.pot_hist) p.pot_hist[hist_off + row * N + col] = U;
           }
-          const float an = state_update(p.state_fn, a, dtw, Gf);
+          float an;
+          if constexpr (V1) {  // leniax/core.py:245-272: clip(A + dt G, 0, 1)
+            const float v = add_nc(a, mul_nc(dtw, Gf));
+            an = v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
+          } else {
+            an = state_update(p.state_fn, a, dtw, Gf);
+          }
           if (part) a1[m] = an; else a0[m] = an;
         }
         sAy1 = fmaf(am, xc, sAy1);
@@ -372,9 +404,7 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
         ctrl[2] = carry.shift[0];
         ctrl[3] = carry.shift[1];
       }
-      __syncthreads();
-      ++t;
-      if (ctrl[1]) break;
+      ++t;  // no barrier: the next step's first one orders thread 0's writes before their readers
     }
 
     float* A1 = p.cells + world * cells_per_world;
@@ -394,13 +424,13 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
 template <int UNUSED>
 __global__ void zero_i32(int32_t* p) { *p = 0; }
 
-template <int N, int GF, bool HIST>
+template <int N, int GF, bool HIST, bool V1>
 int launch_k(const FusedParams& p, int64_t B, hipStream_t stream) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return lx_fail(LX_EDEVICE, "hipGetDevice failed");
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return lx_fail(LX_EDEVICE, "hipDeviceGetAttribute failed");
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused2d_kernel<N, GF, HIST>, F2<N>::NT, 0) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fused2d_kernel<N, GF, HIST, V1>, F2<N>::NT, 0) !=
           hipSuccess ||
       per_cu < 1)
     return lx_fail(LX_EDEVICE, "fused kernel does not fit on a compute unit");
@@ -408,22 +438,22 @@ int launch_k(const FusedParams& p, int64_t B, hipStream_t stream) {
   if (grid > B) grid = B;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(zero_i32<0>, dim3(1), dim3(1), 0, stream, p.queue);
-  hipLaunchKernelGGL((fused2d_kernel<N, GF, HIST>), dim3((unsigned)grid), dim3(F2<N>::NT), 0, stream, p);
+  hipLaunchKernelGGL((fused2d_kernel<N, GF, HIST, V1>), dim3((unsigned)grid), dim3(F2<N>::NT), 0, stream, p);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return lx_fail(LX_EDEVICE, "fused launch failed: %s", hipGetErrorString(e));
   return LX_OK;
 }
 
-template <int N, bool HIST>
+template <int N, bool HIST, bool V1>
 int launch_gf(const FusedParams& p, int64_t B, int gf, hipStream_t stream) {
   switch (gf) {
-    case LX_GF_POLY_QUAD4: return launch_k<N, LX_GF_POLY_QUAD4, HIST>(p, B, stream);
-    case LX_GF_GAUSSIAN: return launch_k<N, LX_GF_GAUSSIAN, HIST>(p, B, stream);
-    case LX_GF_GAUSSIAN_TARGET: return launch_k<N, LX_GF_GAUSSIAN_TARGET, HIST>(p, B, stream);
-    case LX_GF_STEP: return launch_k<N, LX_GF_STEP, HIST>(p, B, stream);
-    case LX_GF_STAIRCASE: return launch_k<N, LX_GF_STAIRCASE, HIST>(p, B, stream);
-    case LX_GF_TRIANGLE: return launch_k<N, LX_GF_TRIANGLE, HIST>(p, B, stream);
-    default: return launch_k<N, LX_GF_IDENTITY, HIST>(p, B, stream);
+    case LX_GF_POLY_QUAD4: return launch_k<N, LX_GF_POLY_QUAD4, HIST, V1>(p, B, stream);
+    case LX_GF_GAUSSIAN: return launch_k<N, LX_GF_GAUSSIAN, HIST, V1>(p, B, stream);
+    case LX_GF_GAUSSIAN_TARGET: return launch_k<N, LX_GF_GAUSSIAN_TARGET, HIST, V1>(p, B, stream);
+    case LX_GF_STEP: return launch_k<N, LX_GF_STEP, HIST, V1>(p, B, stream);
+    case LX_GF_STAIRCASE: return launch_k<N, LX_GF_STAIRCASE, HIST, V1>(p, B, stream);
+    case LX_GF_TRIANGLE: return launch_k<N, LX_GF_TRIANGLE, HIST, V1>(p, B, stream);
+    default: return launch_k<N, LX_GF_IDENTITY, HIST, V1>(p, B, stream);
   }
 }
 
@@ -457,7 +487,10 @@ int launch(const lx_rule* rule, const lx_run_args* a, hipStream_t stream) {
   p.state_fn = rule->state_fn;
   p.average = rule->average;
   const bool hist = a->cells_hist_d || a->field_hist_d || a->potential_hist_d;
-  return hist ? launch_gf<N, true>(p, a->B, rule->gf[0], stream) : launch_gf<N, false>(p, a->B, rule->gf[0], stream);
+  const int gf = rule->gf[0];
+  if (rule->state_fn == LX_STATE_V1)
+    return hist ? launch_gf<N, true, true>(p, a->B, gf, stream) : launch_gf<N, false, true>(p, a->B, gf, stream);
+  return hist ? launch_gf<N, true, false>(p, a->B, gf, stream) : launch_gf<N, false, false>(p, a->B, gf, stream);
 }
 
 }  // namespace lx_fused
