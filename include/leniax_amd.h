@@ -187,9 +187,14 @@ typedef struct lx_run_args {
   float* cells_hist_d;     /* [max_steps][B][C][dims] state before each step, or NULL */
   float* field_hist_d;     /* [max_steps][B][C][dims] or NULL */
   float* potential_hist_d; /* [max_steps][B][K][dims] or NULL */
-  int32_t* queue_d;        /* one int32 of device scratch (the work queue head) */
+  void* scratch_d;         /* lx_run_scratch_bytes(B) bytes of device scratch (queues, parked carries) */
 } lx_run_args;
 
+/* With an early-stop policy and more than 256 steps, lx_run is two launches: every world first
+ * runs at most max_steps / 8 steps (a probe); the worlds still running then resume together from
+ * their parked state and carry, so that the long-lived ones start at once instead of whenever a
+ * compute unit frees up.  Results are those of one uninterrupted run. */
+size_t lx_run_scratch_bytes(int64_t B);
 int lx_run_supported(const lx_rule* rule);
 int lx_run(const lx_rule* rule, const lx_run_args* args, void* stream);
 

@@ -70,7 +70,7 @@ class lx_run_args(ctypes.Structure):
         ('cells_hist_d', ctypes.c_void_p),
         ('field_hist_d', ctypes.c_void_p),
         ('potential_hist_d', ctypes.c_void_p),
-        ('queue_d', ctypes.c_void_p),
+        ('scratch_d', ctypes.c_void_p),
     ]
 
 
@@ -103,6 +103,7 @@ SIGNATURES = {
                                             _I32, _P, _P, _P, _P, _P, _P]),
     'lx_half_to_full': (ctypes.c_int, [_I32, ctypes.POINTER(_I32), _I64, _P, _P, _F, _P]),
     'lx_adjoint_potential': (ctypes.c_int, [_R, _I64, _P, _I64, _P, _P, _P, _P, _P, _P]),
+    'lx_run_scratch_bytes': (ctypes.c_size_t, [_I64]),
     'lx_run_supported': (ctypes.c_int, [_R]),
     'lx_run': (ctypes.c_int, [_R, ctypes.POINTER(lx_run_args), _P]),
 }

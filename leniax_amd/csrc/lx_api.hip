@@ -96,11 +96,13 @@ extern "C" int lx_run_supported(const lx_rule* rule) {
   return lx_fused2d_supported(rule);
 }
 
+extern "C" size_t lx_run_scratch_bytes(int64_t B) { return lx_run_scratch_layout(B < 0 ? 0 : B).total; }
+
 extern "C" int lx_run(const lx_rule* rule, const lx_run_args* a, void* stream) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
   if (!a) return lx_fail(LX_EINVAL, "args is NULL");
-  if (!a->cells0_d || !a->cells_d || !a->K_half_d || !a->gf_d || !a->w_d || !a->dt_d || !a->queue_d)
+  if (!a->cells0_d || !a->cells_d || !a->K_half_d || !a->gf_d || !a->w_d || !a->dt_d || !a->scratch_d)
     return lx_fail(LX_EINVAL, "NULL device pointer in lx_run_args");
   if (a->B < 0 || a->max_steps < 0) return lx_fail(LX_EINVAL, "B and max_steps must be >= 0");
   if (a->stop_policy < LX_STOP_NONE || a->stop_policy > LX_STOP_MASS)

@@ -66,7 +66,12 @@ struct FusedParams {
   float* cells_hist;
   float* field_hist;
   float* pot_hist;
-  int32_t* queue;
+  int32_t* queue;       // this launch's queue head
+  int32_t* surv_count;  // worlds parked by the probe launch
+  int32_t* surv_list;
+  lx_carry* park;       // parked carries, one per world
+  int limit;            // steps this launch may run per world
+  int resume;           // 0: probe (or whole) launch over all worlds; 1: the parked survivors
   int state_fn;
   int average;
 };
@@ -155,12 +160,17 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
   float2 x[P];
 
   for (;;) {
-    if (tid == 0) ctrl[0] = atomicAdd(p.queue, 1);
+    if (tid == 0) {
+      const int i = atomicAdd(p.queue, 1);
+      if (!p.resume) ctrl[0] = i;
+      else ctrl[0] = i < *p.surv_count ? p.surv_list[i] : int(p.B);
+    }
     __syncthreads();
     const int64_t world = ctrl[0];
     if (world >= p.B) break;
 
-    const float* A0 = p.cells0 + world * cells_per_world;
+    // a resumed world continues from its parked state (in the output buffer) and carry
+    const float* A0 = (p.resume ? p.cells : p.cells0) + world * cells_per_world;
 #pragma unroll
     for (int m = 0; m < P; ++m) {
       a0[m] = A0[(2 * r0) * N + TPR * m + t0];
@@ -178,19 +188,24 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
     const float gk = inv_s * 0.84932180028801904272f;  // sqrt(log2(e) / 2)
     const float dtw = p.dt[world * p.dt_bstride];
     if (tid == 0) {
-      for (int i = 0; i < 3; ++i) { carry.shift[i] = 0; carry.centroid[i] = 0.f; }
-      carry.angle = 0.f;
-      carry.t = 0;
-      carry.cont = 1;
-      carry.n = 0;
+      if (p.resume) {
+        carry = p.park[world];
+      } else {
+        for (int i = 0; i < 3; ++i) { carry.shift[i] = 0; carry.centroid[i] = 0.f; }
+        carry.angle = 0.f;
+        carry.t = 0;
+        carry.cont = 1;
+        carry.n = 0;
+      }
       ctrl[1] = 0;
-      ctrl[2] = 0;
-      ctrl[3] = 0;
+      ctrl[2] = carry.shift[0];
+      ctrl[3] = carry.shift[1];
     }
     __syncthreads();
 
-    int t = 0;
-    while (t < p.max_steps) {
+    int t = carry.t;
+    const int t_end = p.limit < p.max_steps ? p.limit : p.max_steps;
+    while (t < t_end) {
       // The lane indices are laundered through an empty asm each step so that the
       // per-lane LDS/global addresses of the transposes are recomputed inside the
       // loop instead of being hoisted out of it and pinned in registers.
@@ -407,6 +422,9 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
       ++t;  // no barrier: the next step's first one orders thread 0's writes before their readers
     }
 
+    // the last step's stop decision (thread 0 wrote it without a barrier after it)
+    __syncthreads();
+    const bool finished = ctrl[1] != 0 || t >= p.max_steps;
     float* A1 = p.cells + world * cells_per_world;
 #pragma unroll
     for (int m = 0; m < P; ++m) {
@@ -414,15 +432,22 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
       A1[(2 * r0 + 1) * N + TPR * m + t0] = a1[m];
     }
     if (tid == 0) {
-      if (p.n_out) p.n_out[world] = carry.n;
-      if (p.steps_out) p.steps_out[world] = t;
+      if (finished) {
+        if (p.n_out) p.n_out[world] = carry.n;
+        if (p.steps_out) p.steps_out[world] = t;
+      } else {  // still running at the probe's end: park it for the resume launch
+        p.park[world] = carry;
+        p.surv_list[atomicAdd(p.surv_count, 1)] = int(world);
+      }
     }
     __syncthreads();
   }
 }
 
 template <int UNUSED>
-__global__ void zero_i32(int32_t* p) { *p = 0; }
+__global__ void zero_i32(int32_t* p) {
+  if (threadIdx.x < 4) p[threadIdx.x] = 0;
+}
 
 template <int N, int GF, bool HIST, bool V1>
 int launch_k(const FusedParams& p, int64_t B, hipStream_t stream) {
@@ -437,8 +462,23 @@ int launch_k(const FusedParams& p, int64_t B, hipStream_t stream) {
   int64_t grid = int64_t(cus) * per_cu;
   if (grid > B) grid = B;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL(zero_i32<0>, dim3(1), dim3(1), 0, stream, p.queue);
-  hipLaunchKernelGGL((fused2d_kernel<N, GF, HIST, V1>), dim3((unsigned)grid), dim3(F2<N>::NT), 0, stream, p);
+  // counters: [0] probe queue head, [1] resume queue head, [2] survivor count
+  int32_t* ctr = p.queue;
+  hipLaunchKernelGGL(zero_i32<0>, dim3(1), dim3(64), 0, stream, ctr);
+  const bool probe = p.stop_policy != LX_STOP_NONE && p.max_steps > 256;
+  FusedParams p1 = p;
+  p1.queue = ctr;
+  p1.surv_count = ctr + 2;
+  p1.resume = 0;
+  p1.limit = probe ? p.max_steps / 8 : p.max_steps;
+  hipLaunchKernelGGL((fused2d_kernel<N, GF, HIST, V1>), dim3((unsigned)grid), dim3(F2<N>::NT), 0, stream, p1);
+  if (probe) {
+    FusedParams p2 = p1;
+    p2.queue = ctr + 1;
+    p2.resume = 1;
+    p2.limit = p.max_steps;
+    hipLaunchKernelGGL((fused2d_kernel<N, GF, HIST, V1>), dim3((unsigned)grid), dim3(F2<N>::NT), 0, stream, p2);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return lx_fail(LX_EDEVICE, "fused launch failed: %s", hipGetErrorString(e));
   return LX_OK;
@@ -483,7 +523,14 @@ int launch(const lx_rule* rule, const lx_run_args* a, hipStream_t stream) {
   p.cells_hist = a->cells_hist_d;
   p.field_hist = a->field_hist_d;
   p.pot_hist = a->potential_hist_d;
-  p.queue = a->queue_d;
+  const lx_run_scratch sl = lx_run_scratch_layout(a->B);
+  char* scr = static_cast<char*>(a->scratch_d);
+  p.queue = reinterpret_cast<int32_t*>(scr);
+  p.surv_count = p.queue + 2;
+  p.surv_list = reinterpret_cast<int32_t*>(scr + sl.list);
+  p.park = reinterpret_cast<lx_carry*>(scr + sl.park);
+  p.limit = a->max_steps;
+  p.resume = 0;
   p.state_fn = rule->state_fn;
   p.average = rule->average;
   const bool hist = a->cells_hist_d || a->field_hist_d || a->potential_hist_d;

@@ -55,6 +55,19 @@ struct lx_stat_carry {
   float part[LX_STAT_PARTS][LX_NSUM];
 };
 
+// Scratch of lx_run: 4 counters (queue heads of the two launches, survivor count), the
+// survivor list and one parked carry per world.
+struct lx_run_scratch {
+  size_t list, park, total;
+};
+inline lx_run_scratch lx_run_scratch_layout(int64_t B) {
+  lx_run_scratch l;
+  l.list = 16;
+  l.park = (l.list + size_t(B) * 4 + 255) / 256 * 256;
+  l.total = l.park + size_t(B) * sizeof(lx_carry);
+  return l;
+}
+
 // Per-world reductions one step's statistics need.  Coordinates are the
 // re-centred ones of leniax/statistics.py:28-33,64 (index - shift mod size - size//2).
 struct lx_sums {

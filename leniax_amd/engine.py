@@ -101,7 +101,7 @@ def run_fused(rule, cells0, K_half, k_per_world, gf, w, dt, max_steps, stop='non
     stats = torch.zeros((B, max_steps, nst), dtype=torch.float32, device=dev) if want_stats else None
     n = torch.zeros(B, dtype=torch.int32, device=dev)
     steps = torch.zeros(B, dtype=torch.int32, device=dev)
-    queue = torch.zeros(1, dtype=torch.int32, device=dev)
+    scratch = torch.empty(max(int(_lib.lib().lx_run_scratch_bytes(B)), 16), dtype=torch.uint8, device=dev)
     hist = None
     if want_hist:
         dims = tuple(cells0.shape[2:])
@@ -130,7 +130,7 @@ def run_fused(rule, cells0, K_half, k_per_world, gf, w, dt, max_steps, stop='non
     a.steps_d = steps.data_ptr()
     if hist is not None:
         a.cells_hist_d, a.field_hist_d, a.potential_hist_d = (h.data_ptr() for h in hist)
-    a.queue_d = queue.data_ptr()
+    a.scratch_d = scratch.data_ptr()
     _lib.call('lx_run', ctypes.byref(rule), ctypes.byref(a), stream_ptr())
     return {'cells': cells, 'stats': stats, 'n': n, 'steps': steps, 'hist': hist}
 

@@ -114,3 +114,44 @@ def test_fused_stats_match_oracle():
     np.testing.assert_allclose(st['channel_mass'][:, 0], o_stats['channel_mass'][:, 0, 0], rtol=1e-5)
     assert int(res['n'][0]) == int(o_stats['N'][0])
     assert int(res['steps'][0]) == 64
+
+
+@pytest.mark.parametrize('stop', ['mass', 'leniax'])
+def test_fused_probe_and_resume_equal_one_uninterrupted_run(stop):
+    """With an early-stop policy and > 256 steps lx_run probes every world for max_steps / 8
+    steps, parks the survivors and resumes them in a second launch.  Every executed step must
+    equal the same step of an uninterrupted run (stop policy 'none', history kept): statistics
+    rows bit for bit, and each world's final state equal to the state before its next step."""
+    engine, _lib = _engine()
+    N, B, steps, R, T = 64, 48, 320, 13., 10.
+    rng = np.random.default_rng(7)
+    cells0 = np.zeros((B, 1, N, N), np.float32)
+    for i in range(B):
+        s = int(rng.integers(8, 40))
+        o = (N - s) // 2
+        cells0[i, 0, o:o + s, o:o + s] = rng.random((s, s), dtype=np.float32)
+    K, mp = L.get_kernels_and_mapping(_bump_rule(R=8.), [N, N], 1, 8., np.float32)
+    rule = engine.make_rule([N, N], 1, [0], [_lib.GROWTH_IDS['gaussian']], 0, 1)
+    Kh = engine.prepare_spectra(rule, K, [0])
+    dev = engine.device()
+    c0 = torch.as_tensor(cells0).to(dev)
+    gf = torch.tensor([[[0.15, 0.015]]], device=dev)
+    w = torch.ones(1, 1, 1, device=dev)
+    dt = torch.tensor([np.float32(1) / np.float32(T)], device=dev)
+    print('probe/resume', stop, flush=True)
+    full = engine.run_fused(rule, c0, Kh, False, gf, w, dt, steps, 'none', 0, 8., T, want_stats=True,
+                            want_hist=True)
+    part = engine.run_fused(rule, c0, Kh, False, gf, w, dt, steps, stop, 64, 8., T, want_stats=True)
+    torch.cuda.synchronize()
+    st = part['steps'].cpu().numpy()
+    print('  steps per world: min', st.min(), 'max', st.max(), 'probed out (>', steps // 8, '):', int((st > steps // 8).sum()))
+    assert (st > steps // 8).any() and (st < steps).any()  # both launches did work
+    fs, ps = full['stats'].cpu().numpy(), part['stats'].cpu().numpy()
+    hist = full['hist'][0].cpu().numpy()
+    fin_full = full['cells'].cpu().numpy()
+    fin_part = part['cells'].cpu().numpy()
+    for b in range(B):
+        n = int(st[b])
+        assert np.array_equal(ps[b, :n], fs[b, :n]), b
+        ref = hist[n, b] if n < steps else fin_full[b]
+        assert np.array_equal(fin_part[b], ref), b
