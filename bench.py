@@ -1,9 +1,10 @@
 """Benchmark: batched init search (1024 worlds of 128x128, one channel, one kernel).
 
-One bench step = one fused persistent launch (lx_run) that steps all 1024 worlds up to
-1024 steps each, with per-step statistics (mass, centroid, growth, ...) and the early
+One bench step = one fused persistent run (lx_run: a probe launch that steps every world
+up to 128 steps, then a resume launch of the worlds still running) over all 1024 worlds, up
+to 1024 steps each, with per-step statistics (mass, centroid, growth, ...) and the early
 stop "mass is 0 or covers more than 90% of the grid", retiring worlds at uneven times and
-refilling their slots from the device queue.  Inputs are resident in HBM before timing.
+refilling their slots from a device queue.  Inputs are resident in HBM before timing.
 
 Printed JSON line: whole-job world-steps/s (value), cell-updates/s, the plain per-world
 and plain batched GPU steps for comparison, the roofline of the fused kernel and the CPU
@@ -391,7 +392,8 @@ def main():
         'fused_vs_plain_single_world': full_ws / plain_single,
         'fused_vs_plain_batched': full_ws / plain_batched,
         'roofline': {'bound': 'mfma', 'compute': 'fp32 VALU (no MFMA: no dense contraction on the path)',
-                     'kernel': 'fused2d_kernel<128>', 'achieved': achieved / 1e12, 'peak': PEAK_FP32 / 1e12,
+                     'kernel': 'fused2d_kernel<128> (probe + resume dispatches of one lx_run)',
+                     'achieved': achieved / 1e12, 'peak': PEAK_FP32 / 1e12,
                      'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32, 'traffic': measured_traffic(),
                      'flops_per_world_step': F, 'launch_ms': launch_ms},
     }
