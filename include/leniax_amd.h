@@ -82,6 +82,14 @@ int lx_workspace_size(const lx_rule* rule, int64_t B, size_t* bytes);
 int lx_potential(const lx_rule* rule, const float* state_d, int64_t B, const void* K_half_d,
                  int64_t k_bstride, float* potential_d, void* work_d, void* stream);
 
+/* leniax/core.py:105-146 get_potential (the direct-convolution potential, helpers' fft=False):
+ * potential_d[b][k][y][x] = sum_{i<kh, j<kw} state[b][cin k][(y+i-kh/2) mod H][(x+j-kw/2) mod W] * kern_d[k][i][j],
+ * the depthwise cross-correlation of jax.lax.conv_general_dilated on the wrap-padded state
+ * (padding of leniax/helpers.py:465-481).  kern_d: the K true kernels [K][kh][kw] (leniax's
+ * [C*max_k, 1, kh, kw] without its padding slots).  2-D worlds of any size; kh, kw <= 64. */
+int lx_potential_direct(const lx_rule* rule, const float* state_d, int64_t B, const float* kern_d, int32_t kh,
+                        int32_t kw, float* potential_d, void* stream);
+
 /* leniax/core.py:163-199 get_field (+ weighted_sum/weighted_mean):
  * field_d [B][C][dims] = sum_k w[c][k] g_k(potential_k) (/ sum_k w[c][k]).
  * gf_d [K][2] (m, s) and w_d [C][K] per world with strides gf_bstride / w_bstride floats (0: shared). */

@@ -88,6 +88,7 @@ SIGNATURES = {
     'lx_kernel_spectrum': (ctypes.c_int, [_I32, ctypes.POINTER(_I32), _I64, _P, _P, _P, _P]),
     'lx_workspace_size': (ctypes.c_int, [_R, _I64, ctypes.POINTER(ctypes.c_size_t)]),
     'lx_potential': (ctypes.c_int, [_R, _P, _I64, _P, _I64, _P, _P, _P]),
+    'lx_potential_direct': (ctypes.c_int, [_R, _P, _I64, _P, _I32, _I32, _P, _P]),
     'lx_field': (ctypes.c_int, [_R, _P, _I64, _P, _I64, _P, _I64, _P, _P]),
     'lx_state': (ctypes.c_int, [_R, _P, _P, _I64, _P, _I64, _P, _P]),
     'lx_update': (ctypes.c_int, [_R, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P]),

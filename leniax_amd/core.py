@@ -109,9 +109,32 @@ def get_potential_fft(state, K, tc_indices=None, channel_first=True):
     return pot
 
 
-def get_potential(state, K, padding, tc_indices=None, channel_first=True):
-    """leniax/core.py:105-146 (direct convolution).  Not on the FFT path."""
-    raise NotImplementedError('the direct-convolution potential is not implemented; use fft=True')
+def get_potential(state, K, padding=None, tc_indices=None, channel_first=True):
+    """leniax/core.py:105-146: depthwise cross-correlation of the wrap-padded state with the
+    real kernels ``K [C * max_k, 1, kh, kw]`` (``jax.lax.conv_general_dilated``, 'VALID',
+    ``feature_group_count=C``), filtered by ``tc_indices``.  ``padding`` is the one
+    ``helpers.build_get_potential_fn`` builds (``kernel_size // 2`` before on each axis)."""
+    if not channel_first:
+        raise NotImplementedError('leniax_amd keeps channel-first arrays ([N, C, *world])')
+    A = engine.as_device(state)
+    Kr = engine.as_device(K)
+    N, C = A.shape[:2]
+    if A.dim() != 4 or Kr.dim() != 4 or Kr.shape[1] != 1 or Kr.shape[0] % C:
+        raise ValueError(f'get_potential: state {tuple(A.shape)} and kernels {tuple(Kr.shape)} '
+                         f'must be [N, C, H, W] and [C * max_k, 1, kh, kw]')
+    kh, kw = int(Kr.shape[2]), int(Kr.shape[3])
+    if padding is not None:
+        before = [int(p[0]) for p in padding][2:]
+        if before != [kh // 2, kw // 2]:
+            raise ValueError(f'padding {padding} is not the kernel-centred one (size // 2 before)')
+    maxk = Kr.shape[0] // C
+    slots = _slots(C, maxk, tc_indices)
+    kern = Kr[list(slots), 0].contiguous()
+    rule = _rule(A.shape[2:], C, len(slots), [s // maxk for s in slots])
+    pot = torch.empty((N, len(slots)) + tuple(A.shape[2:]), dtype=torch.float32, device=A.device)
+    _lib.call('lx_potential_direct', ctypes.byref(rule), engine.ptr(A), N, engine.ptr(kern), kh, kw,
+              engine.ptr(pot), engine.stream_ptr())
+    return pot
 
 
 # ------------------------------------------------------------------ field

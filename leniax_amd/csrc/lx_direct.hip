@@ -1,0 +1,92 @@
+// Direct-convolution potential (leniax/core.py:105-146 get_potential): the depthwise
+// cross-correlation of the wrap-padded state with small real kernels, as
+// jax.lax.conv_general_dilated(..., 'VALID', feature_group_count=C) computes it, with the
+// padding of leniax/helpers.py:465-481 (before = size // 2 on every axis).
+//
+//   U[b][k][y][x] = sum_{i<kh, j<kw} A[b][cin k][(y + i - ph) mod H][(x + j - pw) mod W] * kern[k][i][j]
+//
+// One workgroup computes a TH x TW tile of one (world, kernel) output: the input tile and its
+// halo are staged in LDS (wrap indexing), each thread keeps a 1 x 4 strip of outputs and slides
+// a 4-wide register window along each kernel row, so that one LDS read feeds 4 FMAs; the kernel
+// taps are uniform (scalar loads).  Any world size; kernels up to 64 x 64.
+#include "lx_internal.h"
+
+namespace {
+
+constexpr int TH = 16, TW = 64, MAXK = 64;
+
+struct DirectArgs {
+  const float* A;
+  const float* kern;
+  float* U;
+  int H, W, C, K, kh, kw, ph, pw;
+  int cin[LX_MAX_K];
+};
+
+__global__ __launch_bounds__(256) void direct_kernel(DirectArgs a) {
+  extern __shared__ float tile[];  // (TH + kh - 1) rows x (TW + kw) columns
+  const int tcols = TW + a.kw;     // one spare column: the window's last shift reads it
+  const int trows = TH + a.kh - 1;
+  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  const int64_t bk = blockIdx.z;
+  const int64_t b = bk / a.K;
+  const int k = int(bk - b * a.K);
+  const float* A = a.A + (b * a.C + a.cin[k]) * int64_t(a.H) * a.W;
+  for (int idx = threadIdx.x; idx < trows * tcols; idx += 256) {
+    const int r = idx / tcols, c = idx - r * tcols;
+    int gy = (y0 + r - a.ph) % a.H;
+    gy += gy < 0 ? a.H : 0;
+    int gx = (x0 + c - a.pw) % a.W;
+    gx += gx < 0 ? a.W : 0;
+    tile[idx] = A[int64_t(gy) * a.W + gx];
+  }
+  __syncthreads();
+  const int ty = threadIdx.x / (TW / 4), tx4 = (threadIdx.x % (TW / 4)) * 4;
+  const float* kk = a.kern + int64_t(k) * a.kh * a.kw;
+  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
+  for (int i = 0; i < a.kh; ++i) {
+    const float* row = tile + (ty + i) * tcols + tx4;
+    float w0 = row[0], w1 = row[1], w2 = row[2], w3 = row[3];
+    const float* kr = kk + i * a.kw;
+    for (int j = 0; j < a.kw; ++j) {
+      const float kv = kr[j];
+      acc0 = fmaf(kv, w0, acc0);
+      acc1 = fmaf(kv, w1, acc1);
+      acc2 = fmaf(kv, w2, acc2);
+      acc3 = fmaf(kv, w3, acc3);
+      w0 = w1; w1 = w2; w2 = w3; w3 = row[j + 4];
+    }
+  }
+  const int y = y0 + ty;
+  if (y < a.H) {
+    float* out = a.U + (bk * a.H + y) * int64_t(a.W) + x0 + tx4;
+    const float v[4] = {acc0, acc1, acc2, acc3};
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (x0 + tx4 + e < a.W) out[e] = v[e];
+  }
+}
+
+}  // namespace
+
+extern "C" int lx_potential_direct(const lx_rule* rule, const float* state_d, int64_t B, const float* kern_d,
+                                   int32_t kh, int32_t kw, float* potential_d, void* stream) {
+  int rc = lx_check_rule(rule);
+  if (rc) return rc;
+  if (!state_d || !kern_d || !potential_d) return lx_fail(LX_EINVAL, "NULL pointer");
+  if (rule->ndim != 2) return lx_fail(LX_ENOTSUP, "lx_potential_direct: 2-D worlds only");
+  if (kh < 1 || kw < 1 || kh > MAXK || kw > MAXK) return lx_fail(LX_ENOTSUP, "kernel %dx%d (1..%d)", kh, kw, MAXK);
+  if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
+  if (B == 0) return LX_OK;
+  if (B * rule->K > 65535) return lx_fail(LX_ENOTSUP, "lx_potential_direct: B * K > 65535");
+  DirectArgs a;
+  a.A = state_d; a.kern = kern_d; a.U = potential_d;
+  a.H = rule->dims[0]; a.W = rule->dims[1]; a.C = rule->C; a.K = rule->K;
+  a.kh = kh; a.kw = kw; a.ph = kh / 2; a.pw = kw / 2;
+  for (int k = 0; k < LX_MAX_K; ++k) a.cin[k] = k < rule->K ? rule->cin[k] : 0;
+  const size_t lds = size_t(TH + kh - 1) * size_t(TW + kw) * sizeof(float);
+  const dim3 grid(unsigned((a.W + TW - 1) / TW), unsigned((a.H + TH - 1) / TH), unsigned(B * rule->K));
+  hipLaunchKernelGGL(direct_kernel, grid, dim3(256), lds, (hipStream_t)stream, a);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LX_OK : lx_fail(LX_EDEVICE, "direct potential launch: %s", hipGetErrorString(e));
+}

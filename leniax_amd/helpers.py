@@ -58,7 +58,10 @@ def build_get_potential_fn(kernel_shape, true_channels: Optional[List[bool]] = N
     """leniax/helpers.py:430-488."""
     tc_indices = None if true_channels is None else tuple(i for i, t in enumerate(true_channels) if t)
     if fft is not True:
-        return functools.partial(core.get_potential, padding=None, tc_indices=tc_indices, channel_first=channel_first)
+        # leniax/helpers.py:463-481: wrap padding of size // 2 before, the rest after
+        pad = [(0, 0), (0, 0)] + [(d // 2, d // 2 - 1) if d % 2 == 0 else (d // 2, d // 2) for d in kernel_shape[2:]]
+        return functools.partial(core.get_potential, padding=tuple(pad), tc_indices=tc_indices,
+                                 channel_first=channel_first)
     return functools.partial(core.get_potential_fft, tc_indices=tc_indices, channel_first=channel_first)
 
 

@@ -286,6 +286,36 @@ def get_potential_fft(state, K, tc_indices=None, workers=1):
     return conv_out
 
 
+def conv_padding(kernel_shape):
+    """leniax/helpers.py:463-481 (channel first): wrap padding per world axis."""
+    pad = [(0, 0), (0, 0)]
+    for d in kernel_shape[2:]:
+        pad.append((d // 2, d // 2 - 1) if d % 2 == 0 else (d // 2, d // 2))
+    return tuple(pad)
+
+
+def get_potential(state, K, padding, tc_indices=None):
+    """leniax/core.py:105-146: jnp.pad(mode='wrap') then lax.conv_general_dilated 'VALID' with
+    feature_group_count = C (cross-correlation, no kernel flip; output channel o reads input
+    channel o // max_k).  K: [C * max_k, 1, *kernel_dims]."""
+    f = state.dtype.type
+    N, C = state.shape[:2]
+    maxk = K.shape[0] // C
+    padded = np.pad(state, padding, mode='wrap')
+    kd = K.shape[2:]
+    out = np.zeros((N, K.shape[0]) + state.shape[2:], dtype=state.dtype)
+    for o in range(K.shape[0]):
+        c = o // maxk
+        acc = np.zeros((N,) + state.shape[2:], dtype=state.dtype)
+        for idx in np.ndindex(*kd):
+            sl = (slice(None), c) + tuple(slice(i, i + n) for i, n in zip(idx, state.shape[2:]))
+            acc += f(K[(o, 0) + idx]) * padded[sl]
+        out[:, o] = acc
+    if tc_indices is not None:
+        out = np.take(out, np.asarray(tc_indices), axis=1)
+    return out
+
+
 def weighted_sum(fields, weights):
     """leniax/core.py:202-225."""
     f = fields.dtype.type

@@ -1,0 +1,67 @@
+"""GPU direct-convolution potential (lx_potential_direct) against the oracle, and leniax's
+fft=False pipeline end to end."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from oracle import lenia as L
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip('torch')
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def test_direct_known_answer_on_device():
+    """tests/test_core.py:55-101 through helpers.build_get_potential_fn(fft=False)."""
+    from leniax_amd import helpers
+    cells = np.ones((1, 2, 2, 2), np.float32)
+    cells[0, 0], cells[0, 1] = 0.1, 0.2
+    K = np.ones((4, 1, 3, 3), np.float32)
+    K[0], K[1], K[2], K[3] = 0.1, 0.2, 0.3, 0.0
+    fn = helpers.build_get_potential_fn(K.shape, [True, True, True, False], fft=False)
+    pot = _np(fn(torch.as_tensor(cells).cuda(), torch.as_tensor(K).cuda()))
+    assert pot.shape == (1, 3, 2, 2)
+    np.testing.assert_allclose(pot[0], np.array([0.09, 0.18, 0.54])[:, None, None] * np.ones((3, 2, 2)), atol=1e-7)
+
+
+@pytest.mark.parametrize('N,C,maxk,H,W,kh,kw', [(3, 2, 2, 64, 64, 5, 5), (2, 1, 1, 128, 96, 27, 27),
+                                                  (1, 2, 2, 200, 150, 26, 26), (2, 3, 1, 33, 17, 4, 9),
+                                                  (1, 1, 1, 8, 8, 13, 13)])
+def test_direct_potential_matches_oracle(N, C, maxk, H, W, kh, kw):
+    from leniax_amd import core
+    print('direct', (N, C, maxk, H, W, kh, kw), flush=True)
+    rng = np.random.default_rng(kh * 100 + H)
+    A = rng.uniform(size=(N, C, H, W)).astype(np.float32)
+    K = rng.normal(size=(C * maxk, 1, kh, kw)).astype(np.float32)
+    tc = None
+    if maxk == 2:
+        K[-1] = 0
+        tc = tuple(range(C * maxk - 1))
+    pad = L.conv_padding(K.shape)
+    ref = L.get_potential(A.astype(np.float64), K.astype(np.float64), pad, tc)
+    got = _np(core.get_potential(torch.as_tensor(A).cuda(), torch.as_tensor(K).cuda(), pad, tc))
+    err = float(np.abs(got - ref).max() / np.abs(ref).max())
+    print('  rel err', err)
+    assert err < 2e-6
+
+
+def test_fft_false_pipeline_reproduces_the_golden_frame():
+    """The orbium fixture (tests/test_pipeline.py:18-35) run with the direct potential
+    (leniax's fft=False): its kernel is symmetric, so correlation and the FFT convolution agree."""
+    from leniax_amd import helpers
+    meta, cells0, frame = load_golden('orbium')
+    cfg = {
+        'world_params': {'nb_dims': 2, 'nb_channels': 1, 'R': meta['R'], 'T': meta['T'],
+                         'weighted_average': True, 'get_state_fn_slug': 'v1', 'scale': 1.},
+        'render_params': {'world_size': meta['world_size']},
+        'kernels_params': [dict(k) for k in meta['kernels_params']],
+        'run_params': {'cells': cells0[0], 'max_run_iter': meta['max_run_iter']},
+    }
+    cells, _, _, _ = helpers.init_and_run(None, cfg, with_jit=True, fft=False)
+    err = float(np.abs(_np(cells[-1, 0]) - frame).max())
+    print('orbium fft=False golden err', err)
+    assert err < 1.5e-4
