@@ -90,6 +90,14 @@ int lx_potential(const lx_rule* rule, const float* state_d, int64_t B, const voi
 int lx_potential_direct(const lx_rule* rule, const float* state_d, int64_t B, const float* kern_d, int32_t kh,
                         int32_t kw, float* potential_d, void* stream);
 
+/* Reverse of lx_potential_direct (what jax.grad gives through leniax/core.py:105-146): with
+ * dpot_d = dL/dpotential [B][K][H][W], dstate_d [B][C][H][W] = dL/dstate (overwritten: for each
+ * channel the correlation of its kernels' dpot with the flipped kernels) and dkern_d [K][kh][kw] =
+ * dL/dkernel summed over worlds (overwritten).  Either output may be NULL. */
+int lx_potential_direct_adjoint(const lx_rule* rule, const float* state_d, int64_t B, const float* kern_d,
+                                int32_t kh, int32_t kw, const float* dpot_d, float* dstate_d, float* dkern_d,
+                                void* stream);
+
 /* leniax/core.py:163-199 get_field (+ weighted_sum/weighted_mean):
  * field_d [B][C][dims] = sum_k w[c][k] g_k(potential_k) (/ sum_k w[c][k]).
  * gf_d [K][2] (m, s) and w_d [C][K] per world with strides gf_bstride / w_bstride floats (0: shared). */

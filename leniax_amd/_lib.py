@@ -89,6 +89,7 @@ SIGNATURES = {
     'lx_workspace_size': (ctypes.c_int, [_R, _I64, ctypes.POINTER(ctypes.c_size_t)]),
     'lx_potential': (ctypes.c_int, [_R, _P, _I64, _P, _I64, _P, _P, _P]),
     'lx_potential_direct': (ctypes.c_int, [_R, _P, _I64, _P, _I32, _I32, _P, _P]),
+    'lx_potential_direct_adjoint': (ctypes.c_int, [_R, _P, _I64, _P, _I32, _I32, _P, _P, _P, _P]),
     'lx_field': (ctypes.c_int, [_R, _P, _I64, _P, _I64, _P, _I64, _P, _P]),
     'lx_state': (ctypes.c_int, [_R, _P, _P, _I64, _P, _I64, _P, _P]),
     'lx_update': (ctypes.c_int, [_R, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P]),

@@ -536,6 +536,43 @@ class _Potential(torch.autograd.Function):
         return lam, (sc.grad_K(dspec, K) if dspec is not None else None), None
 
 
+class _PotentialDirect(torch.autograd.Function):
+    """get_potential (leniax/core.py:105-146) with its adjoint (lx_potential_direct_adjoint)."""
+
+    @staticmethod
+    def forward(ctx, state, K, padding, tc_indices):
+        A = engine.as_device(state)
+        Kr = engine.as_device(K)
+        pot = core.get_potential(A, Kr, padding, tc_indices)
+        ctx.meta = tc_indices
+        ctx.save_for_backward(A, Kr)
+        return pot
+
+    @staticmethod
+    def backward(ctx, g):
+        A, Kr = ctx.saved_tensors
+        N, C = A.shape[:2]
+        maxk = Kr.shape[0] // C
+        slots = core._slots(C, maxk, ctx.meta)
+        kh, kw = int(Kr.shape[2]), int(Kr.shape[3])
+        kern = Kr[list(slots), 0].contiguous()
+        rule = core._rule(A.shape[2:], C, len(slots), [s // maxk for s in slots])
+        need_A, need_K = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+        dA = torch.empty_like(A) if need_A else None
+        dk = torch.empty_like(kern) if need_K else None
+        _lib.call('lx_potential_direct_adjoint', ctypes.byref(rule), engine.ptr(A), N, engine.ptr(kern), kh, kw,
+                  engine.ptr(g.contiguous()), engine.ptr(dA), engine.ptr(dk), engine.stream_ptr())
+        dK = None
+        if need_K:
+            dK = torch.zeros_like(Kr)
+            dK[list(slots), 0] = dk
+        return dA, dK, None, None
+
+
+def get_potential(state, K, padding=None, tc_indices=None):
+    return _PotentialDirect.apply(state, K, padding, tc_indices)
+
+
 class _Field(torch.autograd.Function):
     """get_field (leniax/core.py:163-199) with its adjoint (lx_adjoint_pointwise, lambda = 0)."""
 

@@ -116,6 +116,9 @@ def get_potential(state, K, padding=None, tc_indices=None, channel_first=True):
     ``helpers.build_get_potential_fn`` builds (``kernel_size // 2`` before on each axis)."""
     if not channel_first:
         raise NotImplementedError('leniax_amd keeps channel-first arrays ([N, C, *world])')
+    if _needs_grad(state, K):
+        from . import autograd
+        return autograd.get_potential(state, K, padding, tc_indices)
     A = engine.as_device(state)
     Kr = engine.as_device(K)
     N, C = A.shape[:2]

@@ -67,7 +67,122 @@ __global__ __launch_bounds__(256) void direct_kernel(DirectArgs a) {
   }
 }
 
+// Reverse of the direct potential, state side: dA[b][c] = sum over kernels k fed by c of the
+// correlation of dU[b][k] with the flipped kernel, wrap-padded with kh - 1 - ph before (grid:
+// tiles x B * C).  The tile of each kernel's dU is staged in LDS in turn.
+__global__ __launch_bounds__(256) void direct_adj_kernel(DirectArgs a, const float* dU, float* dA) {
+  extern __shared__ float tile[];
+  const int tcols = TW + a.kw;
+  const int trows = TH + a.kh - 1;
+  const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+  const int64_t bc = blockIdx.z;
+  const int64_t b = bc / a.C;
+  const int c = int(bc - b * a.C);
+  const int qh = a.kh - 1 - a.ph, qw = a.kw - 1 - a.pw;
+  const int ty = threadIdx.x / (TW / 4), tx4 = (threadIdx.x % (TW / 4)) * 4;
+  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
+  for (int k = 0; k < a.K; ++k) {
+    if (a.cin[k] != c) continue;
+    const float* G = dU + (b * a.K + k) * int64_t(a.H) * a.W;
+    __syncthreads();  // the previous kernel's tile is consumed
+    for (int idx = threadIdx.x; idx < trows * tcols; idx += 256) {
+      const int r = idx / tcols, cc = idx - r * tcols;
+      int gy = (y0 + r - qh) % a.H;
+      gy += gy < 0 ? a.H : 0;
+      int gx = (x0 + cc - qw) % a.W;
+      gx += gx < 0 ? a.W : 0;
+      tile[idx] = G[int64_t(gy) * a.W + gx];
+    }
+    __syncthreads();
+    const float* kk = a.kern + int64_t(k) * a.kh * a.kw;
+    for (int i = 0; i < a.kh; ++i) {
+      const float* row = tile + (ty + i) * tcols + tx4;
+      float w0 = row[0], w1 = row[1], w2 = row[2], w3 = row[3];
+      const float* kr = kk + (a.kh - 1 - i) * a.kw;  // flipped rows and columns
+      for (int j = 0; j < a.kw; ++j) {
+        const float kv = kr[a.kw - 1 - j];
+        acc0 = fmaf(kv, w0, acc0);
+        acc1 = fmaf(kv, w1, acc1);
+        acc2 = fmaf(kv, w2, acc2);
+        acc3 = fmaf(kv, w3, acc3);
+        w0 = w1; w1 = w2; w2 = w3; w3 = row[j + 4];
+      }
+    }
+  }
+  const int y = y0 + ty;
+  if (y < a.H) {
+    float* out = dA + (bc * a.H + y) * int64_t(a.W) + x0 + tx4;
+    const float v[4] = {acc0, acc1, acc2, acc3};
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (x0 + tx4 + e < a.W) out[e] = v[e];
+  }
+}
+
+// Reverse of the direct potential, kernel side: dK[k][i][j] = sum over worlds and cells of
+// dU[b][k][y][x] A[b][cin k][(y + i - ph) mod H][(x + j - pw) mod W] (grid: kh * kw x K; one block
+// per tap, so the sum needs no atomics and is deterministic).
+__global__ __launch_bounds__(256) void direct_kgrad_kernel(DirectArgs a, int64_t B, const float* dU, float* dK) {
+  const int tap = blockIdx.x;
+  const int i = tap / a.kw, j = tap - i * a.kw;
+  const int k = blockIdx.y;
+  const int64_t n = int64_t(a.H) * a.W;
+  float acc = 0.f;
+  for (int64_t b = 0; b < B; ++b) {
+    const float* G = dU + (b * a.K + k) * n;
+    const float* A = a.A + (b * a.C + a.cin[k]) * n;
+    for (int64_t p = threadIdx.x; p < n; p += 256) {
+      const int y = int(p / a.W), x = int(p - int64_t(y) * a.W);
+      int gy = (y + i - a.ph) % a.H;
+      gy += gy < 0 ? a.H : 0;
+      int gx = (x + j - a.pw) % a.W;
+      gx += gx < 0 ? a.W : 0;
+      acc = fmaf(G[p], A[int64_t(gy) * a.W + gx], acc);
+    }
+  }
+  __shared__ float red[4];
+  const float s = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) dK[(int64_t(k) * a.kh + i) * a.kw + j] = red[0] + red[1] + red[2] + red[3];
+}
+
+DirectArgs direct_args(const lx_rule* rule, const float* A, const float* kern, int kh, int kw) {
+  DirectArgs a;
+  a.A = A; a.kern = kern; a.U = nullptr;
+  a.H = rule->dims[0]; a.W = rule->dims[1]; a.C = rule->C; a.K = rule->K;
+  a.kh = kh; a.kw = kw; a.ph = kh / 2; a.pw = kw / 2;
+  for (int k = 0; k < LX_MAX_K; ++k) a.cin[k] = k < rule->K ? rule->cin[k] : 0;
+  return a;
+}
+
 }  // namespace
+
+extern "C" int lx_potential_direct_adjoint(const lx_rule* rule, const float* state_d, int64_t B,
+                                           const float* kern_d, int32_t kh, int32_t kw, const float* dpot_d,
+                                           float* dstate_d, float* dkern_d, void* stream) {
+  int rc = lx_check_rule(rule);
+  if (rc) return rc;
+  if (!kern_d || !dpot_d || (!dstate_d && !dkern_d) || (dkern_d && !state_d)) return lx_fail(LX_EINVAL, "NULL pointer");
+  if (rule->ndim != 2) return lx_fail(LX_ENOTSUP, "lx_potential_direct_adjoint: 2-D worlds only");
+  if (kh < 1 || kw < 1 || kh > MAXK || kw > MAXK) return lx_fail(LX_ENOTSUP, "kernel %dx%d (1..%d)", kh, kw, MAXK);
+  if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
+  if (B == 0) return LX_OK;
+  if (B * rule->C > 65535) return lx_fail(LX_ENOTSUP, "lx_potential_direct_adjoint: B * C > 65535");
+  const DirectArgs a = direct_args(rule, state_d, kern_d, kh, kw);
+  hipStream_t st = (hipStream_t)stream;
+  if (dstate_d) {
+    const size_t lds = size_t(TH + kh - 1) * size_t(TW + kw) * sizeof(float);
+    const dim3 grid(unsigned((a.W + TW - 1) / TW), unsigned((a.H + TH - 1) / TH), unsigned(B * rule->C));
+    hipLaunchKernelGGL(direct_adj_kernel, grid, dim3(256), lds, st, a, dpot_d, dstate_d);
+  }
+  if (dkern_d) {
+    hipLaunchKernelGGL(direct_kgrad_kernel, dim3(unsigned(kh * kw), unsigned(rule->K)), dim3(256), 0, st, a, B,
+                       dpot_d, dkern_d);
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? LX_OK : lx_fail(LX_EDEVICE, "direct adjoint launch: %s", hipGetErrorString(e));
+}
 
 extern "C" int lx_potential_direct(const lx_rule* rule, const float* state_d, int64_t B, const float* kern_d,
                                    int32_t kh, int32_t kw, float* potential_d, void* stream) {
@@ -79,11 +194,8 @@ extern "C" int lx_potential_direct(const lx_rule* rule, const float* state_d, in
   if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
   if (B == 0) return LX_OK;
   if (B * rule->K > 65535) return lx_fail(LX_ENOTSUP, "lx_potential_direct: B * K > 65535");
-  DirectArgs a;
-  a.A = state_d; a.kern = kern_d; a.U = potential_d;
-  a.H = rule->dims[0]; a.W = rule->dims[1]; a.C = rule->C; a.K = rule->K;
-  a.kh = kh; a.kw = kw; a.ph = kh / 2; a.pw = kw / 2;
-  for (int k = 0; k < LX_MAX_K; ++k) a.cin[k] = k < rule->K ? rule->cin[k] : 0;
+  DirectArgs a = direct_args(rule, state_d, kern_d, kh, kw);
+  a.U = potential_d;
   const size_t lds = size_t(TH + kh - 1) * size_t(TW + kw) * sizeof(float);
   const dim3 grid(unsigned((a.W + TW - 1) / TW), unsigned((a.H + TH - 1) / TH), unsigned(B * rule->K));
   hipLaunchKernelGGL(direct_kernel, grid, dim3(256), lds, (hipStream_t)stream, a);

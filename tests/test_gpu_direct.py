@@ -65,3 +65,27 @@ def test_fft_false_pipeline_reproduces_the_golden_frame():
     err = float(np.abs(_np(cells[-1, 0]) - frame).max())
     print('orbium fft=False golden err', err)
     assert err < 1.5e-4
+
+
+def test_direct_adjoint_dot_product_identities():
+    """The potential is linear in the state and in the kernels, so its adjoint is pinned by
+    <dU, P(A; k)> = <P^T_A dU, A> = <P^T_k dU, k>, evaluated in float64 by the oracle forward."""
+    from leniax_amd import core
+    rng = np.random.default_rng(3)
+    N, C, maxk, H, W, kh, kw = 2, 2, 2, 40, 56, 7, 6
+    A = rng.uniform(size=(N, C, H, W)).astype(np.float32)
+    K = rng.uniform(size=(C * maxk, 1, kh, kw)).astype(np.float32)  # positive: no cancellation in the sums
+    K[-1] = 0
+    tc = tuple(range(C * maxk - 1))
+    pad = L.conv_padding(K.shape)
+    dU = rng.uniform(size=(N, len(tc), H, W)).astype(np.float32)
+    tA = torch.as_tensor(A).cuda().requires_grad_()
+    tK = torch.as_tensor(K).cuda().requires_grad_()
+    pot = core.get_potential(tA, tK, pad, tc)
+    (pot * torch.as_tensor(dU).cuda()).sum().backward()
+    lhs = float(np.sum(dU.astype(np.float64) * L.get_potential(A.astype(np.float64), K.astype(np.float64), pad, tc)))
+    rhs_A = float(np.sum(_np(tA.grad).astype(np.float64) * A))
+    rhs_K = float(np.sum(_np(tK.grad).astype(np.float64) * K))
+    print('  <dU,P> %.6f  <dA,A> %.6f  <dK,K> %.6f' % (lhs, rhs_A, rhs_K))
+    assert abs(lhs - rhs_A) <= 1e-5 * abs(lhs) and abs(lhs - rhs_K) <= 1e-5 * abs(lhs)
+    assert float(np.abs(_np(tK.grad)[-1]).max()) == 0.0  # the padding slot gets no gradient
