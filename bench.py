@@ -253,6 +253,28 @@ def measured_traffic():
         return json.load(fh).get('hbm_bytes_per_launch')
 
 
+def direct_rate(worlds=256, n=128, reps=10):
+    """Direct-convolution potential (leniax get_potential, fft=False) of the rule above: the
+    cropped 25 x 25 kernel of R=13 over `worlds` worlds of n x n; 2 flops per tap and cell."""
+    import torch
+    from leniax_amd import core, kernels
+    dev = torch.device('cuda', 0)
+    K, _ = kernels.get_kernels_and_mapping(kernel_params(), [n, n], 1, R, fft=False)  # [1, 1, kh, kw]
+    A = torch.as_tensor(make_worlds(worlds, n)).to(dev)
+    pad = ((0, 0), (0, 0)) + tuple((d // 2, d // 2 - 1) if d % 2 == 0 else (d // 2, d // 2) for d in K.shape[2:])
+    core.get_potential(A, K, pad)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        core.get_potential(A, K, pad)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / reps
+    flops = 2.0 * worlds * n * n * K.shape[2] * K.shape[3]
+    return {'config': f'{worlds} worlds of {n}x{n}, {K.shape[2]}x{K.shape[3]} kernel', 'ms': sec * 1e3,
+            'world_potentials_per_s': worlds / sec, 'tflops': flops / sec / 1e12,
+            'frac_of_fp32_peak': flops / sec / PEAK_FP32}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -357,6 +379,8 @@ def main():
     c4 = c4_adjoint_rate()
     log(f"3-D adjoint 16 x 96^3, 128 steps: forward {c4['fwd_ms']:.1f} ms, forward+backward "
         f"{c4['fwd_bwd_ms']:.1f} ms ({c4['ratio']:.2f}x)")
+    dr = direct_rate()
+    log(f"direct potential {dr['config']}: {dr['ms']:.3f} ms, {dr['tflops']:.1f} TFLOP/s")
     c2 = c2_rate()
     log(f"3C6K 256 x 256^2 plain step + statistics: {c2['ms_per_step']:.2f} ms/step, "
         f"{c2['world_steps_per_s']:.0f} world-steps/s")
@@ -387,6 +411,7 @@ def main():
         'gradient_256x256_64_steps': adj,
         'adjoint_3d_16x96cubed_128_steps': c4,
         'plain_3c6k_256_worlds_256x256': c2,
+        'direct_potential': dr,
         'plain_single_world_steps_per_s': plain_single,
         'plain_batched_world_steps_per_s': plain_batched,
         'fused_vs_plain_single_world': full_ws / plain_single,

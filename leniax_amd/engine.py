@@ -46,6 +46,17 @@ def as_device(x, dtype=torch.float32):
     return t.to(device=device(), dtype=dtype).contiguous()
 
 
+def as_kernels(K):
+    """Device kernels keeping their kind: complex64 spectra (FFT potential) or float32 real
+    kernels ``[C * max_k, 1, *kernel_dims]`` (direct potential)."""
+    if isinstance(K, torch.Tensor):
+        cplx = K.is_complex()
+    else:
+        cplx = np.iscomplexobj(np.asarray(K)) if not hasattr(K, '__dlpack__') else \
+            torch.from_dlpack(K).is_complex()
+    return as_device(K, torch.complex64 if cplx else torch.float32)
+
+
 def make_rule(dims, C, cin, gf_ids, state_fn=0, average=1):
     nd = len(dims)
     if nd not in (2, 3):

@@ -92,7 +92,7 @@ def run_scan(rng_key, cells0, K, gf_params, kernels_weight_per_channel, T, max_r
     """leniax/runner.py:119-164.  Returns ``cells [T, N, C, ...]`` (state before each step),
     ``field``, ``potential`` and the stats dict (``[T, N]`` arrays, ``N`` from the heuristics)."""
     cells0 = engine.as_device(cells0)
-    K = engine.as_device(K, torch.complex64)
+    K = engine.as_kernels(K)
     fr = _fused_rule(update_fn, compute_stats_fn, cells0, K)
     if fr is None:
         return _run_scan_generic(rng_key, cells0, K, gf_params, kernels_weight_per_channel, T, max_run_iter, R,
@@ -154,7 +154,7 @@ def run_scan_mem_optimized(rng_key, cells0, K, gf_params, kernels_weight_per_cha
     """
     cells0 = engine.as_device(cells0)
     S, N_init = cells0.shape[:2]
-    K = engine.as_device(K, torch.complex64)
+    K = engine.as_kernels(K)
     Kf = K.reshape((S,) + tuple(K.shape[-(cells0.dim() - 1):]))  # [S, C, maxk, *world]
     fr = _fused_rule(update_fn, compute_stats_fn, cells0[0], Kf)
     if fr is None:
