@@ -359,23 +359,49 @@ LX_DEV void fft_lanes(float2* x, int t, int lane, const float2* tw) {
 // (k1 = J t + j) and transforms them over the TPR rows in registers.
 //  forward: in  x[m] = data[TPR m + t];  out x[j TPR + k2] = DATA[(J t + j) + P k2]
 //  inverse: in  x[j TPR + k2] = DATA[(J t + j) + P k2];  out x[m] = data[TPR m + t] (x TPR P)
-// ``tw`` = exp(-2 pi i j / (TPR P)); ``xch``: TPR rows of XRS(P) complex, private to the
-// group.  The group lives in one wave, whose LDS accesses complete in issue order; the
-// wavefront-scope fences keep the compiler from moving them across the exchange.
+// ``twl``: row t of the lane twiddle table, twl[k] = exp(-2 pi i t k / (TPR P)) for k < P
+// (twl_row(P) elements per row, 16-byte aligned): the lane reads its twiddles at constant
+// offsets from one base, two per 16-byte load.  ``xch``: TPR rows of xch_row(P) complex,
+// private to the group.  The group lives in one wave, whose LDS accesses complete in issue
+// order; the wavefront-scope fences keep the compiler from moving them across the exchange.
 constexpr int xch_row(int P) { return P + 2; }  // row stride: 16-byte aligned pairs, rotated banks
+constexpr int twl_row(int P) { return P + 2; }  // lane twiddle row stride (rotated banks)
+
+// Row t of the lane twiddle table for length L = TPR P, entry k = exp(-2 pi i t k / L).
+LX_DEV float2 twl_entry(int L, int t, int k) {
+  double sn, cs;
+  sincospi(2.0 * double((t * k) % L) / double(L), &sn, &cs);
+  return make_float2((float)cs, (float)-sn);
+}
+
+template <int P>
+LX_DEV void twl_mul(float2* x, const float2* twl) {
+  static_assert(P % 2 == 0, "lane twiddle rows are read in pairs");
+#pragma unroll
+  for (int k = 0; k < P; k += 2) {
+    const float4 w = *reinterpret_cast<const float4*>(twl + k);
+    if (k > 0) x[k] = cmul(x[k], make_float2(w.x, w.y));
+    x[k + 1] = cmul(x[k + 1], make_float2(w.z, w.w));
+  }
+}
+
+template <int P>
+LX_DEV void twl_mulc(float2* x, const float2* twl) {
+#pragma unroll
+  for (int k = 0; k < P; k += 2) {
+    const float4 w = *reinterpret_cast<const float4*>(twl + k);
+    if (k > 0) x[k] = cmulc(x[k], make_float2(w.x, w.y));
+    x[k + 1] = cmulc(x[k + 1], make_float2(w.z, w.w));
+  }
+}
 
 template <int P, int TPR, bool INV>
-LX_DEV void fft_xlds(float2* x, int t, const float2* tw, float2* xch) {
+LX_DEV void fft_xlds(float2* x, int t, const float2* twl, float2* xch) {
   constexpr int J = P / TPR, RS = xch_row(P);
   static_assert(P % TPR == 0 && J >= 1, "fft_xlds needs P >= TPR");
   if constexpr (!INV) {
     fft_reg<P, false>(x);
-    int j = 0;
-#pragma unroll
-    for (int k = 1; k < P; ++k) {
-      j += t;  // t * k < TPR * P
-      x[k] = cmul(x[k], tw[j]);
-    }
+    twl_mul<P>(x, twl);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 #pragma unroll
     for (int k = 0; k < P; k += 2) {
@@ -426,12 +452,7 @@ LX_DEV void fft_xlds(float2* x, int t, const float2* tw, float2* xch) {
       x[k + 1] = make_float2(v.z, v.w);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    int j = 0;
-#pragma unroll
-    for (int k = 1; k < P; ++k) {
-      j += t;
-      x[k] = cmulc(x[k], tw[j]);
-    }
+    twl_mulc<P>(x, twl);
     fft_reg<P, true>(x);
   }
 }
@@ -617,5 +638,33 @@ LX_DEV float growth_val_grad(int kind, float m, float s, float U, float& dU, flo
 LX_DEV float wave_sum(float v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum over the 64 lanes of a wave, complete in lane 63 only (the other lanes hold partial
+// sums): DPP butterflies within each row of 16 lanes (quad permutes, half-row and row
+// mirrors), then the row broadcasts of lanes 15 and 31.  No LDS traffic.
+template <typename T>
+LX_DEV T wave_sum_last(T v) {
+  static_assert(sizeof(T) == 4, "32-bit values");
+  auto mv = [](T a, int ctrl_sel) -> T {
+    const int i = __builtin_bit_cast(int, a);
+    int r;
+    switch (ctrl_sel) {
+      case 0: r = __builtin_amdgcn_update_dpp(0, i, LX_QP(1, 0, 3, 2), 0xF, 0xF, false); break;
+      case 1: r = __builtin_amdgcn_update_dpp(0, i, LX_QP(2, 3, 0, 1), 0xF, 0xF, false); break;
+      case 2: r = __builtin_amdgcn_update_dpp(0, i, 0x141, 0xF, 0xF, false); break;  // row_half_mirror
+      case 3: r = __builtin_amdgcn_update_dpp(0, i, 0x140, 0xF, 0xF, false); break;  // row_mirror
+      case 4: r = __builtin_amdgcn_update_dpp(0, i, 0x142, 0xA, 0xF, false); break;  // row_bcast:15 -> rows 1, 3
+      default: r = __builtin_amdgcn_update_dpp(0, i, 0x143, 0xC, 0xF, false); break; // row_bcast:31 -> rows 2, 3
+    }
+    return __builtin_bit_cast(T, r);
+  };
+  v += mv(v, 0);
+  v += mv(v, 1);
+  v += mv(v, 2);
+  v += mv(v, 3);
+  v += mv(v, 4);
+  v += mv(v, 5);
   return v;
 }

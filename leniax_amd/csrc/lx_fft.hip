@@ -13,7 +13,8 @@
 //
 // Transform lengths: 2^k in [16, 512] and 3 * 2^k in [48, 384].  A length-L transform is
 // spread over a group of TPR lanes of one wave (P = L / TPR points per lane): an in-register
-// FFT of P points, a twiddle multiply and a radix-2 network across the group through DPP.
+// FFT of P points, a twiddle multiply and the cross-lane step (a transpose through LDS when
+// P is a multiple of TPR, else a radix-2 network across the group through DPP).
 // Workspace of a step: S [B][C][half] and U [B][K][half] complex64, half = prod(dims[:-1]) (W/2+1).
 #include <algorithm>
 #include <type_traits>
@@ -61,16 +62,24 @@ int dispatch_len(int L, F&& f) {
   }
 }
 
-// twiddle table exp(-2 pi i j / L) in global memory, one per length and device
+// twiddle tables in global memory, one per length and device: exp(-2 pi i j / L) for the DPP
+// network, and the lane rows of fft_xlds (row t, entry k = exp(-2 pi i t k / L))
 template <int L>
 __device__ float2 g_tw[L];
+template <int L>
+__device__ __align__(16) float2 g_twl[Lanes<L>::TPR * twl_row(Lanes<L>::P)];
 
 template <int L>
 __global__ void init_tw_kernel() {
+  using LC = Lanes<L>;
   for (int j = threadIdx.x; j < L; j += blockDim.x) {
     double sn, cs;
     sincospi(2.0 * j / L, &sn, &cs);
     g_tw<L>[j] = make_float2((float)cs, (float)-sn);
+  }
+  for (int j = threadIdx.x; j < LC::TPR * twl_row(LC::P); j += blockDim.x) {
+    const int t = j / twl_row(LC::P), k = j % twl_row(LC::P);
+    g_twl<L>[j] = k < LC::P ? twl_entry(L, t, k) : make_float2(0.f, 0.f);
   }
 }
 
@@ -116,7 +125,7 @@ LX_DEV int freq_of(int t, int e) {
 template <int L, bool INV>
 LX_DEV void fft_group(float2* x, int t, int lane, float2* xch) {
   using LC = Lanes<L>;
-  if constexpr (LC::XL) fft_xlds<LC::P, LC::TPR, INV>(x, t, g_tw<L>, xch);
+  if constexpr (LC::XL) fft_xlds<LC::P, LC::TPR, INV>(x, t, g_twl<L> + t * twl_row(LC::P), xch);
   else fft_lanes<LC::P, LC::TPR, INV>(x, t, lane, g_tw<L>);
 }
 

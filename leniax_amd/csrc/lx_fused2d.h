@@ -107,22 +107,29 @@ LX_DEV int group_lane(int lane, int t) {
   else return (lane & ~11) | (t & 3) | ((t & 4) << 1);
 }
 
-// Weighted field of the one kernel, w g_k(U) / w (leniax/core.py:163-242), per growth kind.
+// Per-world constants of the field of the one kernel, G = w g(U) / w (leniax/core.py:163-242).
+// The inverse transforms leave xr = U / sc in registers (sc = 1 / (2 N^2), a power of two, so
+// U = xr sc exactly and fma(xr, sc, -m) is U - m rounded once): the Gaussian's
+// exp(-((U - m) / s)^2 / 2) becomes one exp2 of (U - m) gk, gk = sqrt(log2(e) / 2) / s, and
+// 2 g - 1 one FMA c2 e - c1.
+struct FieldK {
+  float sc, m, s, gk, inv_9s2, wk, inv_wden, c1, c2;
+};
+
 template <int GF>
-LX_DEV float field_of(float U, float m, float s, float gk, float inv_9s2, float wk, float inv_wden, float c2,
-                      float c1) {
+LX_DEV float field_of(float xr, const FieldK& f) {
   if constexpr (GF == LX_GF_GAUSSIAN || GF == LX_GF_GAUSSIAN_TARGET) {
-    const float z = (U - m) * gk;
+    const float z = fmaf(xr, f.sc, -f.m) * f.gk;
     const float e = __builtin_amdgcn_exp2f(-(z * z));
-    return GF == LX_GF_GAUSSIAN ? fmaf(c2, e, -c1) : c1 * e;
+    return GF == LX_GF_GAUSSIAN ? fmaf(f.c2, e, -f.c1) : f.c1 * e;
   } else if constexpr (GF == LX_GF_POLY_QUAD4) {
-    const float d = U - m;
-    float o = fmaxf(0.f, 1.f - (d * d) * inv_9s2);
+    const float d = fmaf(xr, f.sc, -f.m);  // U - m, one rounding as in the reference
+    float o = fmaxf(0.f, 1.f - (d * d) * f.inv_9s2);
     o = o * o;
     o = o * o;
-    return fmaf(c2, o, -c1);
+    return fmaf(f.c2, o, -f.c1);
   } else {
-    return mul_nc(wk, growth_k<GF>(m, s, U)) * inv_wden;
+    return mul_nc(f.wk, growth_k<GF>(f.m, f.s, xr * f.sc)) * f.inv_wden;
   }
 }
 
@@ -134,7 +141,7 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
   constexpr bool SPREAD = (TPR == 8);
   __shared__ float2 buf[G::BUF];
   __shared__ __align__(16) float2 xch_all[(NT / TPR) * G::XG];
-  __shared__ float2 tw[N];
+  __shared__ __align__(16) float2 twl[TPR * twl_row(P)];  // lane twiddle rows (fft_xlds)
   __shared__ float red[NW][G::NRED];
   __shared__ int ctrl[4];
   __shared__ lx_carry carry;  // statistics + heuristics carry of the current world (thread 0)
@@ -147,14 +154,14 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
   const int lane = tid & 63, wave = tid >> 6;
   constexpr int J = G::J;
   float2* const xch = xch_all + r0 * G::XG;
-  const float inv_n2 = 1.f / float(N * N);
   const int64_t cells_per_world = int64_t(N) * N;
 
-  for (int j = tid; j < N; j += NT) {
-    double sn, cs;
-    sincospi(2.0 * j / N, &sn, &cs);
-    tw[j] = make_float2((float)cs, (float)-sn);
+  for (int j = tid; j < TPR * twl_row(P); j += NT) {
+    const int tt = j / twl_row(P), k = j % twl_row(P);
+    twl[j] = k < P ? twl_entry(N, tt, k) : make_float2(0.f, 0.f);
   }
+  // the last wave finishes each step's statistics (wave 0 carries the extra work of column 0)
+  constexpr int FIN = NT - 64;
 
   float a0[P], a1[P];
   float2 x[P];
@@ -177,17 +184,19 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
       a1[m] = A0[(2 * r0 + 1) * N + TPR * m + t0];
     }
     const float2* kh = p.kh + world * p.k_bstride;
-    const float gm = p.gf[world * p.gf_bstride + 0];
-    const float gs = p.gf[world * p.gf_bstride + 1];
-    const float wk = p.w[world * p.w_bstride];
-    const float inv_wden = 1.f / (p.average ? wk : 1.f);
-    const float inv_s = 1.f / gs, inv_9s2 = 1.f / (9.f * (gs * gs));
-    // field of the single kernel: G = w g(U) / w; the growths 2 e - 1 and 2 o^4 - 1 become one
-    // FMA c2 e - c1, and the Gaussian's exp(-z^2 / 2) one exp2 of the pre-scaled z
-    const float c1 = wk * inv_wden, c2 = 2.f * c1;
-    const float gk = inv_s * 0.84932180028801904272f;  // sqrt(log2(e) / 2)
+    FieldK fk;
+    fk.sc = 0.5f / float(N * N);
+    fk.m = p.gf[world * p.gf_bstride + 0];
+    fk.s = p.gf[world * p.gf_bstride + 1];
+    fk.wk = p.w[world * p.w_bstride];
+    fk.inv_wden = 1.f / (p.average ? fk.wk : 1.f);
+    fk.inv_9s2 = 1.f / (9.f * (fk.s * fk.s));
+    fk.c1 = fk.wk * fk.inv_wden;
+    fk.c2 = 2.f * fk.c1;
+    fk.gk = (1.f / fk.s) * 0.84932180028801904272f;  // sqrt(log2(e) / 2) / s
+    const float xr_eps = LX_EPS / fk.sc;  // U > eps  <=>  xr > eps / sc (exact scaling)
     const float dtw = p.dt[world * p.dt_bstride];
-    if (tid == 0) {
+    if (tid == FIN) {
       if (p.resume) {
         carry = p.park[world];
       } else {
@@ -215,7 +224,7 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
       // ---- rows forward: packed row r = A[2r] + i A[2r+1]
 #pragma unroll
       for (int m = 0; m < P; ++m) x[m] = make_float2(a0[m], a1[m]);
-      fft_xlds<P, TPR, false>(x, fresh(q), tw, xch);  // x[j TPR + k2] = Z_r[(J q + j) + P k2]
+      fft_xlds<P, TPR, false>(x, q, twl + fresh(q) * twl_row(P), xch);  // x[j TPR + k2] = Z_r[(J q + j) + P k2]
 #pragma unroll
       for (int j = 0; j < J; ++j)
 #pragma unroll
@@ -226,14 +235,14 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
       if (ctrl[1]) break;
       const int sh0 = ctrl[2], sh1 = ctrl[3];
 
-      // ---- columns forward: c = r, rows h = TPR m + q; unpack the packed rows:
-      //      row 2h' = (za + conj zb)/2, row 2h'+1 = (za - conj zb)/2i; column 0 packs
-      //      the real columns 0 and N/2: (Re/Im za, Re/Im zb)
+      // ---- columns forward: c = r, rows h = TPR m + q; unpack the packed rows, doubled:
+      //      row 2h' = za + conj zb, row 2h'+1 = (za - conj zb)/i (the factor 2 stays with the
+      //      data to the end: sc = 1 / (2 N^2)); column 0 packs the real columns 0 and N/2:
+      //      (Re/Im za, Re/Im zb), undoubled (its spectrum multiply doubles it)
       {
         const int c = r;
         const bool partb = (q & 1) != 0;
         const float sig = partb ? -1.f : 1.f;
-        const float2 cm = partb ? make_float2(0.f, -0.5f) : make_float2(0.5f, 0.f);
         const int ia = G::zidx(q >> 1, c);
         const int ib = G::zidx(q >> 1, c == 0 ? HALF : N - c);
         if (wave != 0) {
@@ -241,7 +250,8 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
           for (int m = 0; m < P; ++m) {
             const float2 za = buf[ia + (TPR / 2) * m * G::ZRS];
             const float2 zb = buf[ib + (TPR / 2) * m * G::ZRS];
-            x[m] = cmul(make_float2(fmaf(sig, zb.x, za.x), fmaf(-sig, zb.y, za.y)), cm);
+            const float u = fmaf(sig, zb.x, za.x), v = fmaf(-sig, zb.y, za.y);
+            x[m] = make_float2(partb ? v : u, partb ? -u : v);
             if ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
           }
         } else {
@@ -250,16 +260,18 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
           for (int m = 0; m < P; ++m) {
             const float2 za = buf[ia + (TPR / 2) * m * G::ZRS];
             const float2 zb = buf[ib + (TPR / 2) * m * G::ZRS];
-            const float2 v = cmul(make_float2(fmaf(sig, zb.x, za.x), fmaf(-sig, zb.y, za.y)), cm);
-            x[m] = c0 ? make_float2(partb ? za.y : za.x, partb ? zb.y : zb.x) : v;
+            const float u = fmaf(sig, zb.x, za.x), v = fmaf(-sig, zb.y, za.y);
+            const float2 d = make_float2(partb ? v : u, partb ? -u : v);
+            x[m] = c0 ? make_float2(partb ? za.y : za.x, partb ? zb.y : zb.x) : d;
             if ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
           }
         }
       }
-      fft_xlds<P, TPR, false>(x, fresh(q), tw, xch);  // x[j TPR + k2] = F[kh = (J q + j) + P k2][c]
+      fft_xlds<P, TPR, false>(x, q, twl + fresh(q) * twl_row(P), xch);  // x[j TPR + k2] = F[kh = (J q + j) + P k2][c]
 
       // ---- multiply by the half spectrum; column 0 carries columns 0 and N/2:
-      //      Q[f] = X[f] a[f] + conj(X[-f]) b[f], a = (K0 + KN)/2, b = (K0 - KN)/2.
+      //      Q[f] = X[f] a[f] + conj(X[-f]) b[f], a = K0 + KN, b = K0 - KN (doubled, as the
+      //      other columns).
       //      X[-f] lives in another lane of the group: column 0 goes through its scratch.
       {
         const float2* kcol = kh + r;
@@ -277,7 +289,7 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
               const int f = J * q + j + P * k2;
               const float2 pm = xch[(N - f) & (N - 1)];
               const float2 kv = kh[f * KS], kn = kh[f * KS + HALF];
-              const float2 a = cscale(cadd(kv, kn), 0.5f), b = cscale(csub(kv, kn), 0.5f);
+              const float2 a = cadd(kv, kn), b = csub(kv, kn);
               x[j * TPR + k2] = cadd(cmul(x[j * TPR + k2], a), cmul(cconj(pm), b));
             }
           }
@@ -293,7 +305,7 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
       }
 
       // ---- columns inverse -> Y
-      fft_xlds<P, TPR, true>(x, fresh(q), tw, xch);  // x[m] = column c at row h = TPR m + q (x N)
+      fft_xlds<P, TPR, true>(x, q, twl + fresh(q) * twl_row(P), xch);  // x[m] = column c at row h = TPR m + q (x N)
       __syncthreads();  // every column gather of Z is done before Y overwrites it
 #pragma unroll
       for (int m = 0; m < P; ++m) buf[(TPR * m + q) * G::YRS + G::ycol(r)] = x[m];
@@ -326,7 +338,7 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
           if ((j & 1) == 1) __builtin_amdgcn_sched_barrier(0);
         }
       }
-      fft_xlds<P, TPR, true>(x, fresh(q), tw, xch);  // x[m] = (U[2r][TPR m+q], U[2r+1][TPR m+q]) * N^2
+      fft_xlds<P, TPR, true>(x, q, twl + fresh(q) * twl_row(P), xch);  // x[m] = (U[2r][TPR m+q], U[2r+1][TPR m+q]) / sc
 
       // ---- growth, field, update, statistics sums.  A lane owns rows 2r, 2r+1 at columns
       //      TPR m + q: row moments factor into per-row sums, column moments into per-column
@@ -343,25 +355,24 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
 #pragma unroll
         for (int part = 0; part < 2; ++part) {
           const int row = 2 * r + part;
-          const float U = (part ? x[m].y : x[m].x) * inv_n2;
+          const float xr = part ? x[m].y : x[m].x;
           const float a = part ? a1[m] : a0[m];
-          const float Gf = field_of<GF>(U, gm, gs, gk, inv_9s2, wk, inv_wden, c2, c1);
-          const float gp = maxnan(Gf, 0.f);
+          const float Gf = field_of<GF>(xr, fk);
+          const float gp = fmaxf(Gf, 0.f);
           if (part) { sa1 += a; sg1 += gp; } else { sa0 += a; sg0 += gp; }
           am += a;
           gpm += gp;
           nA += (a > LX_EPS) ? 1u : 0u;
-          nG += (gp > LX_EPS) ? 1u : 0u;
-          nU += (U > LX_EPS) ? 1u : 0u;
+          nG += (Gf > LX_EPS) ? 1u : 0u;
+          nU += (xr > xr_eps) ? 1u : 0u;
           if constexpr (HIST) {
             if (p.cells_hist) p.cells_hist[hist_off + row * N + col] = a;
             if (p.field_hist) p.field_hist[hist_off + row * N + col] = Gf;
-            if (p.pot_hist) p.pot_hist[hist_off + row * N + col] = U;
+            if (p.pot_hist) p.pot_hist[hist_off + row * N + col] = xr * fk.sc;
           }
           float an;
           if constexpr (V1) {  // leniax/core.py:245-272: clip(A + dt G, 0, 1)
-            const float v = add_nc(a, mul_nc(dtw, Gf));
-            an = v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
+            an = fminf(fmaxf(add_nc(a, mul_nc(dtw, Gf)), 0.f), 1.f);
           } else {
             an = state_update(p.state_fn, a, dtw, Gf);
           }
@@ -382,21 +393,21 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
       constexpr int slot[8] = {0, 1, 5, 6, 7, 8, 9, 10};
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        const float v = wave_sum(vals[i]);
-        if (lane == 0) red[wave][slot[i]] = v;
+        const float v = wave_sum_last(vals[i]);
+        if (lane == 63) red[wave][slot[i]] = v;
       }
       {
-        const uint32_t cnt[3] = {nA, nG, nU};
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          uint32_t v = cnt[i];
-#pragma unroll
-          for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-          if (lane == 0) red[wave][2 + i] = float(v);
+        // per-wave counts are at most 2048: nA and nG share one 32-bit sum
+        const uint32_t ag = wave_sum_last(nA | (nG << 16));
+        const uint32_t u = wave_sum_last(nU);
+        if (lane == 63) {
+          red[wave][2] = float(ag & 0xFFFFu);
+          red[wave][3] = float(ag >> 16);
+          red[wave][4] = float(u);
         }
       }
       __syncthreads();
-      if (tid == 0) {
+      if (tid == FIN) {
         lx_sums sm;
         float tot[11];
 #pragma unroll
@@ -431,7 +442,7 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
       A1[(2 * r0) * N + TPR * m + t0] = a0[m];
       A1[(2 * r0 + 1) * N + TPR * m + t0] = a1[m];
     }
-    if (tid == 0) {
+    if (tid == FIN) {
       if (finished) {
         if (p.n_out) p.n_out[world] = carry.n;
         if (p.steps_out) p.steps_out[world] = t;
