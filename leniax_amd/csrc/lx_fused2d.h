@@ -31,7 +31,9 @@ struct F2 {
   static constexpr int HALF = N / 2;
   static constexpr int NT = HALF * TPR;          // one lane group per packed row / column
   static constexpr int NW = (NT + 63) / 64;      // waves
-  static constexpr int ZRS = TPR * (P + 1);      // packed-row spectrum stride (blocks of P padded by one)
+  // packed-row spectrum stride: blocks of P padded by one, plus one so that the Z stores and
+  // gathers of 32 lanes (4 rows x 8 lanes) spread over the 64 LDS banks
+  static constexpr int ZRS = TPR * (P + 1) + 1;
   static constexpr int YRS = HALF + 2;           // column-spectrum row stride
   static constexpr int ZSZ = HALF * ZRS;
   static constexpr int YSZ = N * YRS;
@@ -93,20 +95,6 @@ LX_DEV float state_update(int fn, float a, float dt, float G) {
   return v < 0.f ? 0.f : (v > 1.f ? 1.f : v);
 }
 
-// value held by group member t ^ (TPR - 1)
-template <int TPR>
-LX_DEV float2 group_mirror(float2 v) {
-  if constexpr (TPR == 4) return dpp2<LX_QP(3, 2, 1, 0)>(v);
-  else return dpp2<0x128>(dpp2<LX_QP(3, 2, 1, 0)>(v));  // spread group: lane ^ 3, then lane ^ 8
-}
-
-// lane (within the wave) of group member t, given any lane of the group
-template <int TPR>
-LX_DEV int group_lane(int lane, int t) {
-  if constexpr (TPR == 4) return (lane & ~3) | t;
-  else return (lane & ~11) | (t & 3) | ((t & 4) << 1);
-}
-
 // Per-world constants of the field of the one kernel, G = w g(U) / w (leniax/core.py:163-242).
 // The inverse transforms leave xr = U / sc in registers (sc = 1 / (2 N^2), a power of two, so
 // U = xr sc exactly and fma(xr, sc, -m) is U - m rounded once): the Gaussian's
@@ -138,7 +126,6 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
   using G = F2<N>;
   constexpr int P = G::P, TPR = G::TPR, NT = G::NT, NW = G::NW, HALF = G::HALF;
   constexpr int KS = HALF + 1;  // half-spectrum row length
-  constexpr bool SPREAD = (TPR == 8);
   __shared__ float2 buf[G::BUF];
   __shared__ __align__(16) float2 xch_all[(NT / TPR) * G::XG];
   __shared__ __align__(16) float2 twl[TPR * twl_row(P)];  // lane twiddle rows (fft_xlds)
@@ -147,10 +134,10 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
   __shared__ lx_carry carry;  // statistics + heuristics carry of the current world (thread 0)
 
   const int tid = threadIdx.x;
-  // lane group of a transform: TPR = 4: 4 consecutive lanes; TPR = 8 (SPREAD): lanes
-  // {0..3, 8..11} + 4*(r & 1) + 16*(r >> 1), so that t ^ 4 is lane ^ 8 (one DPP rotate)
-  const int t0 = SPREAD ? ((tid & 3) | (((tid >> 3) & 1) << 2)) : (tid % TPR);
-  const int r0 = SPREAD ? (((tid >> 2) & 1) | ((tid >> 4) << 1)) : (tid / TPR);
+  // lane group of a transform: TPR consecutive lanes (the exchange's 16-byte accesses of 16
+  // lanes then cover the 64 LDS banks once)
+  const int t0 = tid % TPR;
+  const int r0 = tid / TPR;
   const int lane = tid & 63, wave = tid >> 6;
   constexpr int J = G::J;
   float2* const xch = xch_all + r0 * G::XG;
@@ -160,8 +147,11 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
     const int tt = j / twl_row(P), k = j % twl_row(P);
     twl[j] = k < P ? twl_entry(N, tt, k) : make_float2(0.f, 0.f);
   }
-  // the last wave finishes each step's statistics (wave 0 carries the extra work of column 0)
+  // the last two waves close each step (wave 0 carries the extra work of column 0): lane 0 of
+  // the last wave (FIN) keeps the carried shift, thread HT the heuristics
   constexpr int FIN = NT - 64;
+  constexpr int HT = NW >= 2 ? NT - 128 : 1;
+  static_assert(G::NRED >= 11, "eleven world sums per wave");
 
   float a0[P], a1[P];
   float2 x[P];
@@ -407,28 +397,52 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
         }
       }
       __syncthreads();
-      if (tid == FIN) {
-        lx_sums sm;
-        float tot[11];
-#pragma unroll
-        for (int i = 0; i < 11; ++i) {
+      if (wave == NW - 1) {
+        // The step's record: lane i < 11 adds up world sum i (waves in order) and stores it; the
+        // statistics pass after the run turns the records into leniax's statistics.
+        if (p.stats && lane < 11) {
           float acc = 0.f;
 #pragma unroll
-          for (int wv = 0; wv < NW; ++wv) acc += red[wv][i];
-          tot[i] = acc;
+          for (int wv = 0; wv < NW; ++wv) acc += red[wv][lane];
+          p.stats[(world * p.max_steps + t) * (LX_NSTAT_BASE + 1) + lane] = acc;
         }
-        sm.sA = tot[0]; sm.sG = tot[1]; sm.nA = tot[2]; sm.nG = tot[3]; sm.nU = tot[4];
-        sm.sAy[0] = tot[5]; sm.sAy[1] = tot[6]; sm.sAy[2] = 0.f;
-        sm.sGy[0] = tot[7]; sm.sGy[1] = tot[8]; sm.sGy[2] = 0.f;
-        sm.sAyy[0] = tot[9]; sm.sAyy[1] = tot[10]; sm.sAyy[2] = 0.f;
-        sm.sAc[0] = tot[0];
-        float* out = p.stats ? p.stats + (world * p.max_steps + t) * (LX_NSTAT_BASE + 1) : nullptr;
-        const int32_t dims[3] = {N, N, 1};
-        const int stop = lx_stats_finish(sm, carry, 2, dims, 1, p.R2, p.R, p.dt_stats, p.stop_policy, p.min_steps,
-                                         float(N * N), out);
-        ctrl[1] = stop;
-        ctrl[2] = carry.shift[0];
-        ctrl[3] = carry.shift[1];
+        if (lane == 0) {
+          // the carried shift (leniax/statistics.py:117-124) and the mass stop
+          float sA = 0.f, nA = 0.f, sAy[2] = {0.f, 0.f};
+#pragma unroll
+          for (int wv = 0; wv < NW; ++wv) {
+            sA += red[wv][0];
+            nA += red[wv][2];
+            sAy[0] += red[wv][5];
+            sAy[1] += red[wv][6];
+          }
+          const float m00e = sA + LX_EPS;
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            // |sAy| < m00e / 2 truncates the centroid sAy / m00e to 0: no division needed
+            if (!(fabsf(sAy[i]) < 0.5f * m00e)) {
+              const int si = (int)(sAy[i] / m00e);
+              const int v = (carry.shift[i] + si) % N;
+              carry.shift[i] = v < 0 ? v + N : v;
+            }
+          }
+          if (p.stop_policy == LX_STOP_MASS) ctrl[1] = (sA == 0.f || nA > 0.9f * float(N * N)) ? 1 : 0;
+          ctrl[2] = carry.shift[0];
+          ctrl[3] = carry.shift[1];
+        }
+      }
+      if (tid == HT) {
+        // heuristics (leniax/statistics.py:134-205), on another wave: counters, the cumulative
+        // continue flag, N, the step count, and the leniax stop
+        float sA = 0.f, nA = 0.f;
+#pragma unroll
+        for (int wv = 0; wv < NW; ++wv) {
+          sA += red[wv][0];
+          nA += red[wv][2];
+        }
+        const float mass = sA / p.R2;
+        lx_heuristic_step(carry, mass, &mass, nA / p.R2, 1);
+        if (p.stop_policy == LX_STOP_LENIAX) ctrl[1] = (!carry.cont && carry.t >= p.min_steps) ? 1 : 0;
       }
       ++t;  // no barrier: the next step's first one orders thread 0's writes before their readers
     }
@@ -458,6 +472,36 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
 template <int UNUSED>
 __global__ void zero_i32(int32_t* p) {
   if (threadIdx.x < 4) p[threadIdx.x] = 0;
+}
+
+// The statistics of a fused run (leniax/statistics.py:64-126), from the world sums each step
+// left in its record, written over that record.  One wave per world; lane k walks the steps
+// [k L, (k + 1) L) in order.  A lane's starting carry (centroid and angle) follows from the two
+// steps before its range, which it replays first: every lane reads those records before any
+// lane of the wave writes one (the writes depend on the loads' values).
+template <int UNUSED>
+__global__ __launch_bounds__(64) void stats_pass_kernel(float* stats, const int32_t* steps, int64_t B, int max_steps,
+                                                        float R2, float R, float dt) {
+  const int64_t b = blockIdx.x;
+  if (b >= B) return;
+  constexpr int NS = LX_NSTAT_BASE + 1;
+  const int n = steps[b] < max_steps ? steps[b] : max_steps;
+  const int L = (max_steps + 63) / 64;
+  const int t0 = int(threadIdx.x) * L;
+  const int t1 = t0 + L < n ? t0 + L : n;
+  float* rec = stats + b * int64_t(max_steps) * NS;
+  float centroid[3] = {0.f, 0.f, 0.f}, angle = 0.f;
+  if (t0 < n) {
+    for (int t = t0 >= 2 ? t0 - 2 : 0; t < t0; ++t) {
+      const lx_sums s = lx_sums_of_record(rec + int64_t(t) * NS);
+      lx_stats_values(s, centroid, angle, 2, 1, R2, R, dt, nullptr);
+    }
+  }
+  for (int t = t0; t < t1; ++t) {
+    float* r = rec + int64_t(t) * NS;
+    const lx_sums s = lx_sums_of_record(r);
+    lx_stats_values(s, centroid, angle, 2, 1, R2, R, dt, r);
+  }
 }
 
 template <int N, int GF, bool HIST, bool V1>
@@ -490,6 +534,9 @@ int launch_k(const FusedParams& p, int64_t B, hipStream_t stream) {
     p2.limit = p.max_steps;
     hipLaunchKernelGGL((fused2d_kernel<N, GF, HIST, V1>), dim3((unsigned)grid), dim3(F2<N>::NT), 0, stream, p2);
   }
+  if (p.stats && B > 0)
+    hipLaunchKernelGGL(stats_pass_kernel<0>, dim3((unsigned)B), dim3(64), 0, stream, p.stats, p.steps_out, B,
+                       p.max_steps, p.R2, p.R, p.dt_stats);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return lx_fail(LX_EDEVICE, "fused launch failed: %s", hipGetErrorString(e));
   return LX_OK;
@@ -540,6 +587,7 @@ int launch(const lx_rule* rule, const lx_run_args* a, hipStream_t stream) {
   p.surv_count = p.queue + 2;
   p.surv_list = reinterpret_cast<int32_t*>(scr + sl.list);
   p.park = reinterpret_cast<lx_carry*>(scr + sl.park);
+  if (!p.steps_out) p.steps_out = reinterpret_cast<int32_t*>(scr + sl.steps);  // the statistics pass needs them
   p.limit = a->max_steps;
   p.resume = 0;
   p.state_fn = rule->state_fn;

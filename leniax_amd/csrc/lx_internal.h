@@ -56,15 +56,17 @@ struct lx_stat_carry {
 };
 
 // Scratch of lx_run: 4 counters (queue heads of the two launches, survivor count), the
-// survivor list and one parked carry per world.
+// survivor list, one parked carry per world and the steps each world ran (for the statistics
+// pass when the caller does not ask for them).
 struct lx_run_scratch {
-  size_t list, park, total;
+  size_t list, park, steps, total;
 };
 inline lx_run_scratch lx_run_scratch_layout(int64_t B) {
   lx_run_scratch l;
   l.list = 16;
   l.park = (l.list + size_t(B) * 4 + 255) / 256 * 256;
-  l.total = l.park + size_t(B) * sizeof(lx_carry);
+  l.steps = (l.park + size_t(B) * sizeof(lx_carry) + 255) / 256 * 256;
+  l.total = l.steps + size_t(B) * 4;
   return l;
 }
 
@@ -113,11 +115,11 @@ LX_DEV void lx_heuristic_step(lx_carry& c, float mass, const float* cm, float ma
   c.t += 1;
 }
 
-// Statistics of one step from the sums, carry update and stop decision.
-// Follows leniax/statistics.py:64-126 (stats), :134-205 (heuristics, online).
+// Statistics of one step (leniax/statistics.py:64-115) from its sums and the carried centroid
+// and angle, which it then advances (:117-124).  out (11 + C floats) may be NULL.
 // R2 = float(R**2), R, dt = float(1/T) are the config's values.
-LX_DEV int lx_stats_finish(const lx_sums& s, lx_carry& c, int ndim, const int32_t* dims, int C, float R2, float R,
-                           float dt, int stop_policy, int min_steps, float cells_total, float* out) {
+LX_DEV void lx_stats_values(const lx_sums& s, float* centroid, float& angle, int ndim, int C, float R2, float R,
+                            float dt, float* out) {
   const float mass = s.sA / R2;
   const float mass_volume = s.nA / R2;
   const float mass_density = mass / (mass_volume + LX_EPS);
@@ -130,7 +132,7 @@ LX_DEV int lx_stats_finish(const lx_sums& s, lx_carry& c, int ndim, const int32_
   const float m00e = s.sA + LX_EPS, g00e = s.sG + LX_EPS, m002e = s.sA * s.sA + LX_EPS;
   for (int i = 0; i < ndim; ++i) {
     cen[i] = s.sAy[i] / m00e;
-    d[i] = cen[i] - c.centroid[i];
+    d[i] = cen[i] - centroid[i];
     dist2 += d[i] * d[i];
     gc[i] = s.sGy[i] / g00e;
     float e = gc[i] - cen[i];
@@ -141,11 +143,8 @@ LX_DEV int lx_stats_finish(const lx_sums& s, lx_carry& c, int ndim, const int32_
   const float mass_speed = dist / R / dt;
   const float keep = (dist / R > 0.001f) ? 1.f : 0.f;
   const float ang = (atan2f(d[1], d[0]) * 57.29577951308232f) * keep;
-  const float mass_angle_speed = (lx_fmod_floor(ang - c.angle + 540.f, 360.f) - 180.f) / dt;
+  const float mass_angle_speed = (lx_fmod_floor(ang - angle + 540.f, 360.f) - 180.f) / dt;
   const float mass_growth_dist = sqrtf(gd2) / R;
-
-  float cm[LX_MAX_C];
-  for (int ch = 0; ch < C; ++ch) cm[ch] = s.sAc[ch] / R2;
   if (out) {
     out[LX_ST_MASS] = mass;
     out[LX_ST_MASS_VOLUME] = mass_volume;
@@ -158,18 +157,27 @@ LX_DEV int lx_stats_finish(const lx_sums& s, lx_carry& c, int ndim, const int32_
     out[LX_ST_MASS_GROWTH_DIST] = mass_growth_dist;
     out[LX_ST_INERTIA] = inertia;
     out[LX_ST_POTENTIAL_VOLUME] = potential_volume;
-    for (int ch = 0; ch < C; ++ch) out[LX_ST_CHANNEL_MASS0 + ch] = cm[ch];
+    for (int ch = 0; ch < C; ++ch) out[LX_ST_CHANNEL_MASS0 + ch] = s.sAc[ch] / R2;
   }
-  // carry: leniax/statistics.py:117-124 (truncating shift, floor-mod wrap)
+  // carried centroid: the fractional part left after the truncating shift (:117-124)
+  for (int i = 0; i < ndim; ++i) centroid[i] = cen[i] - (float)(int)cen[i];
+  angle = ang;
+}
+
+// What steers a run after one step: the carried shift (leniax/statistics.py:117-124, truncating
+// shift, floor-mod wrap), the heuristics (:134-205, online) and the stop decision.
+LX_DEV int lx_stats_control(const lx_sums& s, lx_carry& c, int ndim, const int32_t* dims, int C, float R2,
+                            int stop_policy, int min_steps, float cells_total) {
+  const float m00e = s.sA + LX_EPS;
   for (int i = 0; i < ndim; ++i) {
-    int si = (int)cen[i];
+    const float cen = s.sAy[i] / m00e;
+    int si = (int)cen;
     int v = (c.shift[i] + si) % dims[i];
     c.shift[i] = v < 0 ? v + dims[i] : v;
-    c.centroid[i] = cen[i] - (float)si;
   }
-  c.angle = ang;
-
-  lx_heuristic_step(c, mass, cm, mass_volume, C);
+  float cm[LX_MAX_C];
+  for (int ch = 0; ch < C; ++ch) cm[ch] = s.sAc[ch] / R2;
+  lx_heuristic_step(c, s.sA / R2, cm, s.nA / R2, C);
   switch (stop_policy) {
     case LX_STOP_LENIAX:
       return (!c.cont && c.t >= min_steps) ? 1 : 0;
@@ -178,6 +186,26 @@ LX_DEV int lx_stats_finish(const lx_sums& s, lx_carry& c, int ndim, const int32_
     default:
       return 0;
   }
+}
+
+// Statistics, carry update and stop decision of one step.
+LX_DEV int lx_stats_finish(const lx_sums& s, lx_carry& c, int ndim, const int32_t* dims, int C, float R2, float R,
+                           float dt, int stop_policy, int min_steps, float cells_total, float* out) {
+  lx_stats_values(s, c.centroid, c.angle, ndim, C, R2, R, dt, out);
+  return lx_stats_control(s, c, ndim, dims, C, R2, stop_policy, min_steps, cells_total);
+}
+
+// Record of one fused 2-D one-channel step before the statistics pass: the world sums
+// sA, sG, nA, nG, nU, sAy[0..1], sGy[0..1], sAyy[0..1] in the step's 12-float statistics slot.
+LX_DEV lx_sums lx_sums_of_record(const float* r) {
+  lx_sums s;
+  s.sA = r[0]; s.sG = r[1]; s.nA = r[2]; s.nG = r[3]; s.nU = r[4];
+  s.sAy[0] = r[5]; s.sAy[1] = r[6]; s.sAy[2] = 0.f;
+  s.sGy[0] = r[7]; s.sGy[1] = r[8]; s.sGy[2] = 0.f;
+  s.sAyy[0] = r[9]; s.sAyy[1] = r[10]; s.sAyy[2] = 0.f;
+  s.sAc[0] = r[0];
+  for (int ch = 1; ch < LX_MAX_C; ++ch) s.sAc[ch] = 0.f;
+  return s;
 }
 
 // Error reporting (thread-local message), lx_api.hip
