@@ -60,22 +60,30 @@ def flops_per_world_step(n_side=N, C=1, K=1):
     return 2.5 * n * math.log2(n) * (C + K) + 6 * K * nh + 8 * K * n + 2 * K * n + 3 * C * n
 
 
-def cpu_baseline(sample_worlds=64, steps=384):
-    """float32 oracle (scipy pocketfft, all host cores for the FFTs) on a bounded sample."""
+def cpu_baseline(sample_worlds=64, steps=384, single_worlds=16, single_steps=192):
+    """float32 oracle (scipy pocketfft) on a bounded sample: all host cores for the FFTs (the
+    reported value), and one thread on a smaller sample."""
     from oracle import lenia as L
     cores = min(len(os.sched_getaffinity(0)), 16)
     K, mp = L.get_kernels_and_mapping(kernel_params(), [N, N], 1, R, np.float32)
-    rule = L.Rule(mp.gf_slugs(), None, True, 'v1', workers=cores)
-    cells = make_worlds(sample_worlds)
     sfn = L.build_compute_stats_fn([N, N], R, T, np.float32)
-    t0 = time.perf_counter()
-    L.run_scan(cells, K, mp.get_gf_params(), mp.get_kernels_weight_per_channel(), T, steps, R, rule, sfn)
-    dt = time.perf_counter() - t0
-    ws = sample_worlds * steps / dt
+
+    def rate(workers, worlds, nsteps):
+        rule = L.Rule(mp.gf_slugs(), None, True, 'v1', workers=workers)
+        cells = make_worlds(worlds)
+        t0 = time.perf_counter()
+        L.run_scan(cells, K, mp.get_gf_params(), mp.get_kernels_weight_per_channel(), T, nsteps, R, rule, sfn)
+        dt = time.perf_counter() - t0
+        return worlds * nsteps / dt, dt
+
+    ws, dt = rate(cores, sample_worlds, steps)
+    ws1, dt1 = rate(1, single_worlds, single_steps)
     return {'value': ws, 'unit': 'world-steps/s', 'cores': cores, 'kind': 'port',
             'sample': f'{sample_worlds} of the 1024 worlds x {steps} steps with per-step statistics, '
                       f'float32 oracle (scipy.fft workers={cores}), {dt:.1f} s',
-            'cell_updates_per_s': ws * N * N}
+            'cell_updates_per_s': ws * N * N,
+            'single_thread': {'value': ws1, 'unit': 'world-steps/s', 'cores': 1,
+                              'sample': f'{single_worlds} worlds x {single_steps} steps, workers=1, {dt1:.1f} s'}}
 
 
 def adjoint_rate(n=256, steps=64, ckpt=16, reps=5):
@@ -417,7 +425,8 @@ def main():
         'fused_vs_plain_single_world': full_ws / plain_single,
         'fused_vs_plain_batched': full_ws / plain_batched,
         'roofline': {'bound': 'mfma', 'compute': 'fp32 VALU (no MFMA: no dense contraction on the path)',
-                     'kernel': 'fused2d_kernel<128> (probe + resume dispatches of one lx_run)',
+                     'kernel': 'fused2d_kernel<128> (probe + resume dispatches) and the statistics pass of '
+                               'one lx_run',
                      'achieved': achieved / 1e12, 'peak': PEAK_FP32 / 1e12,
                      'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32, 'traffic': measured_traffic(),
                      'flops_per_world_step': F, 'launch_ms': launch_ms},
