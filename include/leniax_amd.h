@@ -39,6 +39,27 @@ enum lx_stop { LX_STOP_NONE = 0,   /* run every step (leniax run_scan*), N from 
                LX_STOP_LENIAX = 1, /* retire a world once leniax's heuristics fail (and t >= min_steps) */
                LX_STOP_MASS = 2 }; /* retire when the mass is 0 or covers > 90% of the grid */
 
+/* kernel shapes, leniax/kernels.py:312-317 register (raw kernels are passed as arrays) */
+enum lx_kernel_shape { LX_KSHAPE_CIRCLE = 0, LX_KSHAPE_ELLIPSE = 1, LX_KSHAPE_ORIENTED_ELLIPSE = 2 };
+/* kernel functions, registry order of leniax/kernel_functions.py:253-261 */
+enum lx_kernel_fn { LX_KF_POLY_QUAD = 0, LX_KF_GAUSS_BUMP = 1, LX_KF_GAUSS = 2, LX_KF_STEP = 3,
+                    LX_KF_THRESHOLD = 4, LX_KF_STAIRCASE = 5, LX_KF_TRIANGLE = 6 };
+#define LX_MAX_RINGS 8 /* ring weights b per kernel */
+
+/* One kernel of leniax's kernels_params (leniax/kernels.py:176-309): k_params = [r, b, a, b_axis,
+ * theta] and the kernel function's kf_params.  128 bytes, float64 parameters. */
+typedef struct lx_kernel_desc {
+  int32_t shape;             /* enum lx_kernel_shape */
+  int32_t kf;                /* enum lx_kernel_fn */
+  int32_t nb;                /* rings, 1..LX_MAX_RINGS */
+  int32_t kr;                /* math.ceil(r * R): the kernel grid is 2 kr x 2 kr */
+  double r;                  /* k_params[0] */
+  double a, b;               /* ellipse axes k_params[2], k_params[3] (unused by circles) */
+  double theta;              /* k_params[4]; the rotation angle is theta * pi */
+  double kf_params[2];       /* kf_params (q, or m and s) */
+  double bs[LX_MAX_RINGS];   /* ring weights k_params[1] */
+} lx_kernel_desc;
+
 /* The static part of leniax's update closure (leniax/helpers.py:401-515):
  * world shape, channel mapping, growth function per kernel, weighting, state fn. */
 typedef struct lx_rule {
@@ -72,6 +93,36 @@ int lx_prepare_spectra(const lx_rule* rule, const void* K_full_d, int64_t S, int
  * complex64 elements. */
 int lx_kernel_spectrum(int32_t ndim, const int32_t* dims, int64_t n, const float* k_d, void* K_full_d,
                        void* work_d, void* stream);
+
+/* leniax/kernels.py:66-149 for many kernels at once, on the device in float64: the kernels of
+ * desc_d [n] (device memory) at world radius R, each normalised (sum 1; oriented ellipses: sum of
+ * |k| 1), rounded to float32, zero-padded to the H x W world ((H - 2 kr) // 2 cells before) and
+ * fftshift-ed, into out_d [n][H][W].  max_kr bounds every desc's kr (2 max_kr <= H, W).  With
+ * basis != 0: per desc LX_MAX_RINGS unnormalised rings instead (ring i with weight 1, the others
+ * 0; rings >= nb stay zero), out_d [n][LX_MAX_RINGS][H][W], the basis of the ring-weight gradient. */
+int lx_build_kernels(int32_t ndim, const int32_t* dims, const lx_kernel_desc* desc_d, int64_t n, double R,
+                     int32_t max_kr, int32_t basis, float* out_d, void* stream);
+/* lx_build_kernels then their real FFTs: K_half_d [n][H][W/2+1] complex64, the half spectra of
+ * leniax's K = fftn(fftshift(k)) (the lx_prepare_spectra layout for real kernels);
+ * work_d: n * H * W floats. */
+int lx_build_spectra(int32_t ndim, const int32_t* dims, const lx_kernel_desc* desc_d, int64_t n, double R,
+                     int32_t max_kr, void* K_half_d, float* work_d, void* stream);
+
+/* jax.random's stream (leniax's rng keys, leniax/utils.py:612-632): the Threefry-2x32 cipher
+ * (20 rounds) under key[2] (host memory) of the counts 0..n-1 split in halves, as
+ * jax.random.bits / uniform lay them out: out_d [n] uint32. */
+int lx_random_bits(const uint32_t* key, int64_t n, uint32_t* out_d, void* stream);
+/* leniax/initializations.py:10-33 random_uniform: nb states of `cells` cells, value c of state b
+ * max(0, u maxval_d[b]) with u the stream word b * cells + c as a float in [0, 1), quantised
+ * (leniax/loader.py:16-30).  key[2] is the subkey leniax draws from (host memory). */
+int lx_init_uniform(const uint32_t* key, int64_t nb, int64_t cells, const float* maxval_d, float* out_d,
+                    void* stream);
+/* leniax/initializations.py:36-80 + leniax/perlin.py:17-71: nb perlin noises of H x W over an
+ * r0 x r1 lattice, gradient angles 2 pi u from the stream of key[2] (host memory; the angles land in
+ * angles_d [nb][r0][r1]), each shifted to minimum 0, divided by its maximum, times scaling_d[b],
+ * quantised when quantise != 0: out_d [nb][H][W]. */
+int lx_init_perlin(const uint32_t* key, int32_t nb, int32_t H, int32_t W, int32_t r0, int32_t r1,
+                   const float* scaling_d, int32_t quantise, float* angles_d, float* out_d, void* stream);
 
 /* Workspace bytes lx_potential / lx_update / lx_stats need for a batch of B worlds. */
 int lx_workspace_size(const lx_rule* rule, int64_t B, size_t* bytes);

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, 'libleniax_amd.so')
 LX_MAX_K = 16
 LX_MAX_C = 4
 LX_NSTAT_BASE = 11
+LX_MAX_RINGS = 8
 
 LX_OK = 0
 STATE_FN = {'v1': 0, 'v2': 1, 'simple': 2}
@@ -43,6 +44,27 @@ class lx_rule(ctypes.Structure):
         ('gf', ctypes.c_int32 * LX_MAX_K),
         ('state_fn', ctypes.c_int32),
         ('average', ctypes.c_int32),
+    ]
+
+
+# enum lx_kernel_shape / lx_kernel_fn
+KERNEL_SHAPES = {'circle_2d': 0, 'ellipse_2d': 1, 'oriented_ellipse_2d': 2}
+KERNEL_FNS = {'poly_quad': 0, 'gauss_bump': 1, 'gauss': 2, 'step': 3, 'threshold': 4, 'staircase': 5,
+              'triangle': 6}
+
+
+class lx_kernel_desc(ctypes.Structure):
+    _fields_ = [
+        ('shape', ctypes.c_int32),
+        ('kf', ctypes.c_int32),
+        ('nb', ctypes.c_int32),
+        ('kr', ctypes.c_int32),
+        ('r', ctypes.c_double),
+        ('a', ctypes.c_double),
+        ('b', ctypes.c_double),
+        ('theta', ctypes.c_double),
+        ('kf_params', ctypes.c_double * 2),
+        ('bs', ctypes.c_double * LX_MAX_RINGS),
     ]
 
 
@@ -106,6 +128,12 @@ SIGNATURES = {
     'lx_half_to_full': (ctypes.c_int, [_I32, ctypes.POINTER(_I32), _I64, _P, _P, _F, _P]),
     'lx_adjoint_potential': (ctypes.c_int, [_R, _I64, _P, _I64, _P, _P, _P, _P, _P, _P]),
     'lx_run_scratch_bytes': (ctypes.c_size_t, [_I64]),
+    'lx_random_bits': (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32), _I64, _P, _P]),
+    'lx_init_uniform': (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32), _I64, _I64, _P, _P, _P]),
+    'lx_init_perlin': (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32), _I32, _I32, _I32, _I32, _I32, _P, _I32, _P,
+                                      _P, _P]),
+    'lx_build_kernels': (ctypes.c_int, [_I32, ctypes.POINTER(_I32), _P, _I64, ctypes.c_double, _I32, _I32, _P, _P]),
+    'lx_build_spectra': (ctypes.c_int, [_I32, ctypes.POINTER(_I32), _P, _I64, ctypes.c_double, _I32, _P, _P, _P]),
     'lx_run_supported': (ctypes.c_int, [_R]),
     'lx_run': (ctypes.c_int, [_R, ctypes.POINTER(lx_run_args), _P]),
 }

@@ -8,6 +8,7 @@
 #include "lx_device.h"
 
 #define LX_EPS 1e-7f  // leniax/constant.py:8
+#define LX_NB_CHARS 112  // leniax/constant.py:13: 25 + 25 + 62 characters of the cell codec
 #define LX_MONOTONIC_STOP_STEP 128  // leniax/statistics.py:284
 #define LX_MASS_VOLUME_THRESHOLD 10.f  // leniax/statistics.py:313
 #define LX_MASS_VOLUME_STOP_STEP 128  // leniax/statistics.py:314

@@ -16,6 +16,7 @@ import numpy as np
 import torch
 
 from . import _lib, engine
+from . import kernels as lx_kernels
 from . import statistics as lx_stat
 from .constant import EPSILON, START_CHECK_STOP
 from .core import standard_parts, fused_update, _spectra, _slots, _rule, _pad_gf
@@ -154,14 +155,22 @@ def run_scan_mem_optimized(rng_key, cells0, K, gf_params, kernels_weight_per_cha
     """
     cells0 = engine.as_device(cells0)
     S, N_init = cells0.shape[:2]
-    K = engine.as_kernels(K)
-    Kf = K.reshape((S,) + tuple(K.shape[-(cells0.dim() - 1):]))  # [S, C, maxk, *world]
+    if isinstance(K, lx_kernels.HalfSpectra):  # built on the device (kernels.build_kernel_spectra)
+        Kf = K
+    else:
+        K = engine.as_kernels(K)
+        Kf = K.reshape((S,) + tuple(K.shape[-(cells0.dim() - 1):]))  # [S, C, maxk, *world]
     fr = _fused_rule(update_fn, compute_stats_fn, cells0[0], Kf)
     if fr is None:
         raise NotImplementedError('run_scan_mem_optimized needs the standard update and statistics functions')
     rule, slots = fr
     B = S * N_init
-    Kh = engine.prepare_spectra(rule, Kf, slots)  # [S, K, ...]
+    if isinstance(K, lx_kernels.HalfSpectra):
+        if K.half.shape[0] != S or K.half.shape[1] != rule.K:
+            raise ValueError(f'half spectra {tuple(K.half.shape)} do not match {S} solutions of {rule.K} kernels')
+        Kh = K.half.contiguous()
+    else:
+        Kh = engine.prepare_spectra(rule, Kf, slots)  # [S, K, ...]
     if N_init > 1:
         Kh = Kh.repeat_interleave(N_init, dim=0).contiguous()
     kbs = int(Kh[0].numel())

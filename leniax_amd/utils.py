@@ -119,3 +119,15 @@ def center_world(cells, field, potential, shift_idx, axes):
         outs.append(torch.stack([torch.roll(x[i], tuple(int(-s) for s in shift_idx[i]), dims=tuple(axes))
                                  for i in range(x.shape[0])]))
     return tuple(outs)
+
+
+def seed_everything(seed: int):
+    """leniax/utils.py:612-632: seeds Python's and numpy's generators and returns the jax.random key
+    leniax derives (PRNGKey of the third draw of Python's generator), as a uint32[2] array."""
+    import os
+    import random
+    from .initializations import PRNGKey
+    random.seed(seed)
+    np.random.seed(random.randint(1, 1_000_000))
+    os.environ['PYTHONHASHSEED'] = str(random.randint(1, 1_000_000))
+    return PRNGKey(random.randint(1, 1_000_000))

@@ -165,6 +165,51 @@ def circle_2d(R, k_params, kf_slug, kf_params, f=np.float32):
     return kernel[np.newaxis].astype(f)
 
 
+def _rotated(R, k_params, f):
+    """Grid, centred coordinates and their rotation by k_params[4] * pi (leniax/kernels.py:231-243)."""
+    r = k_params[0]
+    k_radius_px = math.ceil(r * R)
+    coords = np.indices([k_radius_px * 2, k_radius_px * 2]).astype(np.int32)
+    c = (coords - k_radius_px).astype(f) / f(r * R)
+    theta = f(k_params[4] * np.pi)  # the product in Python floats, then the array dtype (as jnp.cos sees it)
+    ct, st = np.cos(theta).astype(f), np.sin(theta).astype(f)
+    rot = np.stack([c[0] * ct + c[1] * st, -c[0] * st + c[1] * ct])
+    a, b = f(k_params[2]), f(k_params[3])
+    distances = np.sqrt(int_pow(rot[0] / a, 2) + int_pow(rot[1] / b, 2))
+    return rot, distances
+
+
+def _ring_core(bs, distances, kf_slug, kf_params, f):
+    nb_b = bs.shape[0]
+    B_dist = f(nb_b) * distances
+    bs_mat = bs[np.minimum(np.floor(B_dist).astype(np.int32), nb_b - 1)]
+    raw_kernel = KF[kf_slug](kf_params, np.fmod(B_dist, f(1)), f)
+    return (distances < 1).astype(f) * raw_kernel * bs_mat
+
+
+def ellipse_2d(R, k_params, kf_slug, kf_params, f=np.float32):
+    """leniax/kernels.py:215-262: the ring rule over elliptic distances, normalised to sum 1, then
+    times the sign of the rotated first coordinate (the coordinate itself inside (-0.01, 0.01))."""
+    bs = np.asarray(k_params[1], dtype=f)
+    rot, distances = _rotated(R, k_params, f)
+    kernel = _ring_core(bs, distances, kf_slug, kf_params, f)
+    kernel = kernel / kernel.sum(dtype=f)
+    grad = rot[0].copy()
+    grad[rot[0] < -0.01] = -1
+    grad[grad > 0.01] = 1
+    return (kernel * grad)[np.newaxis].astype(f)
+
+
+def oriented_ellipse_2d(R, k_params, kf_slug, kf_params, f=np.float32):
+    """leniax/kernels.py:265-309: the ring rule over elliptic distances times the rotated first
+    coordinate, normalised to sum |k| = 1."""
+    bs = np.asarray(k_params[1], dtype=f)
+    rot, distances = _rotated(R, k_params, f)
+    kernel = _ring_core(bs, distances, kf_slug, kf_params, f) * rot[0]
+    kernel = kernel / np.absolute(kernel).sum(dtype=f)
+    return kernel[np.newaxis].astype(f)
+
+
 def raw(R, k_params, kf_slug, kf_params, f=np.float32):
     """leniax/kernels.py:161-173."""
     return np.asarray(k_params, dtype=f)
@@ -190,7 +235,8 @@ def sphere_nd(R, r, bs, kf_slug, kf_params, nb_dims, f=np.float64):
     return kernel[np.newaxis].astype(f)
 
 
-K_REGISTER = {'raw': raw, 'circle_2d': circle_2d}  # leniax/kernels.py:312-317
+K_REGISTER = {'raw': raw, 'circle_2d': circle_2d, 'ellipse_2d': ellipse_2d,
+              'oriented_ellipse_2d': oriented_ellipse_2d}  # leniax/kernels.py:312-317
 
 
 class Mapping:

@@ -300,3 +300,38 @@ def test_config_v1_to_v2():
                                      'gf_id': 1, 'c_in': 0, 'c_out': 2}])[0]
     assert kp['k_params'] == [1, [1.0, 0.25]] and kp['kf_slug'] == 'gauss' and kp['gf_slug'] == 'gaussian'
     assert kp['gf_params'] == [0.2, 0.03] and kp['c_out'] == 2
+
+
+def test_ellipse_kernels_reduce_and_rotate():
+    """leniax/kernels.py:215-309: with round axes and no rotation the ellipse is the circle
+    (pinned by the golden frames) signed by the first coordinate, and the oriented ellipse is the
+    circle times that coordinate, normalised to sum |k| = 1; a rotation by pi/2 (theta = 0.5)
+    transposes the kernel; the ellipse is odd in its first rotated coordinate."""
+    R = 13.
+    circ = L.circle_2d(R, [1., [1.]], 'gauss_bump', [1.], np.float64)[0]
+    kr = 13
+    x0 = ((np.arange(2 * kr) - kr) / R)[:, None] * np.ones((1, 2 * kr))
+    g = np.where(x0 < -0.01, -1., np.where(x0 > 0.01, 1., x0))
+    e = L.ellipse_2d(R, [1., [1.], 1., 1., 0.], 'gauss_bump', [1.], np.float64)[0]
+    np.testing.assert_allclose(e, circ * g, atol=1e-15)
+    o = L.oriented_ellipse_2d(R, [1., [1.], 1., 1., 0.], 'gauss_bump', [1.], np.float64)[0]
+    assert abs(np.abs(o).sum() - 1) < 1e-12
+    mask = circ * x0 != 0
+    ratio = o[mask] / (circ * x0)[mask]
+    assert np.ptp(ratio) < 1e-12 * np.abs(ratio).max()
+    e0 = L.ellipse_2d(R, [1., [1., .5], 1.3, 0.7, 0.], 'poly_quad', [4], np.float64)[0]
+    e2 = L.ellipse_2d(R, [1., [1., .5], 1.3, 0.7, 0.5], 'poly_quad', [4], np.float64)[0]
+    np.testing.assert_allclose(e2, e0.T, atol=1e-15)
+    np.testing.assert_array_equal(e0[1:], -e0[1:][::-1])
+    # outside the ellipse (x/a)^2 + (y/b)^2 >= 1 every value is 0
+    c1 = x0.T
+    assert np.all(e0[(x0 / 1.3) ** 2 + (c1 / 0.7) ** 2 >= 1] == 0)
+
+
+def test_host_ellipse_kernels_match_oracle_float32():
+    from leniax_amd import kernels
+    for slug in ('ellipse_2d', 'oriented_ellipse_2d'):
+        kp = [0.9, [1., .5], 1.2, 0.8, 0.3]
+        a = kernels.register[slug](12., kp, 'gauss_bump', [1.])
+        b = L.K_REGISTER[slug](12., kp, 'gauss_bump', [1.], np.float32)
+        np.testing.assert_array_equal(a, b)
