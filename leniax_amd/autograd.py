@@ -442,9 +442,15 @@ def kernel_spectrum(kernel, nd=2):
     return _Spectrum.apply(kernel, nd)
 
 
-def ring_basis(R, r, nb_rings, kf_slug, kf_params, world_size):
+def ring_basis(R, r, nb_rings, kf_slug, kf_params, world_size, host=False):
     """Per-ring cores of circle_2d (leniax/kernels.py:176-212), padded to the world and
-    fftshift-ed: ``raw = sum_i b_i basis_i`` before the normalisation.  float32 like leniax."""
+    fftshift-ed: ``raw = sum_i b_i basis_i`` before the normalisation.  Built on the GPU in
+    float64 (lx_build_kernels, ring basis mode) for 2-D worlds; ``host=True`` (or a 3-D world)
+    builds it on the host in float32 like leniax."""
+    if not host and len(world_size) == 2:
+        from . import kernels as lx_kernels
+        param = {'k_slug': 'circle_2d', 'k_params': [r, [1.] * nb_rings], 'kf_slug': kf_slug, 'kf_params': kf_params}
+        return lx_kernels.build_kernels([param], list(world_size), R, basis=True)[0, :nb_rings].contiguous()
     f32 = np.float32
     rad = math.ceil(r * R)
     coords = np.indices([rad * 2, rad * 2]).astype(np.int32)

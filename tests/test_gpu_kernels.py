@@ -160,3 +160,12 @@ def test_qd_staging_runs_like_the_host_build():
     assert torch.equal(st_dev['N'], st_host['N'])
     inds = qd.update_individuals(sols, st_dev)
     assert [i.fitness for i in inds] == [float(x) for x in st_dev['N'].max(dim=1).values.cpu()]
+
+
+def test_autograd_ring_basis_device_matches_host():
+    """autograd.ring_basis built on the device (float64) against leniax's float32 host build."""
+    from leniax_amd import autograd as AG
+    d = AG.ring_basis(13., 1., 3, 'gauss_bump', [1.], (64, 64)).cpu().numpy()
+    h = AG.ring_basis(13., 1., 3, 'gauss_bump', [1.], (64, 64), host=True).cpu().numpy()
+    assert d.shape == h.shape == (3, 64, 64)
+    assert np.abs(d - h).max() < 1e-6 * np.abs(h).max()
