@@ -577,6 +577,25 @@ LX_DEV float growth(int kind, float m, float s, float U) {
   }
 }
 
+// g(U) by runtime id with the kernel's reciprocals precomputed (inv_s = 1/s, inv_9s2 = 1/(9 s^2)):
+// the Gaussians and the quartic polynomial without a division per cell (one rounding away from
+// leniax's divisions), the others as growth().
+LX_DEV float growth_rcp(int kind, float m, float s, float inv_s, float inv_9s2, float U) {
+  if (kind == LX_GF_GAUSSIAN || kind == LX_GF_GAUSSIAN_TARGET) {
+    const float z = (U - m) * inv_s;
+    const float e = __expf(-0.5f * (z * z));
+    return kind == LX_GF_GAUSSIAN ? 2.f * e - 1.f : e;
+  }
+  if (kind == LX_GF_POLY_QUAD4) {
+    const float d = U - m;
+    float o = fmaxf(0.f, 1.f - (d * d) * inv_9s2);
+    o = o * o;
+    o = o * o;
+    return 2.f * o - 1.f;
+  }
+  return growth(kind, m, s, U);
+}
+
 // dg/dU, dg/dm, dg/ds for the differentiable growth functions (adjoint).
 LX_DEV void growth_grad(int kind, float m, float s, float U, float& dU, float& dm, float& ds) {
   switch (kind) {

@@ -418,7 +418,8 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
     }
     wave_lds_fence();
     fft_group<W, true>(x, t, lane, xch);
-    const float gm = a.sel ? 0.f : pg[2 * k], gs = a.sel ? 0.f : pg[2 * k + 1];
+    const float gm = a.sel ? 0.f : pg[2 * k], gs = a.sel ? 1.f : pg[2 * k + 1];
+    const float inv_s = 1.f / gs, inv_9s2 = 1.f / (9.f * (gs * gs));
     float* po0 = want_pot ? a.pot + ((b * a.K + k) * a.rows + r2) * W : nullptr;
 #pragma unroll
     for (int m = 0; m < P; ++m) {
@@ -428,8 +429,8 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
         po0[W + TPR * m + t] = U1;
       }
       if (a.update) {
-        G0[m] = add_nc(G0[m], mul_nc(wk, a.sel ? U0 : growth(a.gfk[k], gm, gs, U0)));
-        G1[m] = add_nc(G1[m], mul_nc(wk, a.sel ? U1 : growth(a.gfk[k], gm, gs, U1)));
+        G0[m] = add_nc(G0[m], mul_nc(wk, a.sel ? U0 : growth_rcp(a.gfk[k], gm, gs, inv_s, inv_9s2, U0)));
+        G1[m] = add_nc(G1[m], mul_nc(wk, a.sel ? U1 : growth_rcp(a.gfk[k], gm, gs, inv_s, inv_9s2, U1)));
       }
     }
   }
