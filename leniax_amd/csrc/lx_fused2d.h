@@ -202,8 +202,63 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
     }
     __syncthreads();
 
+    // Closing of step ts from the world sums its waves left in red, run by three threads on two
+    // waves while the others start the next step: lanes 0-10 of the last wave store the step's
+    // record (the statistics pass after the run turns records into leniax's statistics), lane 0
+    // of the last wave (FIN) updates the carried shift and the mass stop, thread HT runs the
+    // heuristics and the leniax stop.
+    auto close_step = [&](int ts) {
+      if (wave == NW - 1) {
+        if (p.stats && lane < 11) {  // world sum `lane`, waves added in order
+          float acc = 0.f;
+#pragma unroll
+          for (int wv = 0; wv < NW; ++wv) acc += red[wv][lane];
+          p.stats[(world * p.max_steps + ts) * (LX_NSTAT_BASE + 1) + lane] = acc;
+        }
+        if (lane == 0) {
+          // the carried shift (leniax/statistics.py:117-124) and the mass stop
+          float sA = 0.f, nA = 0.f, sAy[2] = {0.f, 0.f};
+#pragma unroll
+          for (int wv = 0; wv < NW; ++wv) {
+            sA += red[wv][0];
+            nA += red[wv][2];
+            sAy[0] += red[wv][5];
+            sAy[1] += red[wv][6];
+          }
+          const float m00e = sA + LX_EPS;
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            // |sAy| < m00e / 2 truncates the centroid sAy / m00e to 0: no division needed
+            if (!(fabsf(sAy[i]) < 0.5f * m00e)) {
+              const int si = (int)(sAy[i] / m00e);
+              const int v = (carry.shift[i] + si) % N;
+              carry.shift[i] = v < 0 ? v + N : v;
+            }
+          }
+          if (p.stop_policy == LX_STOP_MASS) ctrl[1] = (sA == 0.f || nA > 0.9f * float(N * N)) ? 1 : 0;
+          ctrl[2] = carry.shift[0];
+          ctrl[3] = carry.shift[1];
+        }
+      }
+      if (tid == HT) {
+        // heuristics (leniax/statistics.py:134-205), on another wave: counters, the cumulative
+        // continue flag, N, the step count, and the leniax stop
+        float sA = 0.f, nA = 0.f;
+#pragma unroll
+        for (int wv = 0; wv < NW; ++wv) {
+          sA += red[wv][0];
+          nA += red[wv][2];
+        }
+        const float mass = sA / p.R2;
+        lx_heuristic_step(carry, mass, &mass, nA / p.R2, 1);
+        if (p.stop_policy == LX_STOP_LENIAX) ctrl[1] = (!carry.cont && carry.t >= p.min_steps) ? 1 : 0;
+      }
+    };
+
     int t = carry.t;
+    const int t_first = t;  // first step of this residency: nothing to close before it
     const int t_end = p.limit < p.max_steps ? p.limit : p.max_steps;
+    bool stopped = false;
     while (t < t_end) {
       // The lane indices are laundered through an empty asm each step so that the
       // per-lane LDS/global addresses of the transposes are recomputed inside the
@@ -220,10 +275,9 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
 #pragma unroll
         for (int k2 = 0; k2 < TPR; ++k2) buf[r * G::ZRS + k2 * (P + 1) + J * q + j] = x[j * TPR + k2];
       __syncthreads();
-      // The previous step's statistics were finished by thread 0 while the other waves ran
-      // this step's row transforms; this barrier publishes its stop decision and shifts.
-      if (ctrl[1]) break;
-      const int sh0 = ctrl[2], sh1 = ctrl[3];
+      // the previous step is closed here, while the other waves run this step's column
+      // transforms; the Y barrier below publishes its stop decision and shifts
+      if (t > t_first) close_step(t - 1);
 
       // ---- columns forward: c = r, rows h = TPR m + q; unpack the packed rows, doubled:
       //      row 2h' = za + conj zb, row 2h'+1 = (za - conj zb)/i (the factor 2 stays with the
@@ -300,6 +354,12 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
 #pragma unroll
       for (int m = 0; m < P; ++m) buf[(TPR * m + q) * G::YRS + G::ycol(r)] = x[m];
       __syncthreads();
+      // the previous step ended the world: this one is not applied (the state is unchanged)
+      if (ctrl[1]) {
+        stopped = true;
+        break;
+      }
+      const int sh0 = ctrl[2], sh1 = ctrl[3];
 
       // ---- rows inverse: repack rows 2r, 2r+1 from their half spectra, k = (J q + j) + P k2:
       //      k < N/2 read directly, k > N/2 through conj(Y[N-k]); k = 0 and N/2 are real
@@ -396,58 +456,12 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
           red[wave][4] = float(u);
         }
       }
-      __syncthreads();
-      if (wave == NW - 1) {
-        // The step's record: lane i < 11 adds up world sum i (waves in order) and stores it; the
-        // statistics pass after the run turns the records into leniax's statistics.
-        if (p.stats && lane < 11) {
-          float acc = 0.f;
-#pragma unroll
-          for (int wv = 0; wv < NW; ++wv) acc += red[wv][lane];
-          p.stats[(world * p.max_steps + t) * (LX_NSTAT_BASE + 1) + lane] = acc;
-        }
-        if (lane == 0) {
-          // the carried shift (leniax/statistics.py:117-124) and the mass stop
-          float sA = 0.f, nA = 0.f, sAy[2] = {0.f, 0.f};
-#pragma unroll
-          for (int wv = 0; wv < NW; ++wv) {
-            sA += red[wv][0];
-            nA += red[wv][2];
-            sAy[0] += red[wv][5];
-            sAy[1] += red[wv][6];
-          }
-          const float m00e = sA + LX_EPS;
-#pragma unroll
-          for (int i = 0; i < 2; ++i) {
-            // |sAy| < m00e / 2 truncates the centroid sAy / m00e to 0: no division needed
-            if (!(fabsf(sAy[i]) < 0.5f * m00e)) {
-              const int si = (int)(sAy[i] / m00e);
-              const int v = (carry.shift[i] + si) % N;
-              carry.shift[i] = v < 0 ? v + N : v;
-            }
-          }
-          if (p.stop_policy == LX_STOP_MASS) ctrl[1] = (sA == 0.f || nA > 0.9f * float(N * N)) ? 1 : 0;
-          ctrl[2] = carry.shift[0];
-          ctrl[3] = carry.shift[1];
-        }
-      }
-      if (tid == HT) {
-        // heuristics (leniax/statistics.py:134-205), on another wave: counters, the cumulative
-        // continue flag, N, the step count, and the leniax stop
-        float sA = 0.f, nA = 0.f;
-#pragma unroll
-        for (int wv = 0; wv < NW; ++wv) {
-          sA += red[wv][0];
-          nA += red[wv][2];
-        }
-        const float mass = sA / p.R2;
-        lx_heuristic_step(carry, mass, &mass, nA / p.R2, 1);
-        if (p.stop_policy == LX_STOP_LENIAX) ctrl[1] = (!carry.cont && carry.t >= p.min_steps) ? 1 : 0;
-      }
-      ++t;  // no barrier: the next step's first one orders thread 0's writes before their readers
+      ++t;  // no barrier: the next step's Z barrier orders the red writes before the closing  // no barrier: the next step's first one orders thread 0's writes before their readers
     }
 
-    // the last step's stop decision (thread 0 wrote it without a barrier after it)
+    // close the last step run (its sums are in red), unless the loop stopped on a closed one
+    __syncthreads();
+    if (!stopped && t > t_first) close_step(t - 1);
     __syncthreads();
     const bool finished = ctrl[1] != 0 || t >= p.max_steps;
     float* A1 = p.cells + world * cells_per_world;
