@@ -638,21 +638,6 @@ LX_DEV void growth_grad(int kind, float m, float s, float U, float& dU, float& d
   }
 }
 
-// growth value and its partial derivatives together (one exponential for the Gaussians)
-LX_DEV float growth_val_grad(int kind, float m, float s, float U, float& dU, float& dm, float& ds) {
-  if (kind == LX_GF_GAUSSIAN || kind == LX_GF_GAUSSIAN_TARGET) {
-    const float z = (U - m) / s;
-    const float e = __expf(-(z * z) / 2.f);
-    const float amp = kind == LX_GF_GAUSSIAN ? 2.f : 1.f;
-    dU = -amp * e * z / s;
-    dm = -dU;
-    ds = amp * e * z * z / s;
-    return kind == LX_GF_GAUSSIAN ? 2.f * e - 1.f : e;
-  }
-  growth_grad(kind, m, s, U, dU, dm, ds);
-  return growth(kind, m, s, U);
-}
-
 // ----------------------------------------------------------------- wave reductions
 LX_DEV float wave_sum(float v) {
 #pragma unroll

@@ -44,7 +44,10 @@ __global__ void field_kernel(RuleDev rd, const float* __restrict__ pot, int64_t 
     const float* pg = gf + b * gf_bs;
     const float* pw = w + b * w_bs;
     float g[LX_MAX_K];
-    for (int k = 0; k < rd.K; ++k) g[k] = growth(rd.gf[k], pg[2 * k], pg[2 * k + 1], pot[(b * rd.K + k) * n + cell]);
+    for (int k = 0; k < rd.K; ++k) {
+      const float s = pg[2 * k + 1];
+      g[k] = growth_rcp(rd.gf[k], pg[2 * k], s, 1.f / s, 1.f / (9.f * (s * s)), pot[(b * rd.K + k) * n + cell]);
+    }
     for (int c = 0; c < rd.C; ++c) {
       float acc = 0.f, den = 0.f;
       for (int k = 0; k < rd.K; ++k) {
@@ -348,6 +351,7 @@ __global__ __launch_bounds__(256) void adj_growth_kernel(AdjArgs a) {
   const float* pg = a.gf + b * a.gf_bs;
   const float dtw = a.dt[b * a.dt_bs];
   const float gm = pg[2 * k], gs = pg[2 * k + 1];
+  const float inv_s = 1.f / gs, inv_9s2 = 1.f / (9.f * (gs * gs));
   const int kind = a.gfk[k];
   float coef[LX_MAX_C], inv_den[LX_MAX_C];  // w_ck / D_c and 1 / D_c (mean), w_ck and 1 (sum)
 #pragma unroll
@@ -368,7 +372,10 @@ __global__ __launch_bounds__(256) void adj_growth_kernel(AdjArgs a) {
   for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < a.cells; p += int64_t(gridDim.x) * blockDim.x) {
     const float u = Uk[p];
     float dgdu, dgdm, dgds;
-    const float g = growth_val_grad(kind, gm, gs, u, dgdu, dgdm, dgds);
+    // the value as the forward passes compute it (growth_rcp), so that g - G cancels exactly
+    // where the field is one kernel's growth
+    const float g = growth_rcp(kind, gm, gs, inv_s, inv_9s2, u);
+    growth_grad(kind, gm, gs, u, dgdu, dgdm, dgds);
     float dg = 0.f;
 #pragma unroll
     for (int c = 0; c < LX_MAX_C; ++c) {
@@ -399,6 +406,7 @@ __global__ __launch_bounds__(256) void adj_11_kernel(AdjArgs a) {
   const int64_t b = blockIdx.y;
   const float w = a.w[b * a.w_bs];
   const float gm = a.gf[b * a.gf_bs], gs = a.gf[b * a.gf_bs + 1];
+  const float inv_s = 1.f / gs, inv_9s2 = 1.f / (9.f * (gs * gs));
   const float dtw = a.dt[b * a.dt_bs];
   const int kind = a.gfk[0];
   const float coef = a.average ? w / w : w;      // dG -> dg (weighted mean of one kernel: w / w)
@@ -411,7 +419,8 @@ __global__ __launch_bounds__(256) void adj_11_kernel(AdjArgs a) {
     float dG = dtw * dz;
     if (a.dfield) dG += a.dfield[base + p];
     float dgdu, dgdm, dgds;
-    const float g = growth_val_grad(kind, gm, gs, U, dgdu, dgdm, dgds);
+    const float g = growth_rcp(kind, gm, gs, inv_s, inv_9s2, U);  // as the forward passes
+    growth_grad(kind, gm, gs, U, dgdu, dgdm, dgds);
     const float dg = dG * coef;
     a.dU[base + p] = a.dpot_in ? fmaf(dg, dgdu, a.dpot_in[base + p]) : dg * dgdu;
     acc[0] += dg * dgdm;
