@@ -45,6 +45,57 @@ def ch2val(c: str) -> int:
     return idx(c[0]) * NB_CHARS + idx(c[1])
 
 
+def val2ch(v: int) -> str:
+    """Integer -> two characters, the inverse of ch2val (leniax/loader.py:177-203): digits in
+    base NB_CHARS drawn from A-Z, then a-z, then the Latin-1 letters from À."""
+    def char(i):
+        if i <= ord('Z') - ord('A'):
+            return chr(ord('A') + i)
+        if i - (ord('Z') - ord('A')) <= ord('z') - ord('a'):
+            return chr(ord('a') + i - (ord('Z') - ord('A')))
+        return chr(ord('À') + i - (ord('Z') - ord('A')) - (ord('z') - ord('a')))
+    return char(v // NB_CHARS) + char(v % NB_CHARS)
+
+
+def val2char_deprecated(v: int) -> str:
+    """leniax/loader.py:298-304: the run-length format's value characters."""
+    if v == 0:
+        return '.'
+    if v < 25:
+        return chr(ord('A') + v - 1)
+    return chr(ord('p') + (v - 25) // 24) + chr(ord('A') + (v - 25) % 24)
+
+
+def char2val_deprecated(ch: str) -> int:
+    """leniax/loader.py:287-295."""
+    return _rle_value(ch)
+
+
+def deprecated_compress_array(cells) -> str:
+    """Run-length encoding of a state (leniax/loader.py:307-320): values rounded to 0..255, runs
+    of equal values as count + character, rows and higher dimensions joined by the delimiters
+    of DIM_DELIM, '!' at the end."""
+    a = np.rint(np.asarray(cells, np.float64) * 255).astype(int)
+    nb_dims = a.ndim
+
+    def row(vals):
+        out, i = [], 0
+        while i < len(vals):
+            j = i
+            while j < len(vals) and vals[j] == vals[i]:
+                j += 1
+            out.append((str(j - i) if j - i > 1 else '') + val2char_deprecated(int(vals[i])).strip())
+            i = j
+        return out
+
+    def join(d, sub):
+        if d < nb_dims - 1:
+            return DIM_DELIM[nb_dims - 1 - d].join(join(d + 1, e) for e in sub)
+        return DIM_DELIM[nb_dims - 1 - d].join(row(list(sub)))
+
+    return join(0, a) + '!'
+
+
 def _rle_value(tok):
     # leniax/loader.py:287-295
     if tok in ('.', 'b'):

@@ -96,3 +96,35 @@ def test_config_migration_matches_oracle():
     utils.set_param(d, 'a.0.b', 2.)
     utils.set_param(d, 'c.1', 3.)
     assert utils.get_param(d, 'a.0.b') == 2. and d['c'] == [None, 3.]
+
+
+def test_two_character_codec_round_trip():
+    """tests/test_loader.py:15-29: val2ch / ch2val and the run-length characters are inverses."""
+    from leniax_amd import loader
+    vals = list(range(256))
+    assert [loader.ch2val(loader.val2ch(v)) for v in vals] == vals
+    assert [loader.char2val_deprecated(loader.val2char_deprecated(v)) for v in vals] == vals
+    top = loader.NB_CHARS ** 2 - 1
+    assert loader.ch2val(loader.val2ch(top)) == top
+
+
+def test_compress_round_trips():
+    """tests/test_loader.py:32-75: gzip codec round trips (twice), compressible arrays exact,
+    and the run-length encoder against its decoder."""
+    from leniax_amd import loader
+    cells = np.arange(0, 1e5, 100) / 1e5
+    out = loader.decompress_array(loader.compress_array(cells), 1)
+    np.testing.assert_array_almost_equal(cells, out, decimal=4)
+    out2 = loader.decompress_array(loader.compress_array(out), 1)
+    np.testing.assert_array_almost_equal(cells, out2, decimal=4)
+    comp = loader.make_array_compressible(cells)
+    np.testing.assert_array_almost_equal(comp, loader.decompress_array(loader.compress_array(comp), 1), decimal=9)
+    m = loader.NB_CHARS ** 2 - 1
+    grid = (np.arange(0, loader.NB_CHARS ** 2, 8, dtype=np.int32) / m).reshape(32, 49)
+    np.testing.assert_array_almost_equal(grid, loader.decompress_array(loader.compress_array(grid)), decimal=4)
+    rng = np.random.default_rng(1)
+    a = (rng.integers(0, 256, (2, 4, 9)) * (rng.random((2, 4, 9)) < 0.5)) / 255.
+    code = loader.deprecated_compress_array(a)
+    assert code.endswith('!')
+    np.testing.assert_allclose(loader.deprecated_decompress_array(code, 3), a, atol=1e-7)
+    np.testing.assert_allclose(codec.rle_decode(code, 3), a, atol=1e-7)
