@@ -5,6 +5,9 @@
  * may be NULL for the default stream).  Return value: LX_OK (0) or a negative
  * errno-style code; lx_last_error() gives a thread-local message.  Array
  * layouts are leniax's channel-first ones: state [B][C][dims...] float32.
+ * An empty batch (B == 0, n == 0) is valid: the call checks its sizes and
+ * descriptors, launches nothing and returns LX_OK; its device pointers may
+ * then be NULL.
  *
  * Each entry point names the reference interface it replaces (paths relative
  * to the leniax repository).

@@ -16,6 +16,10 @@ LX_NSTAT_BASE = 11
 LX_MAX_RINGS = 8
 
 LX_OK = 0
+LX_EINVAL = -22
+LX_ENOTSUP = -95
+LX_EDEVICE = -5
+LX_ENOMEM = -12
 STATE_FN = {'v1': 0, 'v2': 1, 'simple': 2}
 STOP = {'none': 0, 'leniax': 1, 'mass': 2}
 

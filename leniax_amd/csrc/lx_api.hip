@@ -102,15 +102,15 @@ extern "C" int lx_run(const lx_rule* rule, const lx_run_args* a, void* stream) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
   if (!a) return lx_fail(LX_EINVAL, "args is NULL");
-  if (!a->cells0_d || !a->cells_d || !a->K_half_d || !a->gf_d || !a->w_d || !a->dt_d || !a->scratch_d)
-    return lx_fail(LX_EINVAL, "NULL device pointer in lx_run_args");
   if (a->B < 0 || a->max_steps < 0) return lx_fail(LX_EINVAL, "B and max_steps must be >= 0");
   if (a->stop_policy < LX_STOP_NONE || a->stop_policy > LX_STOP_MASS)
     return lx_fail(LX_EINVAL, "unknown stop policy %d", a->stop_policy);
   if (!(a->R > 0.f) || !(a->T > 0.f)) return lx_fail(LX_EINVAL, "R and T must be positive");
-  if (a->B == 0) return LX_OK;
   if (!lx_fused2d_supported(rule))
     return lx_fail(LX_ENOTSUP, "lx_run: no fused kernel for ndim=%d dims=%dx%d C=%d K=%d (use lx_update)",
                    rule->ndim, rule->dims[0], rule->dims[1], rule->C, rule->K);
+  if (a->B == 0) return LX_OK;  // no world: pointers may be NULL
+  if (!a->cells0_d || !a->cells_d || !a->K_half_d || !a->gf_d || !a->w_d || !a->dt_d || !a->scratch_d)
+    return lx_fail(LX_EINVAL, "NULL device pointer in lx_run_args");
   return lx_fused2d_launch(rule, a, (hipStream_t)stream);
 }

@@ -163,11 +163,11 @@ extern "C" int lx_potential_direct_adjoint(const lx_rule* rule, const float* sta
                                            float* dstate_d, float* dkern_d, void* stream) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
-  if (!kern_d || !dpot_d || (!dstate_d && !dkern_d) || (dkern_d && !state_d)) return lx_fail(LX_EINVAL, "NULL pointer");
   if (rule->ndim != 2) return lx_fail(LX_ENOTSUP, "lx_potential_direct_adjoint: 2-D worlds only");
   if (kh < 1 || kw < 1 || kh > MAXK || kw > MAXK) return lx_fail(LX_ENOTSUP, "kernel %dx%d (1..%d)", kh, kw, MAXK);
   if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
   if (B == 0) return LX_OK;
+  if (!kern_d || !dpot_d || (!dstate_d && !dkern_d) || (dkern_d && !state_d)) return lx_fail(LX_EINVAL, "NULL pointer");
   if (B * rule->C > 65535) return lx_fail(LX_ENOTSUP, "lx_potential_direct_adjoint: B * C > 65535");
   const DirectArgs a = direct_args(rule, state_d, kern_d, kh, kw);
   hipStream_t st = (hipStream_t)stream;
@@ -188,11 +188,11 @@ extern "C" int lx_potential_direct(const lx_rule* rule, const float* state_d, in
                                    int32_t kh, int32_t kw, float* potential_d, void* stream) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
-  if (!state_d || !kern_d || !potential_d) return lx_fail(LX_EINVAL, "NULL pointer");
   if (rule->ndim != 2) return lx_fail(LX_ENOTSUP, "lx_potential_direct: 2-D worlds only");
   if (kh < 1 || kw < 1 || kh > MAXK || kw > MAXK) return lx_fail(LX_ENOTSUP, "kernel %dx%d (1..%d)", kh, kw, MAXK);
   if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
   if (B == 0) return LX_OK;
+  if (!state_d || !kern_d || !potential_d) return lx_fail(LX_EINVAL, "NULL pointer");
   if (B * rule->K > 65535) return lx_fail(LX_ENOTSUP, "lx_potential_direct: B * K > 65535");
   DirectArgs a = direct_args(rule, state_d, kern_d, kh, kw);
   a.U = potential_d;

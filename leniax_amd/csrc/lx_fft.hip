@@ -617,10 +617,12 @@ extern "C" int lx_update(const lx_rule* rule, const float* state_d, int64_t B, c
                          float* potential_d, void* work_d, void* stream) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
+  Geom g;
+  if ((rc = make_geom(rule->ndim, rule->dims, g))) return rc;
+  if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
+  if (B == 0) return LX_OK;  // nothing to read or write: pointers may be NULL
   if (!state_d || !K_half_d || !gf_d || !w_d || !dt_d || !state_out_d || !work_d)
     return lx_fail(LX_EINVAL, "NULL pointer");
-  if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
-  if (B == 0) return LX_OK;
   if (state_out_d == state_d) return lx_fail(LX_EINVAL, "lx_update: state_out must not alias state");
   return plain_step(rule, state_d, B, K_half_d, k_bstride, gf_d, gf_bstride, w_d, w_bstride, dt_d, dt_bstride,
                     state_out_d, field_d, potential_d, work_d, (hipStream_t)stream, 0);
@@ -633,10 +635,12 @@ extern "C" int lx_update_with_spectra(const lx_rule* rule, const float* state_d,
                                       void* stream) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
-  if (!state_d || !K_half_d || !gf_d || !w_d || !dt_d || !state_out_d || !state_spec_d || !work_d)
-    return lx_fail(LX_EINVAL, "NULL pointer");
+  Geom g;
+  if ((rc = make_geom(rule->ndim, rule->dims, g))) return rc;
   if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
   if (B == 0) return LX_OK;
+  if (!state_d || !K_half_d || !gf_d || !w_d || !dt_d || !state_out_d || !state_spec_d || !work_d)
+    return lx_fail(LX_EINVAL, "NULL pointer");
   if (state_out_d == state_d) return lx_fail(LX_EINVAL, "lx_update_with_spectra: state_out must not alias state");
   return plain_step(rule, state_d, B, K_half_d, k_bstride, gf_d, gf_bstride, w_d, w_bstride, dt_d, dt_bstride,
                     state_out_d, field_d, potential_d, work_d, (hipStream_t)stream, 0,
@@ -647,19 +651,22 @@ extern "C" int lx_potential(const lx_rule* rule, const float* state_d, int64_t B
                             int64_t k_bstride, float* potential_d, void* work_d, void* stream) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
-  if (!state_d || !K_half_d || !potential_d || !work_d) return lx_fail(LX_EINVAL, "NULL pointer");
+  Geom g;
+  if ((rc = make_geom(rule->ndim, rule->dims, g))) return rc;
   if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
   if (B == 0) return LX_OK;
+  if (!state_d || !K_half_d || !potential_d || !work_d) return lx_fail(LX_EINVAL, "NULL pointer");
   // selection mode: kernel k's potential is written by the row pass of its input channel; no update
   return plain_step(rule, state_d, B, K_half_d, k_bstride, nullptr, 0, nullptr, 0, nullptr, 0, nullptr, nullptr,
                     potential_d, work_d, (hipStream_t)stream, 1);
 }
 
 extern "C" int lx_rfft2(int32_t ndim, const int32_t* dims, int64_t n, const float* x_d, void* X_d, void* stream) {
-  if (!dims || !x_d || !X_d || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  if (!dims || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
   Geom g;
   if (int rc = make_geom(ndim, dims, g)) return rc;
   if (n == 0) return LX_OK;
+  if (!x_d || !X_d) return lx_fail(LX_EINVAL, "NULL pointer");
   hipStream_t st = (hipStream_t)stream;
   float2* X = static_cast<float2*>(X_d);
   if (int rc = launch_rows_fwd(g, x_d, X, n, st)) return rc;
@@ -668,29 +675,32 @@ extern "C" int lx_rfft2(int32_t ndim, const int32_t* dims, int64_t n, const floa
 
 extern "C" int lx_irfft2(int32_t ndim, const int32_t* dims, int64_t n, const void* X_d, float* x_d, void* work_d,
                          void* stream) {
-  if (!dims || !x_d || !X_d || !work_d || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  if (!dims || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
   Geom g;
   if (int rc = make_geom(ndim, dims, g)) return rc;
   if (n == 0) return LX_OK;
+  if (!x_d || !X_d || !work_d) return lx_fail(LX_EINVAL, "NULL pointer");
   return irfft(g, n, static_cast<const float2*>(X_d), x_d, static_cast<float2*>(work_d), (hipStream_t)stream);
 }
 
 extern "C" int lx_half_to_full(int32_t ndim, const int32_t* dims, int64_t n, const void* half_d, void* full_d,
                                float scale, void* stream) {
-  if (!dims || !half_d || !full_d || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  if (!dims || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
   Geom g;
   if (int rc = make_geom(ndim, dims, g)) return rc;
   if (n == 0) return LX_OK;
+  if (!half_d || !full_d) return lx_fail(LX_EINVAL, "NULL pointer");
   return half_to_full(g, n, static_cast<const float2*>(half_d), static_cast<float2*>(full_d), scale,
                       (hipStream_t)stream);
 }
 
 extern "C" int lx_kernel_spectrum(int32_t ndim, const int32_t* dims, int64_t n, const float* k_d, void* K_full_d,
                                   void* work_d, void* stream) {
-  if (!dims || !k_d || !K_full_d || !work_d || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  if (!dims || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
   Geom g;
   if (int rc = make_geom(ndim, dims, g)) return rc;
   if (n == 0) return LX_OK;
+  if (!k_d || !K_full_d || !work_d) return lx_fail(LX_EINVAL, "NULL pointer");
   hipStream_t st = (hipStream_t)stream;
   float2* work = static_cast<float2*>(work_d);
   if (int rc = launch_rows_fwd(g, k_d, work, n, st)) return rc;
@@ -704,13 +714,13 @@ extern "C" int lx_adjoint_potential(const lx_rule* rule, int64_t B, const void* 
                                     void* work_d, void* stream) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
-  if (!K_half_d || !dpot_d || !lambda_d || !work_d) return lx_fail(LX_EINVAL, "NULL pointer");
   if ((state_spec_d == nullptr) != (dspec_d == nullptr))
     return lx_fail(LX_EINVAL, "state_spec and dspec are given together or not at all");
-  if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
-  if (B == 0) return LX_OK;
   Geom g;
   if ((rc = make_geom(rule->ndim, rule->dims, g))) return rc;
+  if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
+  if (B == 0) return LX_OK;
+  if (!K_half_d || !dpot_d || !lambda_d || !work_d) return lx_fail(LX_EINVAL, "NULL pointer");
   hipStream_t st = (hipStream_t)stream;
   const int K = rule->K;
   float2* X = static_cast<float2*>(work_d);

@@ -172,8 +172,9 @@ unsigned grid_for(int64_t h) { return unsigned(std::max<int64_t>(1, std::min<int
 }  // namespace
 
 extern "C" int lx_random_bits(const uint32_t* key, int64_t n, uint32_t* out_d, void* stream) {
-  if (!key || !out_d || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  if (n < 0) return lx_fail(LX_EINVAL, "bad arguments");
   if (n == 0) return LX_OK;
+  if (!key || !out_d) return lx_fail(LX_EINVAL, "NULL pointer");
   if (n > 0xFFFFFFFFll) return lx_fail(LX_ENOTSUP, "at most 2^32 words per request");
   const Stream s = make_stream(key, n);
   hipLaunchKernelGGL(bits_kernel, dim3(grid_for(s.h)), dim3(256), 0, (hipStream_t)stream, s, out_d);
@@ -183,8 +184,9 @@ extern "C" int lx_random_bits(const uint32_t* key, int64_t n, uint32_t* out_d, v
 
 extern "C" int lx_init_uniform(const uint32_t* key, int64_t nb, int64_t cells, const float* maxval_d, float* out_d,
                                void* stream) {
-  if (!key || !maxval_d || !out_d || nb < 0 || cells < 1) return lx_fail(LX_EINVAL, "bad arguments");
+  if (nb < 0 || cells < 1) return lx_fail(LX_EINVAL, "bad arguments");
   if (nb == 0) return LX_OK;
+  if (!key || !maxval_d || !out_d) return lx_fail(LX_EINVAL, "NULL pointer");
   if (nb * cells > 0xFFFFFFFFll) return lx_fail(LX_ENOTSUP, "at most 2^32 cells per request");
   const Stream s = make_stream(key, nb * cells);
   hipLaunchKernelGGL(uniform_kernel, dim3(grid_for(s.h)), dim3(256), 0, (hipStream_t)stream, s, cells, maxval_d,
@@ -195,10 +197,11 @@ extern "C" int lx_init_uniform(const uint32_t* key, int64_t nb, int64_t cells, c
 
 extern "C" int lx_init_perlin(const uint32_t* key, int32_t nb, int32_t H, int32_t W, int32_t r0, int32_t r1,
                               const float* scaling_d, int32_t quantise, float* angles_d, float* out_d, void* stream) {
-  if (!key || !scaling_d || !angles_d || !out_d || nb < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  if (nb < 0) return lx_fail(LX_EINVAL, "bad arguments");
   if (H < 1 || W < 1 || r0 < 1 || r1 < 1 || r0 > H || r1 > W)
     return lx_fail(LX_EINVAL, "perlin lattice %dx%d over a %dx%d world", r0, r1, H, W);
   if (nb == 0) return LX_OK;
+  if (!key || !scaling_d || !angles_d || !out_d) return lx_fail(LX_EINVAL, "NULL pointer");
   hipStream_t st = (hipStream_t)stream;
   const Stream s = make_stream(key, int64_t(nb) * r0 * r1);
   hipLaunchKernelGGL(angles_kernel, dim3(grid_for(s.h)), dim3(256), 0, st, s, angles_d);

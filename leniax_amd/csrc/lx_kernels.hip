@@ -146,13 +146,14 @@ __global__ __launch_bounds__(kThreads) void build_kernel_kernel(BuildArgs a) {
 
 extern "C" int lx_build_kernels(int32_t ndim, const int32_t* dims, const lx_kernel_desc* desc_d, int64_t n,
                                 double R, int32_t max_kr, int32_t basis, float* out_d, void* stream) {
-  if (!dims || !desc_d || !out_d || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  if (!dims || n < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  if (n == 0) return LX_OK;
+  if (!desc_d || !out_d) return lx_fail(LX_EINVAL, "NULL pointer");
   if (ndim != 2) return lx_fail(LX_ENOTSUP, "lx_build_kernels: 2-D kernels (3-D worlds pass theirs as raw)");
   const int H = dims[0], W = dims[1];
   if (H < 1 || W < 1) return lx_fail(LX_EINVAL, "world %dx%d", H, W);
   if (max_kr < 1 || 2 * max_kr > H || 2 * max_kr > W)
     return lx_fail(LX_EINVAL, "kernel radius %d does not fit the %dx%d world", max_kr, H, W);
-  if (n == 0) return LX_OK;
   if (n > 2147483647) return lx_fail(LX_ENOTSUP, "too many kernels");
   hipStream_t st = (hipStream_t)stream;
   const int64_t count = n * (basis ? LX_MAX_RINGS : 1) * int64_t(H) * W;
@@ -167,6 +168,7 @@ extern "C" int lx_build_kernels(int32_t ndim, const int32_t* dims, const lx_kern
 
 extern "C" int lx_build_spectra(int32_t ndim, const int32_t* dims, const lx_kernel_desc* desc_d, int64_t n, double R,
                                 int32_t max_kr, void* K_half_d, float* work_d, void* stream) {
+  if (n == 0) return LX_OK;
   if (!K_half_d || !work_d) return lx_fail(LX_EINVAL, "NULL pointer");
   if (int rc = lx_build_kernels(ndim, dims, desc_d, n, R, max_kr, 0, work_d, stream)) return rc;
   return lx_rfft2(ndim, dims, n, work_d, K_half_d, stream);

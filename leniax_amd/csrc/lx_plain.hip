@@ -126,9 +126,9 @@ extern "C" int lx_field(const lx_rule* rule, const float* potential_d, int64_t B
                         int64_t gf_bstride, const float* w_d, int64_t w_bstride, float* field_d, void* stream) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
-  if (!potential_d || !gf_d || !w_d || !field_d) return lx_fail(LX_EINVAL, "NULL pointer");
   if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
   if (B == 0) return LX_OK;
+  if (!potential_d || !gf_d || !w_d || !field_d) return lx_fail(LX_EINVAL, "NULL pointer");
   const int64_t n = cells_of(rule);
   hipLaunchKernelGGL(field_kernel, dim3(grid_for(B * n, 256)), dim3(256), 0, (hipStream_t)stream, to_dev(rule),
                      potential_d, B, n, gf_d, gf_bstride, w_d, w_bstride, field_d);
@@ -139,9 +139,9 @@ extern "C" int lx_state(const lx_rule* rule, const float* state_d, const float* 
                         int64_t dt_bstride, float* out_d, void* stream) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
-  if (!state_d || !field_d || !dt_d || !out_d) return lx_fail(LX_EINVAL, "NULL pointer");
   if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
   if (B == 0) return LX_OK;
+  if (!state_d || !field_d || !dt_d || !out_d) return lx_fail(LX_EINVAL, "NULL pointer");
   const int64_t nc = cells_of(rule) * rule->C;
   hipLaunchKernelGGL(state_kernel, dim3(grid_for(B * nc, 256)), dim3(256), 0, (hipStream_t)stream, rule->state_fn,
                      state_d, field_d, B, nc, dt_d, dt_bstride, out_d);
@@ -150,9 +150,9 @@ extern "C" int lx_state(const lx_rule* rule, const float* state_d, const float* 
 
 extern "C" int lx_check_heuristics(const float* stats_d, int32_t T, int64_t B, int32_t C, float* cont_d, float* n_d,
                                    void* stream) {
-  if (!stats_d) return lx_fail(LX_EINVAL, "NULL stats");
   if (T < 1 || B < 0 || C < 1 || C > LX_MAX_C) return lx_fail(LX_EINVAL, "bad T/B/C");
   if (B == 0) return LX_OK;
+  if (!stats_d) return lx_fail(LX_EINVAL, "NULL stats");  // cont_d and n_d are optional
   hipLaunchKernelGGL(heuristics_kernel, dim3(grid_for(B, 128)), dim3(128), 0, (hipStream_t)stream, stats_d, T, B, C,
                      cont_d, n_d);
   return launched("check_heuristics");
@@ -161,8 +161,9 @@ extern "C" int lx_check_heuristics(const float* stats_d, int32_t T, int64_t B, i
 extern "C" size_t lx_carry_bytes(void) { return sizeof(lx_stat_carry); }
 
 extern "C" int lx_carry_init(int64_t B, void* carry_d, void* stream) {
-  if (!carry_d || B < 0) return lx_fail(LX_EINVAL, "bad carry arguments");
+  if (B < 0) return lx_fail(LX_EINVAL, "bad carry arguments");
   if (B == 0) return LX_OK;
+  if (!carry_d) return lx_fail(LX_EINVAL, "NULL pointer");
   hipLaunchKernelGGL(carry_init_kernel, dim3(grid_for(B, 128)), dim3(128), 0, (hipStream_t)stream,
                      static_cast<lx_stat_carry*>(carry_d), B);
   return launched("carry_init");
@@ -471,10 +472,10 @@ extern "C" int lx_adjoint_pointwise(const lx_rule* rule, int64_t B, const float*
                                     void* stream) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
-  if (!state_d || !field_d || !potential_d || !gf_d || !w_d || !dt_d || !lambda_d || !lambda_out_d || !dpot_d)
-    return lx_fail(LX_EINVAL, "NULL pointer");
   if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
   if (B == 0) return LX_OK;
+  if (!state_d || !field_d || !potential_d || !gf_d || !w_d || !dt_d || !lambda_d || !lambda_out_d || !dpot_d)
+    return lx_fail(LX_EINVAL, "NULL pointer");
   if (B * std::max(rule->C, rule->K) > 65535) return lx_fail(LX_ENOTSUP, "lx_adjoint_pointwise: B * max(C, K) > 65535");
   AdjArgs a;
   a.A = state_d; a.G = field_d; a.U = potential_d; a.lam = lambda_d; a.dfield = dfield_d; a.dpot_in = dpot_in_d;
@@ -502,9 +503,9 @@ extern "C" int lx_stats(const lx_rule* rule, const float* cells_d, const float* 
                         int64_t B, float R, float T, void* carry_d, float* stats_d, void* stream) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
-  if (!cells_d || !field_d || !potential_d || !carry_d || !stats_d) return lx_fail(LX_EINVAL, "NULL pointer");
   if (B < 0 || !(R > 0.f) || !(T > 0.f)) return lx_fail(LX_EINVAL, "bad B / R / T");
   if (B == 0) return LX_OK;
+  if (!cells_d || !field_d || !potential_d || !carry_d || !stats_d) return lx_fail(LX_EINVAL, "NULL pointer");
   StatArgs sa;
   sa.cells = cells_d; sa.field = field_d; sa.pot = potential_d;
   sa.carry = static_cast<lx_stat_carry*>(carry_d);

@@ -66,6 +66,96 @@ def test_argument_errors_are_reported_without_a_device():
     assert _lib.lib().lx_run_supported(ctypes.byref(r)) == 0
 
 
+def _rule(_lib, dims, C=1, K=1):
+    r = _lib.lx_rule()
+    r.ndim = len(dims)
+    for i, d in enumerate(dims):
+        r.dims[i] = d
+    r.C = C
+    r.K = K
+    r.gf[0] = 1
+    return r
+
+
+def test_empty_batches_are_accepted_with_null_pointers():
+    """B == 0 (n == 0) validates sizes and returns LX_OK before any device call, with every
+    device pointer NULL; B < 0 is an error.  No HIP call happens on these paths."""
+    from leniax_amd import _lib
+    if not _lib.available():
+        pytest.skip('libleniax_amd.so not built')
+    L_ = _lib.lib()
+    r = ctypes.byref(_rule(_lib, [64, 64]))
+    dims = (ctypes.c_int32 * 2)(64, 64)
+    key = (ctypes.c_uint32 * 2)(0, 1)
+    N = None
+    calls = {
+        'lx_update': lambda B: L_.lx_update(r, N, B, N, 0, N, 0, N, 0, N, 0, N, N, N, N, N),
+        'lx_update_with_spectra': lambda B: L_.lx_update_with_spectra(r, N, B, N, 0, N, 0, N, 0, N, 0, N, N, N, N,
+                                                                      N, N),
+        'lx_potential': lambda B: L_.lx_potential(r, N, B, N, 0, N, N, N),
+        'lx_potential_direct': lambda B: L_.lx_potential_direct(r, N, B, N, 27, 27, N, N),
+        'lx_potential_direct_adjoint': lambda B: L_.lx_potential_direct_adjoint(r, N, B, N, 27, 27, N, N, N, N),
+        'lx_field': lambda B: L_.lx_field(r, N, B, N, 0, N, 0, N, N),
+        'lx_state': lambda B: L_.lx_state(r, N, N, B, N, 0, N, N),
+        'lx_carry_init': lambda B: L_.lx_carry_init(B, N, N),
+        'lx_stats': lambda B: L_.lx_stats(r, N, N, N, B, 13., 10., N, N, N),
+        'lx_check_heuristics': lambda B: L_.lx_check_heuristics(N, 5, B, 1, N, N, N),
+        'lx_rfft2': lambda B: L_.lx_rfft2(2, dims, B, N, N, N),
+        'lx_irfft2': lambda B: L_.lx_irfft2(2, dims, B, N, N, N, N),
+        'lx_half_to_full': lambda B: L_.lx_half_to_full(2, dims, B, N, N, 1., N),
+        'lx_kernel_spectrum': lambda B: L_.lx_kernel_spectrum(2, dims, B, N, N, N, N),
+        'lx_adjoint_potential': lambda B: L_.lx_adjoint_potential(r, B, N, 0, N, N, N, N, N, N),
+        'lx_adjoint_pointwise': lambda B: L_.lx_adjoint_pointwise(r, B, N, N, N, N, 0, N, 0, N, 0, N, N, N, 0, N, N,
+                                                                  N, N, N, N),
+        'lx_random_bits': lambda B: L_.lx_random_bits(key, B, N, N),
+        'lx_init_uniform': lambda B: L_.lx_init_uniform(key, B, 64 * 64, N, N, N),
+        'lx_init_perlin': lambda B: L_.lx_init_perlin(key, B, 64, 64, 8, 8, N, 1, N, N, N),
+        'lx_build_kernels': lambda B: L_.lx_build_kernels(2, dims, N, B, 13., 13, 0, N, N),
+        'lx_build_spectra': lambda B: L_.lx_build_spectra(2, dims, N, B, 13., 13, N, N, N),
+    }
+    for name, fn in calls.items():
+        assert fn(0) == _lib.LX_OK, (name, L_.lx_last_error())
+        assert fn(-1) != _lib.LX_OK, name
+    a = _lib.lx_run_args()
+    a.B = 0
+    a.max_steps = 10
+    a.R = 13.
+    a.T = 10.
+    assert L_.lx_run(r, ctypes.byref(a), N) == _lib.LX_OK, L_.lx_last_error()
+    a.B = -1
+    assert L_.lx_run(r, ctypes.byref(a), N) != _lib.LX_OK
+
+
+def test_unsupported_sizes_are_rejected_before_the_batch_is_looked_at():
+    """World sizes the FFT does not cover (2^k in [16, 512], 3 2^k in [48, 384]) and worlds the
+    fused run does not cover are errors even for an empty batch."""
+    from leniax_amd import _lib
+    if not _lib.available():
+        pytest.skip('libleniax_amd.so not built')
+    L_ = _lib.lib()
+    N = None
+    for dims in ([100, 100], [1024, 64], [64, 8], [96, 80]):
+        r = ctypes.byref(_rule(_lib, dims))
+        assert L_.lx_update(r, N, 0, N, 0, N, 0, N, 0, N, 0, N, N, N, N, N) == _lib.LX_ENOTSUP, dims
+        assert 'world size' in L_.lx_last_error().decode()
+        assert L_.lx_potential(r, N, 0, N, 0, N, N, N) == _lib.LX_ENOTSUP
+        d = (ctypes.c_int32 * 2)(*dims)
+        assert L_.lx_rfft2(2, d, 0, N, N, N) == _lib.LX_ENOTSUP
+    for dims in ([16, 16], [48, 48], [384, 96], [512, 512]):
+        r = ctypes.byref(_rule(_lib, dims))
+        assert L_.lx_update(r, N, 0, N, 0, N, 0, N, 0, N, 0, N, N, N, N, N) == _lib.LX_OK, dims
+    a = _lib.lx_run_args()
+    a.B = 0
+    a.R = 13.
+    a.T = 10.
+    for dims, ok in (([32, 32], True), ([128, 128], True), ([256, 256], False), ([64, 128], False)):
+        rc = L_.lx_run(ctypes.byref(_rule(_lib, dims)), ctypes.byref(a), N)
+        assert (rc == _lib.LX_OK) == ok, (dims, L_.lx_last_error())
+    r = _rule(_lib, [64, 64], C=2, K=2)
+    r.cin[1] = 1
+    assert L_.lx_run(ctypes.byref(r), ctypes.byref(a), N) == _lib.LX_ENOTSUP
+
+
 def test_circle_2d_matches_oracle():
     from leniax_amd import kernels
     for R, kp, kf, kfp in [(13, [1., [1.]], 'poly_quad', [4]), (12, [0.82, [1 / 6, 1., 0.]], 'gauss', [0.5]),
