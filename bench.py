@@ -141,7 +141,8 @@ def c2_rate(worlds=256, n=256, steps=8, seed=0):
     n x n, per-world kernels (kernel k reads channel k // 2, writes a random channel; r_k =
     R_k / 15 with R_k ~ U[10, 15]; 1-3 rings, b ~ U[0, 1]; gaussian growth mu ~ U[0.1, 0.4],
     sigma ~ U[0.01, 0.08]; weight h ~ U[0.1, 1]; weighted sum), per-channel U[0, 1) 64x64
-    centred patches; one step = lx_update + lx_stats over the batch."""
+    centred patches; one step = the step and its statistics over the batch (lx_step_stats, no
+    field or potential output, as run_scan_mem_optimized), timed beside lx_update + lx_stats."""
     import torch
     from leniax_amd import _lib, engine, kernels
     dev = torch.device('cuda', 0)
@@ -184,26 +185,45 @@ def c2_rate(worlds=256, n=256, steps=8, seed=0):
     kbs = int(Kh[0].numel())
     st = engine.stream_ptr()
 
-    def step(x, y):
+    def step(x, y, one_pass):
+        if one_pass:  # what run_scan_mem_optimized runs per step: no field / potential output
+            _lib.call('lx_step_stats', ctypes.byref(rule), engine.ptr(x), worlds, engine.ptr(Kh), kbs, engine.ptr(gf),
+                      12, engine.ptr(w), 18, engine.ptr(dt), 0, engine.ptr(y), None, None, ctypes.c_float(15.),
+                      ctypes.c_float(10.), engine.ptr(carry), engine.ptr(stats), engine.ptr(work), st)
+            return
         _lib.call('lx_update', ctypes.byref(rule), engine.ptr(x), worlds, engine.ptr(Kh), kbs, engine.ptr(gf),
                   12, engine.ptr(w), 18, engine.ptr(dt), 0, engine.ptr(y), engine.ptr(field), engine.ptr(pot),
                   engine.ptr(work), st)
         _lib.call('lx_stats', ctypes.byref(rule), engine.ptr(x), engine.ptr(field), engine.ptr(pot), worlds,
                   ctypes.c_float(15.), ctypes.c_float(10.), engine.ptr(carry), engine.ptr(stats), st)
 
-    step(a, bbuf)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(steps):
-        step(a, bbuf)
-        a, bbuf = bbuf, a
-    torch.cuda.synchronize()
-    sec = (time.perf_counter() - t0) / steps
+    def timed(one_pass):
+        nonlocal a, bbuf
+        step(a, bbuf, one_pass)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            step(a, bbuf, one_pass)
+            a, bbuf = bbuf, a
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    sec = timed(True)
+    sec2 = timed(False)
     F = 2.5 * n * n * math.log2(n * n) * 9 + 6 * 6 * n * (n // 2 + 1) + 10 * 6 * n * n + 9 * n * n
-    return {'config': f'{worlds} worlds of {n}x{n}, 3 channels, 6 kernels, per-world spectra, step + statistics',
+    # algorithmic HBM bytes per world-step: state read + write, the world's 6 half spectra
+    Bw = 8 * 3 * n * n + 8 * 6 * n * (n // 2 + 1)
+    cells = worlds * 3 * n * n / sec
+    bound = min(157.3e12 / (F / (3 * n * n)), 8.0e12 / (Bw / (3 * n * n)))  # cell-updates/s
+    return {'config': f'{worlds} worlds of {n}x{n}, 3 channels, 6 kernels, per-world spectra, step + statistics '
+                      '(lx_step_stats, as run_scan_mem_optimized)',
             'ms_per_step': sec * 1e3, 'world_steps_per_s': worlds / sec,
-            'cell_updates_per_s': worlds * 3 * n * n / sec,
-            'flops_per_world_step': F, 'achieved_tflops': worlds * F / sec / 1e12}
+            'cell_updates_per_s': cells,
+            'flops_per_world_step': F, 'achieved_tflops': worlds * F / sec / 1e12,
+            'bytes_per_world_step': Bw, 'achieved_gbps': worlds * Bw / sec / 1e9,
+            'roofline_frac': cells / bound,
+            'two_calls_ms_per_step': sec2 * 1e3,
+            'two_calls_note': 'lx_update with field and potential outputs, then lx_stats'}
 
 
 def c4_adjoint_rate(worlds=16, n=96, steps=128, ckpt=32, reps=2):
@@ -390,8 +410,9 @@ def main():
     dr = direct_rate()
     log(f"direct potential {dr['config']}: {dr['ms']:.3f} ms, {dr['tflops']:.1f} TFLOP/s")
     c2 = c2_rate()
-    log(f"3C6K 256 x 256^2 plain step + statistics: {c2['ms_per_step']:.2f} ms/step, "
-        f"{c2['world_steps_per_s']:.0f} world-steps/s")
+    log(f"3C6K 256 x 256^2 step + statistics in one pass: {c2['ms_per_step']:.3f} ms/step, "
+        f"{c2['world_steps_per_s']:.0f} world-steps/s, {c2['achieved_gbps']:.0f} GB/s algorithmic, "
+        f"{c2['roofline_frac']:.3f} of its bound (update + stats as two calls: {c2['two_calls_ms_per_step']:.3f} ms)")
 
     flops_per_launch = F * steps_run.sum()
     achieved = flops_per_launch / (launch_ms / 1e3)

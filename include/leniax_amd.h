@@ -127,7 +127,7 @@ int lx_init_uniform(const uint32_t* key, int64_t nb, int64_t cells, const float*
 int lx_init_perlin(const uint32_t* key, int32_t nb, int32_t H, int32_t W, int32_t r0, int32_t r1,
                    const float* scaling_d, int32_t quantise, float* angles_d, float* out_d, void* stream);
 
-/* Workspace bytes lx_potential / lx_update / lx_stats need for a batch of B worlds. */
+/* Workspace bytes lx_potential / lx_update / lx_step_stats need for a batch of B worlds. */
 int lx_workspace_size(const lx_rule* rule, int64_t B, size_t* bytes);
 
 /* leniax/core.py:52-102 get_potential_fft: potential_d [B][K][dims] = real(ifftn(fftn(A) * K)).
@@ -186,6 +186,17 @@ size_t lx_carry_bytes(void);
 int lx_carry_init(int64_t B, void* carry_d, void* stream);
 int lx_stats(const lx_rule* rule, const float* cells_d, const float* field_d, const float* potential_d,
              int64_t B, float R, float T, void* carry_d, float* stats_d, void* stream);
+
+/* One scan step of leniax/runner.py:295-334 (_scan_fn: update_fn, then compute_stats_fn on the
+ * step's input state, field and potential): lx_update and lx_stats of the same step, with the
+ * statistics summed by the inverse row pass while it holds the state, field and potential, so
+ * that nothing is read back from HBM for them.  field_d / potential_d may be NULL
+ * (run_scan_mem_optimized, leniax/runner.py:167-215, keeps neither).  work_d:
+ * lx_workspace_size() bytes.  Results equal lx_update + lx_stats up to the order of the sums. */
+int lx_step_stats(const lx_rule* rule, const float* state_d, int64_t B, const void* K_half_d, int64_t k_bstride,
+                  const float* gf_d, int64_t gf_bstride, const float* w_d, int64_t w_bstride, const float* dt_d,
+                  int64_t dt_bstride, float* state_out_d, float* field_d, float* potential_d, float R, float T,
+                  void* carry_d, float* stats_d, void* work_d, void* stream);
 
 /* leniax/statistics.py:134-205 check_heuristics over stats_d [T][B][11 + C] (lx_stats layout):
  * cont_d [T][B] (1.0 / 0.0) and n_d [B] = sum_t cont. */

@@ -124,6 +124,8 @@ SIGNATURES = {
     'lx_carry_bytes': (ctypes.c_size_t, []),
     'lx_carry_init': (ctypes.c_int, [_I64, _P, _P]),
     'lx_stats': (ctypes.c_int, [_R, _P, _P, _P, _I64, _F, _F, _P, _P, _P]),
+    'lx_step_stats': (ctypes.c_int, [_R, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P, _F, _F, _P,
+                                     _P, _P, _P]),
     'lx_check_heuristics': (ctypes.c_int, [_P, _I32, _I64, _I32, _P, _P, _P]),
     'lx_rfft2': (ctypes.c_int, [_I32, ctypes.POINTER(_I32), _I64, _P, _P, _P]),
     'lx_irfft2': (ctypes.c_int, [_I32, ctypes.POINTER(_I32), _I64, _P, _P, _P, _P]),

@@ -352,9 +352,15 @@ struct RowArgs {
   // potential (lambda += correlations) and the inverse real FFT.
   int sel;
   int cin[LX_MAX_K];
+  // statistics mode: the step's world sums of leniax/statistics.py:35-126 over each row pair's
+  // input state, field and potential, LX_PAIR_REC floats per pair at part[pair index] (sA, sG,
+  // nA, nG, nU, sAy[3], sGy[3], sAyy[3]), in the re-centred coordinates of the carried shift
+  float* part;
+  const lx_stat_carry* carry;
+  int ndim, d0, d1;  // rows = d0 d1 (d0 = 1 for 2-D worlds)
 };
 
-template <int W>
+template <int W, bool STATS>
 __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
   using LC = Lanes<W>;
   constexpr int TPR = LC::TPR, P = LC::P, WH = W / 2 + 1;
@@ -376,6 +382,7 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
 #pragma unroll
   for (int m = 0; m < P; ++m) G0[m] = G1[m] = 0.f;
   float den = 0.f;
+  float nU = 0.f;  // statistics: potential cells above eps, over the kernels this channel owns
   float2 x[P];
   for (int k = 0; k < a.K; ++k) {
     const float wk = a.sel ? (a.cin[k] == c ? 1.f : 0.f) : pw[c * a.K + k];
@@ -387,8 +394,10 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
       for (int cc = a.C - 1; cc >= 0; --cc)
         if (pw[cc * a.K + k] != 0.f) owner = cc;
     }
-    const bool want_pot = a.pot != nullptr && owner == c;
-    if (wk == 0.f && !want_pot) continue;
+    // every kernel has one owner channel: it writes the kernel's potential and counts its cells
+    const bool own = owner == c;
+    const bool want_pot = a.pot != nullptr && own;
+    if (wk == 0.f && !want_pot && !(STATS && own)) continue;
     int t = threadIdx.x % TPR;  // opaque per iteration: addresses and twiddles are not hoisted
     __asm__ volatile("" : "+v"(t));
     // stage the two rows' half spectra (coalesced), then repack them as one complex row:
@@ -424,6 +433,7 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
 #pragma unroll
     for (int m = 0; m < P; ++m) {
       const float U0 = x[m].x * inv_n, U1 = x[m].y * inv_n;
+      if (STATS && own) nU += (U0 > LX_EPS ? 1.f : 0.f) + (U1 > LX_EPS ? 1.f : 0.f);
       if (want_pot && live) {
         po0[TPR * m + t] = U0;
         po0[W + TPR * m + t] = U1;
@@ -437,6 +447,16 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
   if (!a.update || !live) return;
   const float dtw = a.sel ? 1.f : a.dt[b * a.dt_bs];
   const int64_t off = (img * a.rows + r2) * int64_t(W);
+  // statistics: per row sums of A, A z, A z^2, G+, G+ z (z the re-centred column) and counts
+  float sAr0 = 0.f, sAr1 = 0.f, sAz = 0.f, sAzz = 0.f, sGr0 = 0.f, sGr1 = 0.f, sGz = 0.f, nA = 0.f, nG = 0.f;
+  int sh[3] = {0, 0, 0};  // carried shift reduced into [0, size) along (d0, d1, W)
+  if (STATS) {
+    auto modp = [](int v, int m) { v %= m; return v < 0 ? v + m : v; };
+    const lx_carry& cc = a.carry[b].c;
+    if (a.ndim == 3) sh[0] = modp(cc.shift[0], a.d0);
+    sh[1] = modp(cc.shift[a.ndim - 2], a.d1);
+    sh[2] = modp(cc.shift[a.ndim - 1], W);
+  }
 #pragma unroll
   for (int m = 0; m < P; ++m) {
     const int col = TPR * m + t;
@@ -444,6 +464,22 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
     if (a.average) { g0 = (g0 / den); g1 = (g1 / den); }
     if (a.field) { a.field[off + col] = g0; a.field[off + W + col] = g1; }
     const float v0 = a.A[off + col], v1 = a.A[off + W + col];
+    if (STATS) {
+      int zc = col - sh[2];
+      zc += zc < 0 ? W : 0;
+      const float cz = float(zc - W / 2);
+      const float gp0 = g0 != g0 ? g0 : fmaxf(g0, 0.f), gp1 = g1 != g1 ? g1 : fmaxf(g1, 0.f);
+      sAr0 += v0;
+      sAr1 += v1;
+      const float vs = v0 + v1, gs2 = gp0 + gp1;
+      sAz = fmaf(vs, cz, sAz);
+      sAzz = fmaf(vs * cz, cz, sAzz);
+      sGr0 += gp0;
+      sGr1 += gp1;
+      sGz = fmaf(gs2, cz, sGz);
+      nA += (v0 > LX_EPS ? 1.f : 0.f) + (v1 > LX_EPS ? 1.f : 0.f);
+      nG += (gp0 > LX_EPS ? 1.f : 0.f) + (gp1 > LX_EPS ? 1.f : 0.f);
+    }
     float n0, n1;
     if (a.state_fn == LX_STATE_V2) {
       n0 = add_nc(mul_nc(v0, sub_nc(1.f, dtw)), mul_nc(dtw, g0));
@@ -458,6 +494,44 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
     }
     a.A_out[off + col] = n0;
     a.A_out[off + W + col] = n1;
+  }
+  if constexpr (STATS) {
+    // re-centred row coordinates of the pair's two rows: (x, y) in 3-D, y in 2-D
+    float cx[2], cy[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int row = int(r2) + i;
+      int x = row / a.d1, y = row - (row / a.d1) * a.d1;
+      x -= sh[0]; x += x < 0 ? a.d0 : 0;
+      y -= sh[1]; y += y < 0 ? a.d1 : 0;
+      cx[i] = float(x - a.d0 / 2);
+      cy[i] = float(y - a.d1 / 2);
+    }
+    const bool d3 = a.ndim == 3;
+    float rec[LX_PAIR_REC];
+    rec[0] = sAr0 + sAr1;
+    rec[1] = sGr0 + sGr1;
+    rec[2] = nA;
+    rec[3] = nG;
+    rec[4] = nU;
+    const float ay = fmaf(sAr0, cy[0], sAr1 * cy[1]), ayy = fmaf(sAr0 * cy[0], cy[0], sAr1 * cy[1] * cy[1]);
+    const float gy = fmaf(sGr0, cy[0], sGr1 * cy[1]);
+    const float ax = fmaf(sAr0, cx[0], sAr1 * cx[1]), axx = fmaf(sAr0 * cx[0], cx[0], sAr1 * cx[1] * cx[1]);
+    const float gx = fmaf(sGr0, cx[0], sGr1 * cx[1]);
+    rec[5] = d3 ? ax : ay;     rec[6] = d3 ? ay : sAz;   rec[7] = d3 ? sAz : 0.f;
+    rec[8] = d3 ? gx : gy;     rec[9] = d3 ? gy : sGz;   rec[10] = d3 ? sGz : 0.f;
+    rec[11] = d3 ? axx : ayy;  rec[12] = d3 ? ayy : sAzz; rec[13] = d3 ? sAzz : 0.f;
+    for (int i = 14; i < LX_PAIR_REC; ++i) rec[i] = 0.f;
+    // sum over the pair's TPR lanes (a butterfly: every lane ends with the same sums), then
+    // lane t stores fields t, t + TPR, ...
+#pragma unroll
+    for (int off2 = TPR / 2; off2 >= 1; off2 >>= 1)
+#pragma unroll
+      for (int i = 0; i < 14; ++i) rec[i] += __shfl_xor(rec[i], off2, 64);
+    float* dst = a.part + pr * LX_PAIR_REC;
+#pragma unroll
+    for (int e = 0; e < LX_PAIR_REC; ++e)
+      if (e % TPR == t) dst[e] = rec[e];
   }
 }
 
@@ -494,7 +568,9 @@ int launch_rows_inv(const Geom& g, RowArgs ra, int64_t nimg, hipStream_t st) {
   return dispatch_len(g.W, [&](auto w) {
     constexpr int W = decltype(w)::value;
     if (int rc = ensure_tw<W>(st)) return rc;
-    hipLaunchKernelGGL(rows_inv_kernel<W>, dim3(blocks_for(ra.npairs * Lanes<W>::TPR)), dim3(256), 0, st, ra);
+    const dim3 grid(blocks_for(ra.npairs * Lanes<W>::TPR));
+    if (ra.part) hipLaunchKernelGGL((rows_inv_kernel<W, true>), grid, dim3(256), 0, st, ra);
+    else hipLaunchKernelGGL((rows_inv_kernel<W, false>), grid, dim3(256), 0, st, ra);
     return launched("rows_inv");
   });
 }
@@ -509,6 +585,10 @@ RowArgs row_args(const lx_rule* rule) {
     ra.cin[k] = k < rule->K ? rule->cin[k] : -1;
   }
   ra.state_fn = rule->state_fn; ra.average = rule->average; ra.update = 0; ra.sel = 0;
+  ra.part = nullptr; ra.carry = nullptr;
+  ra.ndim = rule->ndim;
+  ra.d0 = rule->ndim == 3 ? rule->dims[0] : 1;
+  ra.d1 = rule->dims[rule->ndim - 2];
   return ra;
 }
 
@@ -525,7 +605,8 @@ int fwd_leading(const Geom& g, float2* X, int64_t nimg, hipStream_t st) {
 // One leniax step (or, with sel, the potential alone).  A: [B][C][dims]; work: S then U.
 int plain_step(const lx_rule* rule, const float* A, int64_t B, const void* Kh, int64_t k_bs, const float* gf,
                int64_t gf_bs, const float* w, int64_t w_bs, const float* dt, int64_t dt_bs, float* A_out,
-               float* field, float* pot, void* work, hipStream_t st, int sel, float2* FA = nullptr) {
+               float* field, float* pot, void* work, hipStream_t st, int sel, float2* FA = nullptr,
+               float* part = nullptr, const lx_stat_carry* carry = nullptr) {
   Geom g;
   if (int rc = make_geom(rule->ndim, rule->dims, g)) return rc;
   const int C = rule->C, K = rule->K;
@@ -554,6 +635,7 @@ int plain_step(const lx_rule* rule, const float* A, int64_t B, const void* Kh, i
   ra.U = U; ra.A = A; ra.A_out = A_out; ra.field = field; ra.pot = pot;
   ra.gf = gf; ra.gf_bs = gf_bs; ra.w = w; ra.w_bs = w_bs; ra.dt = dt; ra.dt_bs = dt_bs;
   ra.update = A_out != nullptr; ra.sel = sel;
+  ra.part = part; ra.carry = carry;
   return launch_rows_inv(g, ra, B * C, st);
 }
 
@@ -604,10 +686,12 @@ extern "C" int lx_workspace_size(const lx_rule* rule, int64_t B, size_t* bytes) 
   int rc = lx_check_rule(rule);
   if (rc) return rc;
   if (!bytes || B < 0) return lx_fail(LX_EINVAL, "bad arguments");
-  int64_t half = 1;
-  for (int i = 0; i < rule->ndim - 1; ++i) half *= rule->dims[i];
-  half *= rule->dims[rule->ndim - 1] / 2 + 1;
-  *bytes = size_t(B) * size_t(rule->C + rule->K) * size_t(half) * sizeof(float2);
+  int64_t rows = 1;
+  for (int i = 0; i < rule->ndim - 1; ++i) rows *= rule->dims[i];
+  const int64_t half = rows * (rule->dims[rule->ndim - 1] / 2 + 1);
+  // spectra S [B][C] and U [B][K], then lx_step_stats' row-pair sums [B][C][rows / 2]
+  *bytes = size_t(B) * size_t(rule->C + rule->K) * size_t(half) * sizeof(float2) +
+           size_t(B) * size_t(rule->C) * size_t(rows / 2) * LX_PAIR_REC * sizeof(float);
   return LX_OK;
 }
 
@@ -645,6 +729,30 @@ extern "C" int lx_update_with_spectra(const lx_rule* rule, const float* state_d,
   return plain_step(rule, state_d, B, K_half_d, k_bstride, gf_d, gf_bstride, w_d, w_bstride, dt_d, dt_bstride,
                     state_out_d, field_d, potential_d, work_d, (hipStream_t)stream, 0,
                     static_cast<float2*>(state_spec_d));
+}
+
+extern "C" int lx_step_stats(const lx_rule* rule, const float* state_d, int64_t B, const void* K_half_d,
+                             int64_t k_bstride, const float* gf_d, int64_t gf_bstride, const float* w_d,
+                             int64_t w_bstride, const float* dt_d, int64_t dt_bstride, float* state_out_d,
+                             float* field_d, float* potential_d, float R, float T, void* carry_d, float* stats_d,
+                             void* work_d, void* stream) {
+  int rc = lx_check_rule(rule);
+  if (rc) return rc;
+  Geom g;
+  if ((rc = make_geom(rule->ndim, rule->dims, g))) return rc;
+  if (B < 0 || !(R > 0.f) || !(T > 0.f)) return lx_fail(LX_EINVAL, "bad B / R / T");
+  if (B == 0) return LX_OK;
+  if (!state_d || !K_half_d || !gf_d || !w_d || !dt_d || !state_out_d || !carry_d || !stats_d || !work_d)
+    return lx_fail(LX_EINVAL, "NULL pointer");
+  if (state_out_d == state_d) return lx_fail(LX_EINVAL, "lx_step_stats: state_out must not alias state");
+  const int64_t pairs = g.rows / 2;
+  float* part = reinterpret_cast<float*>(static_cast<float2*>(work_d) + B * (rule->C + rule->K) * g.half);
+  hipStream_t st = (hipStream_t)stream;
+  if ((rc = plain_step(rule, state_d, B, K_half_d, k_bstride, gf_d, gf_bstride, w_d, w_bstride, dt_d, dt_bstride,
+                       state_out_d, field_d, potential_d, work_d, st, 0, nullptr, part,
+                       static_cast<const lx_stat_carry*>(carry_d))))
+    return rc;
+  return lx_pair_stats_launch(rule, part, B, pairs, R, T, carry_d, stats_d, st);
 }
 
 extern "C" int lx_potential(const lx_rule* rule, const float* state_d, int64_t B, const void* K_half_d,

@@ -51,6 +51,8 @@ struct lx_carry {
 // block of the per-world reduction (summed in slot order: deterministic).
 #define LX_STAT_PARTS 16
 #define LX_NSUM (14 + LX_MAX_C)
+// row-pair record of lx_step_stats: sA, sG, nA, nG, nU, sAy[3], sGy[3], sAyy[3], 2 unused
+#define LX_PAIR_REC 16
 struct lx_stat_carry {
   lx_carry c;
   float part[LX_STAT_PARTS][LX_NSUM];
@@ -212,6 +214,11 @@ LX_DEV lx_sums lx_sums_of_record(const float* r) {
 // Error reporting (thread-local message), lx_api.hip
 int lx_fail(int code, const char* fmt, ...);
 int lx_check_rule(const lx_rule* rule);
+
+// Statistics of one step from the row-pair records of lx_step_stats' inverse row pass
+// (lx_plain.hip): pairs records per (world, channel), carry [B] lx_stat_carry, stats [B][11 + C]
+int lx_pair_stats_launch(const lx_rule* rule, const float* part, int64_t B, int64_t pairs, float R, float T,
+                         void* carry_d, float* stats_d, hipStream_t st);
 
 // Fused persistent 2-D run (lx_fused2d.hip)
 int lx_fused2d_supported(const lx_rule* rule);

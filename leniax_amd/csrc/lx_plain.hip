@@ -285,6 +285,49 @@ __global__ void stats_finish_kernel(StatArgs a) {
   sc->c = c;
 }
 
+// The step's statistics from the row-pair records of lx_step_stats' inverse row pass: one wave
+// per world; lane l sums records l, l + 64, ... of the world's C x pairs in order, then a
+// butterfly over the wave (every lane ends with the same sums) and lane 0 finishes the step
+// (leniax/statistics.py:64-126) as stats_finish_kernel does.
+__global__ __launch_bounds__(64) void pair_stats_kernel(StatArgs a, const float* __restrict__ part, int64_t pairs) {
+  const int64_t b = blockIdx.x;
+  const int lane = threadIdx.x;
+  float v[14], cm[LX_MAX_C];
+#pragma unroll
+  for (int i = 0; i < 14; ++i) v[i] = 0.f;
+#pragma unroll
+  for (int ch = 0; ch < LX_MAX_C; ++ch) cm[ch] = 0.f;
+  const int64_t per_world = int64_t(a.C) * pairs;
+  const float4* base = reinterpret_cast<const float4*>(part + b * per_world * LX_PAIR_REC);
+  static_assert(LX_PAIR_REC == 16, "four float4 per record");
+  for (int64_t j = lane; j < per_world; j += 64) {
+    const float4 q0 = base[j * 4], q1 = base[j * 4 + 1], q2 = base[j * 4 + 2], q3 = base[j * 4 + 3];
+    const float r[14] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y};
+#pragma unroll
+    for (int i = 0; i < 14; ++i) v[i] += r[i];
+    const int ch = int(j / pairs);
+#pragma unroll
+    for (int c2 = 0; c2 < LX_MAX_C; ++c2) cm[c2] += c2 == ch ? r[0] : 0.f;
+  }
+#pragma unroll
+  for (int i = 0; i < 14; ++i) v[i] = wave_sum(v[i]);
+#pragma unroll
+  for (int ch = 0; ch < LX_MAX_C; ++ch) cm[ch] = wave_sum(cm[ch]);
+  if (lane != 0) return;
+  lx_stat_carry* sc = a.carry + b;
+  lx_sums s;
+  s.sA = v[0]; s.sG = v[1]; s.nA = v[2]; s.nG = v[3]; s.nU = v[4];
+  for (int d = 0; d < 3; ++d) { s.sAy[d] = v[5 + d]; s.sGy[d] = v[8 + d]; s.sAyy[d] = v[11 + d]; }
+  for (int ch = 0; ch < LX_MAX_C; ++ch) s.sAc[ch] = ch < a.C ? cm[ch] : 0.f;
+  int32_t dd[3];
+  for (int i = 0; i < 3; ++i) dd[i] = a.dims[i];
+  const int64_t n = int64_t(a.ndim == 3 ? a.dims[0] : 1) * a.dims[a.ndim - 2] * a.dims[a.ndim - 1];
+  lx_carry c = sc->c;
+  lx_stats_finish(s, c, a.ndim, dd, a.C, a.R2, a.R, a.dt, LX_STOP_NONE, 0, float(n * a.C),
+                  a.stats + b * (LX_NSTAT_BASE + a.C));
+  sc->c = c;
+}
+
 // ---------------------------------------------------------------- adjoint of the step, pointwise part
 // Forward (leniax/core.py:163-319):  G_c = sum_k w_ck g_k(U_k) (/ D_c = sum_k w_ck),
 //   Z_c = A_c + dt G_c (v1, simple) or A_c (1 - dt) + dt G_c (v2),  A'_c = clip(Z_c, 0, 1) (v1).
@@ -518,4 +561,18 @@ extern "C" int lx_stats(const lx_rule* rule, const float* cells_d, const float* 
   hipLaunchKernelGGL(stats_part_kernel, dim3(LX_STAT_PARTS, (unsigned)B), dim3(256), 0, st, sa);
   hipLaunchKernelGGL(stats_finish_kernel, dim3(grid_for(B, 64)), dim3(64), 0, st, sa);
   return launched("stats");
+}
+
+int lx_pair_stats_launch(const lx_rule* rule, const float* part, int64_t B, int64_t pairs, float R, float T,
+                         void* carry_d, float* stats_d, hipStream_t st) {
+  StatArgs sa;
+  sa.cells = nullptr; sa.field = nullptr; sa.pot = nullptr;
+  sa.carry = static_cast<lx_stat_carry*>(carry_d);
+  sa.stats = stats_d; sa.B = B; sa.ndim = rule->ndim;
+  for (int i = 0; i < 3; ++i) sa.dims[i] = i < rule->ndim ? rule->dims[i] : 1;
+  sa.C = rule->C; sa.K = rule->K;
+  sa.R2 = (float)((double)R * (double)R); sa.R = R; sa.dt = (float)(1.0 / (double)T);
+  if (B > 2147483647) return lx_fail(LX_ENOTSUP, "too many worlds");
+  hipLaunchKernelGGL(pair_stats_kernel, dim3((unsigned)B), dim3(64), 0, st, sa, part, pairs);
+  return launched("pair stats");
 }

@@ -4,8 +4,9 @@
   and the standard statistics run as ONE persistent HIP launch (lx_run) when the
   rule has a fused kernel (2-D N x N, one channel, one kernel, N <= 128): every
   world stays on-chip for all steps, statistics and heuristics are evaluated
-  every step.  Other rules step with the plain HIP step (lx_update + lx_stats)
-  from a host loop, then scan the heuristics on the device.
+  every step.  Other rules step with the plain HIP step and its statistics in
+  one pass (lx_step_stats) from a host loop, then scan the heuristics on the
+  device.
 * ``run`` keeps leniax's Python loop (different heuristics, optional truncation).
 """
 import ctypes
@@ -64,21 +65,17 @@ def _scan_plain(rule, slots, cells0, K_half, kbs, gf, w, dt, max_run_iter, R, T,
         cells_h = torch.empty((max_run_iter, B, C) + dims, dtype=torch.float32, device=dev)
         field_h = torch.empty((max_run_iter, B, C) + dims, dtype=torch.float32, device=dev)
         pot_h = torch.empty((max_run_iter, B, Ke) + dims, dtype=torch.float32, device=dev)
-    else:
-        field_h = torch.empty((1, B, C) + dims, dtype=torch.float32, device=dev)
-        pot_h = torch.empty((1, B, Ke) + dims, dtype=torch.float32, device=dev)
     cur = cells0.clone()
     nxt = torch.empty_like(cur)
     for t in range(max_run_iter):
-        f = field_h[t if keep else 0]
-        p = pot_h[t if keep else 0]
+        # one step and its statistics (lx_step_stats); field and potential only where kept
+        f = field_h[t] if keep else None
+        p = pot_h[t] if keep else None
         if keep:
             cells_h[t].copy_(cur)
-        _lib.call('lx_update', ctypes.byref(rule), engine.ptr(cur), B, engine.ptr(K_half), kbs, engine.ptr(gf), gfs,
-                  engine.ptr(w), ws, engine.ptr(dt), dts, engine.ptr(nxt), engine.ptr(f), engine.ptr(p),
-                  engine.ptr(work), engine.stream_ptr())
-        _lib.call('lx_stats', ctypes.byref(rule), engine.ptr(cur), engine.ptr(f), engine.ptr(p), B, float(R), float(T),
-                  engine.ptr(carry), engine.ptr(rows[t]), engine.stream_ptr())
+        _lib.call('lx_step_stats', ctypes.byref(rule), engine.ptr(cur), B, engine.ptr(K_half), kbs, engine.ptr(gf),
+                  gfs, engine.ptr(w), ws, engine.ptr(dt), dts, engine.ptr(nxt), engine.ptr(f), engine.ptr(p),
+                  float(R), float(T), engine.ptr(carry), engine.ptr(rows[t]), engine.ptr(work), engine.stream_ptr())
         cur, nxt = nxt, cur
     cont = torch.empty((max_run_iter, B), dtype=torch.float32, device=dev)
     n = torch.empty((B,), dtype=torch.float32, device=dev)

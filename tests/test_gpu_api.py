@@ -242,3 +242,118 @@ def test_run_scan_mem_optimized_matches_oracle(plain):
     assert e64 <= max(1e-5, 4 * o64)
     np.testing.assert_allclose(_np(stats['mass']), so['mass'], rtol=1e-3, atol=1e-5)
     np.testing.assert_array_equal(_np(stats['N']), so['N'])
+
+
+def _rule_case(case):
+    """(rule, K_half [S, K, ...], gf [B, K, 2], w [B, C, K], cells [B, C, *dims]) of a test case."""
+    from leniax_amd import engine, _lib
+    rng = np.random.default_rng(11)
+    if case == '3c6k_64':
+        B, dims, C = 3, [64, 64], 3
+        cin = [0, 0, 1, 1, 2, 2]
+        Kh_l, gfs, ws = [], [], []
+        for b in range(B):
+            kp = [{'k_slug': 'circle_2d', 'k_params': [float(rng.uniform(0.6, 1.)), [1., float(rng.uniform())]],
+                   'kf_slug': 'gauss_bump', 'kf_params': [1.], 'gf_slug': 'gaussian',
+                   'gf_params': [float(rng.uniform(0.1, 0.4)), float(rng.uniform(0.01, 0.08))],
+                   'h': float(rng.uniform(0.1, 1.)), 'c_in': k // 2, 'c_out': int(rng.integers(0, 3))}
+                  for k in range(6)]
+            Ko, mp = L.get_kernels_and_mapping(kp, dims, C, 10., np.float32)
+            rule = engine.make_rule(dims, C, cin, [_lib.GROWTH_IDS['gaussian']] * 6, 0, 0)
+            Kh_l.append(engine.prepare_spectra(rule, Ko, list(range(6))))  # two kernels per input channel
+            gfs.append(mp.get_gf_params())
+            ws.append(mp.get_kernels_weight_per_channel())
+        Kh = torch.cat(Kh_l)
+        gf = torch.as_tensor(np.stack(gfs)).cuda()
+        w = torch.as_tensor(np.stack(ws)).cuda()
+    else:
+        if case == '1c_48x96':
+            dims, R = [48, 96], 8.
+        else:  # 3-D
+            dims, R = [32, 48, 32], 5.
+        B, C = 2, 1
+        if len(dims) == 2:
+            Ko, mp = L.get_kernels_and_mapping([{'k_slug': 'circle_2d', 'k_params': [1., [1.]],
+                                                 'kf_slug': 'gauss_bump', 'kf_params': [1.], 'gf_slug': 'gaussian',
+                                                 'gf_params': [0.15, 0.02], 'h': 1., 'c_in': 0, 'c_out': 0}],
+                                               dims, 1, R, np.float32)
+        else:
+            k = L.sphere_nd(R, 1.0, [1.0, 0.5], 'gauss_bump', [1.0], 3, np.float64)[0]
+            pads = [((ws_ - ks) // 2, ws_ - ks - (ws_ - ks) // 2) for ws_, ks in zip(dims, k.shape)]
+            Ko = np.fft.fftn(np.fft.fftshift(np.pad(k, pads)))[None, None, None].astype(np.complex64)
+        rule = engine.make_rule(dims, 1, [0], [_lib.GROWTH_IDS['gaussian']], 0, 1)
+        Kh = engine.prepare_spectra(rule, Ko, [0])
+        gf = torch.tensor([[[0.15, 0.02]]]).cuda()
+        w = torch.tensor([[[1.]]]).cuda()
+    cells = np.zeros([B, C] + dims, np.float32)
+    sl = tuple(slice(d // 4, 3 * d // 4) for d in dims)
+    for b in range(B):
+        for c in range(C):
+            cells[(b, c) + sl] = rng.random([d // 2 for d in dims], dtype=np.float32)
+    return rule, Kh.contiguous(), gf.contiguous(), w.contiguous(), torch.as_tensor(cells).cuda()
+
+
+@pytest.mark.parametrize('case', ['3c6k_64', '1c_48x96', '3d_32x48x32'])
+def test_step_stats_equals_update_then_stats(case):
+    """lx_step_stats (statistics summed in the inverse row pass) against lx_update + lx_stats over
+    8 steps: states, fields and potentials bit for bit, statistics to the order of their sums,
+    and the same with the field and potential outputs left out."""
+    import ctypes
+    from leniax_amd import engine, _lib
+    print('step_stats', case, flush=True)
+    rule, Kh, gf, w, a0 = _rule_case(case)
+    B, C = a0.shape[:2]
+    dims = tuple(a0.shape[2:])
+    kbs = int(Kh[0].numel()) if Kh.shape[0] > 1 else 0
+    gfs = int(gf[0].numel()) if gf.shape[0] > 1 else 0
+    wss = int(w[0].numel()) if w.shape[0] > 1 else 0
+    dt = torch.tensor([0.1], device=a0.device)
+    nb = ctypes.c_size_t(0)
+    _lib.call('lx_workspace_size', ctypes.byref(rule), B, ctypes.byref(nb))
+    work = torch.empty(nb.value, dtype=torch.uint8, device=a0.device)
+    st = engine.stream_ptr()
+    R, T = 10., 10.
+    nst = _lib.LX_NSTAT_BASE + C
+    paths = {}
+    for mode in ('two_calls', 'one_pass', 'one_pass_no_outputs'):
+        cur = a0.clone()
+        nxt = torch.empty_like(cur)
+        carry = engine.carry_init(B)
+        field = torch.empty((B, C) + dims, device=a0.device)
+        pot = torch.empty((B, rule.K) + dims, device=a0.device)
+        rows, outs = [], []
+        for t in range(8):
+            row = torch.empty((B, nst), device=a0.device)
+            if mode == 'two_calls':
+                _lib.call('lx_update', ctypes.byref(rule), engine.ptr(cur), B, engine.ptr(Kh), kbs, engine.ptr(gf),
+                          gfs, engine.ptr(w), wss, engine.ptr(dt), 0, engine.ptr(nxt), engine.ptr(field),
+                          engine.ptr(pot), engine.ptr(work), st)
+                _lib.call('lx_stats', ctypes.byref(rule), engine.ptr(cur), engine.ptr(field), engine.ptr(pot), B,
+                          ctypes.c_float(R), ctypes.c_float(T), engine.ptr(carry), engine.ptr(row), st)
+            else:
+                keep = mode == 'one_pass'
+                _lib.call('lx_step_stats', ctypes.byref(rule), engine.ptr(cur), B, engine.ptr(Kh), kbs,
+                          engine.ptr(gf), gfs, engine.ptr(w), wss, engine.ptr(dt), 0, engine.ptr(nxt),
+                          engine.ptr(field) if keep else None, engine.ptr(pot) if keep else None, ctypes.c_float(R),
+                          ctypes.c_float(T), engine.ptr(carry), engine.ptr(row), engine.ptr(work), st)
+            rows.append(row)
+            outs.append((nxt.clone(), field.clone(), pot.clone()))
+            cur, nxt = nxt, cur
+        paths[mode] = (torch.stack(rows).cpu().numpy(), outs)
+    ref_rows, ref_outs = paths['two_calls']
+    for mode in ('one_pass', 'one_pass_no_outputs'):
+        got_rows, got_outs = paths[mode]
+        for t in range(8):
+            assert torch.equal(got_outs[t][0], ref_outs[t][0]), (mode, t)
+            if mode == 'one_pass':
+                assert torch.equal(got_outs[t][1], ref_outs[t][1]) and torch.equal(got_outs[t][2], ref_outs[t][2])
+        # the statistics' own bound against the oracle (test_compute_stats_matches_oracle): the speeds
+        # and the angle difference centroids, which amplifies the order of the sums
+        err = np.abs(got_rows - ref_rows) / np.maximum(1., np.abs(ref_rows))
+        print(mode, 'max stat diff', float(err.max()), 'per stat', np.round(err.max(axis=(0, 1)), 7).tolist())
+        assert float(err.max()) < 2e-3
+        np.testing.assert_allclose(got_rows[..., 0], ref_rows[..., 0], rtol=1e-5)  # mass
+        np.testing.assert_allclose(got_rows[..., _lib.LX_NSTAT_BASE:], ref_rows[..., _lib.LX_NSTAT_BASE:], rtol=1e-5)
+        # counts (volumes) are exact integers whatever the order of the sums
+        for i in (1, 4, 10):  # mass_volume, growth_volume, potential_volume
+            np.testing.assert_array_equal(got_rows[..., i], ref_rows[..., i])
