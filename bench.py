@@ -312,6 +312,8 @@ def main():
     ap.add_argument('--only-fused', action='store_true', help='time the fused launch only (profiling runs)')
     ap.add_argument('--copy-mb', type=int, default=0,
                     help='after timing, copy this many MiB device-to-device (a known-size kernel for PMC scaling)')
+    ap.add_argument('--only-c2', action='store_true',
+                    help='time the 3C6K 256^2 step-and-statistics workload only (profiling runs)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: every world batch fits one MI355X and worlds are independent '
@@ -323,6 +325,15 @@ def main():
 
     dev = torch.device('cuda', 0)
     torch.cuda.set_device(dev)
+    if args.only_c2:
+        print(json.dumps(c2_rate(steps=max(args.steps, 1))), flush=True)
+        if args.copy_mb:
+            src = torch.ones(args.copy_mb * 2 ** 18, device=dev)
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            log(f'copied {args.copy_mb} MiB device to device')
+        return
     K, mp = kernels.get_kernels_and_mapping(kernel_params(), [N, N], 1, R)
     rule = engine.make_rule([N, N], 1, [0], [_lib.GROWTH_IDS['gaussian']], 0, 1)
     Kh = engine.prepare_spectra(rule, K, [0])
