@@ -131,6 +131,16 @@ LX_DEV void fft_group(float2* x, int t, int lane, float2* xch) {
 
 LX_DEV void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
 
+// XCD-aware block order.  The hardware deals a grid's workgroups to the 8 XCDs in turn (block b
+// runs on XCD b % 8 as that XCD's (b / 8)-th block); each XCD has its own L2.  The logical block
+// of b is chosen so that every XCD runs one contiguous range of logical blocks: neighbouring
+// blocks, whose lines share 128-byte cache lines (a column group's 16 columns straddle two lines
+// of each 1032-byte half-spectrum row), then meet in the same L2.  A bijection of [0, nb).
+LX_DEV int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t per = nb / 8, rem = nb % 8, x = b % 8, i = b / 8;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
 // ---------------------------------------------------------------- geometry
 struct Geom {
   int ndim, W, WH;
@@ -250,7 +260,7 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
   const int lane = threadIdx.x & 63;
   const int t = threadIdx.x % TPR;
   float2* xch = xch_all + (LC::XL ? (threadIdx.x / TPR) * (TPR * LC::XRS) : 0);
-  const int64_t lid = (int64_t(blockIdx.x) * 256 + threadIdx.x) / TPR;
+  const int64_t lid = (xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x) / TPR;
   const bool live = lid < a.nlines;
   const int64_t per_img = int64_t(a.A) * a.WH;
   const int64_t img = live ? lid / per_img : 0;
