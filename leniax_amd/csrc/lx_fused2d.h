@@ -456,7 +456,7 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
           red[wave][4] = float(u);
         }
       }
-      ++t;  // no barrier: the next step's Z barrier orders the red writes before the closing  // no barrier: the next step's first one orders thread 0's writes before their readers
+      ++t;  // no barrier: the next step's Z barrier orders the red writes before the closing
     }
 
     // close the last step run (its sums are in red), unless the loop stopped on a closed one

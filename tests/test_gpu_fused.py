@@ -104,9 +104,19 @@ def test_fused_stats_match_oracle():
                                                         hist=False)
     from leniax_amd import engine
     st = engine.stats_to_dict(res['stats'][0].cpu().numpy(), 1)
+    R2 = np.float32(meta['R']) ** 2
     for k in engine.STAT_KEYS:
         o = o_stats[k][:, 0]
         g = st[k]
+        if k == 'potential_volume':
+            # #(U > 1e-7) / R^2: far from the pattern U is float32 FFT noise of order 1e-7 (the
+            # kernel's potential of a state agrees with the oracle's to 2e-7), so a few cells
+            # near the threshold count on one side only: at most 0.1% of the world's cells
+            near = (np.abs(o_pot[:, 0, 0] - np.float32(1e-7)) < 2e-7).sum(axis=(-2, -1))
+            dn = np.abs(g - o) * R2
+            print(f'stat {k:18s} max count diff {dn.max():.0f} (cells within 2e-7 of eps: {near.min()}..{near.max()})')
+            assert dn.max() <= 0.001 * cells0[0, 0].size, k
+            continue
         scale = np.maximum(1., np.abs(o))
         err = float((np.abs(g - o) / scale).max())
         print(f'stat {k:18s} max rel err {err:.2e}')
@@ -155,3 +165,63 @@ def test_fused_probe_and_resume_equal_one_uninterrupted_run(stop):
         assert np.array_equal(ps[b, :n], fs[b, :n]), b
         ref = hist[n, b] if n < steps else fin_full[b]
         assert np.array_equal(fin_part[b], ref), b
+
+
+GROWTH_CASES = [('poly_quad4', [0.15, 0.015], 'v1'), ('gaussian', [0.15, 0.015], 'v1'),
+                ('gaussian_target', [0.15, 0.03], 'v1'), ('step', [0.15, 0.03], 'v1'),
+                ('staircase', [0.15, 0.05], 'v1'), ('triangle', [0.15, 0.05], 'v1'), ('identity', [0., 0.], 'v1'),
+                ('gaussian', [0.15, 0.015], 'v2'), ('gaussian', [0.15, 0.015], 'simple'),
+                ('poly_quad4', [0.15, 0.015], 'v2'), ('staircase', [0.15, 0.05], 'simple')]
+
+
+@pytest.mark.parametrize('slug,gfp,state', GROWTH_CASES, ids=[f'{c[0]}-{c[2]}' for c in GROWTH_CASES])
+def test_fused_every_growth_and_state_function_stage_by_stage(slug, gfp, state):
+    """Each growth-function and state-function specialisation of the fused kernel, one stage at a
+    time on the kernel's own inputs (so that neither chaos nor a threshold of the discontinuous
+    growth functions amplifies rounding): per step, the potential of the state the kernel held
+    against the oracle's, the field against the oracle's growth of the kernel's potential, and the
+    next state against the oracle's update of the kernel's state and field.  The run without
+    history ends in the same state bit for bit."""
+    engine, _lib = _engine()
+    N, B, R, T, steps = 64, 2, 9., 10., 6
+    print('fused stages', slug, state, flush=True)
+    rng = np.random.default_rng(3)
+    cells0 = np.zeros((B, 1, N, N), np.float32)
+    for b in range(B):
+        cells0[b, 0, 16:48, 16:48] = rng.random((32, 32), dtype=np.float32)
+    kp = [{'k_slug': 'circle_2d', 'k_params': [1., [1.]], 'kf_slug': 'gauss_bump', 'kf_params': [1.],
+           'gf_slug': slug, 'gf_params': gfp, 'h': 1., 'c_in': 0, 'c_out': 0}]
+    K, mp = L.get_kernels_and_mapping(kp, [N, N], 1, R, np.float32)
+    rule = engine.make_rule([N, N], 1, [0], [_lib.GROWTH_IDS[slug]], _lib.STATE_FN[state], 1)
+    Kh = engine.prepare_spectra(rule, K, [0])
+    dev = engine.device()
+    c0 = torch.as_tensor(cells0).to(dev)
+    gf = torch.tensor([[gfp]], device=dev, dtype=torch.float32)
+    w = torch.ones(1, 1, 1, device=dev)
+    dt32 = np.float32(1) / np.float32(T)
+    dt = torch.tensor([dt32], device=dev)
+    res = engine.run_fused(rule, c0, Kh, False, gf, w, dt, steps, 'none', 0, R, T, want_stats=True, want_hist=True)
+    res2 = engine.run_fused(rule, c0, Kh, False, gf, w, dt, steps, 'none', 0, R, T, want_stats=True)
+    torch.cuda.synchronize()
+    cells_h, field_h, pot_h = (h.cpu().numpy() for h in res['hist'])
+    final = res['cells'].cpu().numpy()
+    assert torch.equal(res['cells'], res2['cells'])
+    np.testing.assert_array_equal(res['stats'].cpu().numpy(), res2['stats'].cpu().numpy())
+    gfo = np.asarray([gfp], np.float32)
+    wo = np.ones((1, 1), np.float32)
+    eu = eg = ea = 0.
+    for t in range(steps):
+        a = cells_h[t]
+        u = L.get_potential_fft(a, K, mp.tc_indices())
+        eu = max(eu, float(np.abs(pot_h[t] - u).max()))
+        g = L.get_field(pot_h[t], gfo, wo, [slug], True)
+        eg = max(eg, float(np.abs(field_h[t] - g).max()))
+        nxt = L.STATE_FNS[state](a, field_h[t], dt32)
+        after = cells_h[t + 1] if t + 1 < steps else final
+        ea = max(ea, float(np.abs(after - nxt).max()))
+    print(f'  potential {eu:.2e}  field {eg:.2e}  state {ea:.2e}')
+    assert eu < 1e-5
+    assert eg < 2e-5  # the Gaussians' exp2 form; the other growths agree exactly on equal potentials
+    assert ea < 1e-6
+    if slug in ('step', 'staircase', 'identity'):
+        assert eg == 0.
