@@ -357,3 +357,43 @@ def test_step_stats_equals_update_then_stats(case):
         # counts (volumes) are exact integers whatever the order of the sums
         for i in (1, 4, 10):  # mass_volume, growth_volume, potential_volume
             np.testing.assert_array_equal(got_rows[..., i], ref_rows[..., i])
+
+
+HBM_GROWTHS = ['poly_quad4', 'gaussian', 'gaussian_target', 'step', 'staircase', 'triangle', 'identity']
+
+
+@pytest.mark.parametrize('state', ['v1', 'v2', 'simple'])
+@pytest.mark.parametrize('average', [True, False])
+def test_update_every_growth_stage_by_stage(state, average):
+    """The HBM step with all seven growth functions in one rule (2 channels, 7 kernels, one growth
+    function each), stage by stage on the step's own values: potential against the oracle's,
+    field against the oracle's growth and weighting of the step's potential, new state against the
+    oracle's update of the step's field."""
+    from leniax_amd import core
+    from leniax_amd.growth_functions import register as gfr
+    print('update stages', state, average, flush=True)
+    H = W = 64
+    C = 2
+    params = {'poly_quad4': [0.15, 0.015], 'gaussian': [0.2, 0.03], 'gaussian_target': [0.15, 0.03],
+              'step': [0.15, 0.03], 'staircase': [0.2, 0.05], 'triangle': [0.15, 0.05], 'identity': [0., 0.]}
+    kp = [{'k_slug': 'circle_2d', 'k_params': [0.5 + 0.07 * i, [1., 0.5][:1 + i % 2]], 'kf_slug': 'gauss_bump',
+           'kf_params': [1.], 'gf_slug': g, 'gf_params': params[g], 'h': 0.3 + 0.1 * i, 'c_in': i % C,
+           'c_out': (i // 2) % C} for i, g in enumerate(HBM_GROWTHS)]
+    Ko, mp = L.get_kernels_and_mapping([dict(k) for k in kp], [H, W], C, 12., np.float32)
+    rng = np.random.default_rng(21)
+    A = (rng.random((2, C, H, W), dtype=np.float32) * (rng.random((2, C, H, W)) > 0.5)).astype(np.float32)
+    gfn = tuple(gfr[s] for s in mp.gf_slugs())
+    dt = np.float32(0.1)
+    new, field, pot = core.fused_update(A, Ko, mp.get_gf_params(), mp.get_kernels_weight_per_channel(), dt,
+                                        mp.tc_indices(), gfn, core.weighted_mean if average else core.weighted_sum,
+                                        state)
+    pot, field, new = _np(pot), _np(field), _np(new)
+    Uo = L.get_potential_fft(A, Ko, mp.tc_indices())
+    eu = float(np.abs(pot - Uo).max())
+    Go = L.get_field(pot, mp.get_gf_params(), mp.get_kernels_weight_per_channel(), mp.gf_slugs(), average)
+    eg = float(np.abs(field - Go).max())
+    ea = float(np.abs(new - L.STATE_FNS[state](A, field, dt)).max())
+    print(f'  potential {eu:.2e}  field {eg:.2e}  state {ea:.2e}')
+    assert eu < 2e-6
+    assert eg < 1e-5  # the reciprocal forms of the Gaussians and of the quartic
+    assert ea < 1e-6
