@@ -91,8 +91,12 @@ LX_DEV float lx_fmod_floor(float x, float y) {
 // One step of leniax's heuristic scan (leniax/statistics.py:144-194, carry of :186-194):
 // updates the counters and the cumulative continue flag; c.n counts continue steps.
 LX_DEV void lx_heuristic_step(lx_carry& c, float mass, const float* cm, float mass_volume, int C) {
+  // loops run to the compile-time bounds with guards, so that per-channel and per-axis arrays
+  // stay in registers (no private-memory indexing) when C and ndim are run-time values
   if (c.t == 0) {
-    for (int ch = 0; ch < C; ++ch) c.init_cm[ch] = cm[ch];
+#pragma unroll
+    for (int ch = 0; ch < LX_MAX_C; ++ch)
+      if (ch < C) c.init_cm[ch] = cm[ch];
     c.prev_mass = mass;
     c.prev_sign = 0.f;
     c.mono = 0;
@@ -101,7 +105,9 @@ LX_DEV void lx_heuristic_step(lx_carry& c, float mass, const float* cm, float ma
     c.n = 0;
   }
   int cond = 1;
-  for (int ch = 0; ch < C; ++ch) {
+#pragma unroll
+  for (int ch = 0; ch < LX_MAX_C; ++ch) {
+    if (ch >= C) break;
     cond &= (cm[ch] >= LX_EPS) ? 1 : 0;               // min_channel_mass_heuristic :224-236
     cond &= (cm[ch] <= 3.f * c.init_cm[ch]) ? 1 : 0;  // max_channel_mass_heuristic :239-251
   }
@@ -133,7 +139,9 @@ LX_DEV void lx_stats_values(const lx_sums& s, float* centroid, float& angle, int
   float cen[3], d[3], gc[3];
   float dist2 = 0.f, gd2 = 0.f, inertia = 0.f;
   const float m00e = s.sA + LX_EPS, g00e = s.sG + LX_EPS, m002e = s.sA * s.sA + LX_EPS;
-  for (int i = 0; i < ndim; ++i) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (i >= ndim) break;
     cen[i] = s.sAy[i] / m00e;
     d[i] = cen[i] - centroid[i];
     dist2 += d[i] * d[i];
@@ -160,10 +168,14 @@ LX_DEV void lx_stats_values(const lx_sums& s, float* centroid, float& angle, int
     out[LX_ST_MASS_GROWTH_DIST] = mass_growth_dist;
     out[LX_ST_INERTIA] = inertia;
     out[LX_ST_POTENTIAL_VOLUME] = potential_volume;
-    for (int ch = 0; ch < C; ++ch) out[LX_ST_CHANNEL_MASS0 + ch] = s.sAc[ch] / R2;
+#pragma unroll
+    for (int ch = 0; ch < LX_MAX_C; ++ch)
+      if (ch < C) out[LX_ST_CHANNEL_MASS0 + ch] = s.sAc[ch] / R2;
   }
   // carried centroid: the fractional part left after the truncating shift (:117-124)
-  for (int i = 0; i < ndim; ++i) centroid[i] = cen[i] - (float)(int)cen[i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (i < ndim) centroid[i] = cen[i] - (float)(int)cen[i];
   angle = ang;
 }
 
@@ -172,14 +184,17 @@ LX_DEV void lx_stats_values(const lx_sums& s, float* centroid, float& angle, int
 LX_DEV int lx_stats_control(const lx_sums& s, lx_carry& c, int ndim, const int32_t* dims, int C, float R2,
                             int stop_policy, int min_steps, float cells_total) {
   const float m00e = s.sA + LX_EPS;
-  for (int i = 0; i < ndim; ++i) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (i >= ndim) break;
     const float cen = s.sAy[i] / m00e;
     int si = (int)cen;
     int v = (c.shift[i] + si) % dims[i];
     c.shift[i] = v < 0 ? v + dims[i] : v;
   }
   float cm[LX_MAX_C];
-  for (int ch = 0; ch < C; ++ch) cm[ch] = s.sAc[ch] / R2;
+#pragma unroll
+  for (int ch = 0; ch < LX_MAX_C; ++ch) cm[ch] = ch < C ? s.sAc[ch] / R2 : 0.f;
   lx_heuristic_step(c, s.sA / R2, cm, s.nA / R2, C);
   switch (stop_policy) {
     case LX_STOP_LENIAX:

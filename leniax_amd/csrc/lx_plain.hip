@@ -285,40 +285,50 @@ __global__ void stats_finish_kernel(StatArgs a) {
   sc->c = c;
 }
 
-// The step's statistics from the row-pair records of lx_step_stats' inverse row pass: one wave
-// per world; lane l sums records l, l + 64, ... of the world's C x pairs in order, then a
-// butterfly over the wave (every lane ends with the same sums) and lane 0 finishes the step
-// (leniax/statistics.py:64-126) as stats_finish_kernel does.
-__global__ __launch_bounds__(64) void pair_stats_kernel(StatArgs a, const float* __restrict__ part, int64_t pairs) {
+// The step's statistics from the row-pair records of lx_step_stats' inverse row pass: one
+// 256-thread block per world; thread i sums records i, i + 256, ... of the world's C x pairs in
+// order, then butterflies within each wave and the four wave sums in order (deterministic), and
+// thread 0 finishes the step (leniax/statistics.py:64-126) as stats_finish_kernel does.
+constexpr int kPairThreads = 256;
+__global__ __launch_bounds__(kPairThreads) void pair_stats_kernel(StatArgs a, const float* __restrict__ part,
+                                                                  int64_t pairs) {
+  constexpr int NV = 14 + LX_MAX_C;  // the record's 14 sums, then the channel masses
+  __shared__ float red[kPairThreads / 64][NV];
   const int64_t b = blockIdx.x;
-  const int lane = threadIdx.x;
-  float v[14], cm[LX_MAX_C];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  float v[NV];
 #pragma unroll
-  for (int i = 0; i < 14; ++i) v[i] = 0.f;
-#pragma unroll
-  for (int ch = 0; ch < LX_MAX_C; ++ch) cm[ch] = 0.f;
+  for (int i = 0; i < NV; ++i) v[i] = 0.f;
   const int64_t per_world = int64_t(a.C) * pairs;
   const float4* base = reinterpret_cast<const float4*>(part + b * per_world * LX_PAIR_REC);
   static_assert(LX_PAIR_REC == 16, "four float4 per record");
-  for (int64_t j = lane; j < per_world; j += 64) {
+  for (int64_t j = tid; j < per_world; j += kPairThreads) {
     const float4 q0 = base[j * 4], q1 = base[j * 4 + 1], q2 = base[j * 4 + 2], q3 = base[j * 4 + 3];
     const float r[14] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y};
 #pragma unroll
     for (int i = 0; i < 14; ++i) v[i] += r[i];
     const int ch = int(j / pairs);
 #pragma unroll
-    for (int c2 = 0; c2 < LX_MAX_C; ++c2) cm[c2] += c2 == ch ? r[0] : 0.f;
+    for (int c2 = 0; c2 < LX_MAX_C; ++c2) v[14 + c2] += c2 == ch ? r[0] : 0.f;
   }
 #pragma unroll
-  for (int i = 0; i < 14; ++i) v[i] = wave_sum(v[i]);
+  for (int i = 0; i < NV; ++i) {
+    const float w = wave_sum(v[i]);
+    if (lane == 0) red[wave][i] = w;
+  }
+  __syncthreads();
+  if (tid != 0) return;
 #pragma unroll
-  for (int ch = 0; ch < LX_MAX_C; ++ch) cm[ch] = wave_sum(cm[ch]);
-  if (lane != 0) return;
+  for (int i = 0; i < NV; ++i) {
+    float acc = 0.f;
+    for (int wv = 0; wv < kPairThreads / 64; ++wv) acc += red[wv][i];
+    v[i] = acc;
+  }
   lx_stat_carry* sc = a.carry + b;
   lx_sums s;
   s.sA = v[0]; s.sG = v[1]; s.nA = v[2]; s.nG = v[3]; s.nU = v[4];
   for (int d = 0; d < 3; ++d) { s.sAy[d] = v[5 + d]; s.sGy[d] = v[8 + d]; s.sAyy[d] = v[11 + d]; }
-  for (int ch = 0; ch < LX_MAX_C; ++ch) s.sAc[ch] = ch < a.C ? cm[ch] : 0.f;
+  for (int ch = 0; ch < LX_MAX_C; ++ch) s.sAc[ch] = ch < a.C ? v[14 + ch] : 0.f;
   int32_t dd[3];
   for (int i = 0; i < 3; ++i) dd[i] = a.dims[i];
   const int64_t n = int64_t(a.ndim == 3 ? a.dims[0] : 1) * a.dims[a.ndim - 2] * a.dims[a.ndim - 1];
@@ -573,6 +583,6 @@ int lx_pair_stats_launch(const lx_rule* rule, const float* part, int64_t B, int6
   sa.C = rule->C; sa.K = rule->K;
   sa.R2 = (float)((double)R * (double)R); sa.R = R; sa.dt = (float)(1.0 / (double)T);
   if (B > 2147483647) return lx_fail(LX_ENOTSUP, "too many worlds");
-  hipLaunchKernelGGL(pair_stats_kernel, dim3((unsigned)B), dim3(64), 0, st, sa, part, pairs);
+  hipLaunchKernelGGL(pair_stats_kernel, dim3((unsigned)B), dim3(kPairThreads), 0, st, sa, part, pairs);
   return launched("pair stats");
 }
