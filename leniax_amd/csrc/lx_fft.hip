@@ -370,11 +370,16 @@ struct RowArgs {
   int ndim, d0, d1;  // rows = d0 d1 (d0 = 1 for 2-D worlds)
 };
 
-template <int W, bool STATS>
+// UNI: every wave's row pairs lie in one image (the host checks that the pairs per image are a
+// multiple of the pairs per wave), so the world's weights and growth parameters are the same for
+// the whole wave: the wave copies them into LDS once instead of loading them from global memory
+// in each kernel iteration, and its branches on them are uniform.
+template <int W, bool STATS, bool UNI>
 __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
   using LC = Lanes<W>;
   constexpr int TPR = LC::TPR, P = LC::P, WH = W / 2 + 1;
   __shared__ __align__(16) float2 xch_all[LC::XCH];
+  __shared__ float wpar[UNI ? 4 : 1][UNI ? LX_MAX_C * LX_MAX_K + 2 * LX_MAX_K : 1];
   const int lane = threadIdx.x & 63;
   const int t = threadIdx.x % TPR;
   float2* xch = xch_all + (threadIdx.x / TPR) * (TPR * LC::XRS);
@@ -383,11 +388,24 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
   const int64_t half_rows = a.rows / 2;
   const int64_t img = live ? pr / half_rows : 0;
   const int64_t r2 = live ? (pr - img * half_rows) * 2 : 0;
-  const int64_t b = img / a.C;
-  const int c = int(img - b * a.C);
+  int64_t b = img / a.C;
+  int c = int(img - b * a.C);
   const float inv_n = 1.f / float(a.rows * W);
   const float* pw = a.sel ? nullptr : a.w + b * a.w_bs;
   const float* pg = a.sel ? nullptr : a.gf + b * a.gf_bs;
+  if constexpr (UNI) {
+    b = __builtin_amdgcn_readfirstlane(int(b));
+    c = __builtin_amdgcn_readfirstlane(c);
+    if (!a.sel) {
+      float* ws = wpar[threadIdx.x >> 6];
+      const int nw = a.C * a.K;
+      for (int i = lane; i < nw + 2 * a.K; i += 64)
+        ws[i] = i < nw ? a.w[b * a.w_bs + i] : a.gf[b * a.gf_bs + (i - nw)];
+      wave_lds_fence();
+      pw = ws;
+      pg = ws + nw;
+    }
+  }
   float G0[P], G1[P];
 #pragma unroll
   for (int m = 0; m < P; ++m) G0[m] = G1[m] = 0.f;
@@ -579,8 +597,14 @@ int launch_rows_inv(const Geom& g, RowArgs ra, int64_t nimg, hipStream_t st) {
     constexpr int W = decltype(w)::value;
     if (int rc = ensure_tw<W>(st)) return rc;
     const dim3 grid(blocks_for(ra.npairs * Lanes<W>::TPR));
-    if (ra.part) hipLaunchKernelGGL((rows_inv_kernel<W, true>), grid, dim3(256), 0, st, ra);
-    else hipLaunchKernelGGL((rows_inv_kernel<W, false>), grid, dim3(256), 0, st, ra);
+    const bool uni = (ra.rows / 2) % (64 / Lanes<W>::TPR) == 0;  // waves never straddle images
+    if (ra.part) {
+      if (uni) hipLaunchKernelGGL((rows_inv_kernel<W, true, true>), grid, dim3(256), 0, st, ra);
+      else hipLaunchKernelGGL((rows_inv_kernel<W, true, false>), grid, dim3(256), 0, st, ra);
+    } else {
+      if (uni) hipLaunchKernelGGL((rows_inv_kernel<W, false, true>), grid, dim3(256), 0, st, ra);
+      else hipLaunchKernelGGL((rows_inv_kernel<W, false, false>), grid, dim3(256), 0, st, ra);
+    }
     return launched("rows_inv");
   });
 }
