@@ -458,17 +458,40 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
     const float gm = a.sel ? 0.f : pg[2 * k], gs = a.sel ? 1.f : pg[2 * k + 1];
     const float inv_s = 1.f / gs, inv_9s2 = 1.f / (9.f * (gs * gs));
     float* po0 = want_pot ? a.pot + ((b * a.K + k) * a.rows + r2) * W : nullptr;
+    // the growth function is chosen once per kernel, outside the loop over the lane's points
+    // (KIND < 0: selection mode, identity)
+    auto accumulate = [&](auto kind_c) {
+      constexpr int KIND = decltype(kind_c)::value;
 #pragma unroll
-    for (int m = 0; m < P; ++m) {
-      const float U0 = x[m].x * inv_n, U1 = x[m].y * inv_n;
-      if (STATS && own) nU += (U0 > LX_EPS ? 1.f : 0.f) + (U1 > LX_EPS ? 1.f : 0.f);
-      if (want_pot && live) {
-        po0[TPR * m + t] = U0;
-        po0[W + TPR * m + t] = U1;
+      for (int m = 0; m < P; ++m) {
+        const float U0 = x[m].x * inv_n, U1 = x[m].y * inv_n;
+        if (STATS && own) nU += (U0 > LX_EPS ? 1.f : 0.f) + (U1 > LX_EPS ? 1.f : 0.f);
+        if (want_pot && live) {
+          po0[TPR * m + t] = U0;
+          po0[W + TPR * m + t] = U1;
+        }
+        if (a.update) {
+          float g0 = U0, g1 = U1;
+          if constexpr (KIND >= 0) {
+            g0 = growth_pre<KIND>(gm, gs, inv_s, inv_9s2, U0);
+            g1 = growth_pre<KIND>(gm, gs, inv_s, inv_9s2, U1);
+          }
+          G0[m] = add_nc(G0[m], mul_nc(wk, g0));
+          G1[m] = add_nc(G1[m], mul_nc(wk, g1));
+        }
       }
-      if (a.update) {
-        G0[m] = add_nc(G0[m], mul_nc(wk, a.sel ? U0 : growth_rcp(a.gfk[k], gm, gs, inv_s, inv_9s2, U0)));
-        G1[m] = add_nc(G1[m], mul_nc(wk, a.sel ? U1 : growth_rcp(a.gfk[k], gm, gs, inv_s, inv_9s2, U1)));
+    };
+    if (a.sel) {
+      accumulate(IntC<-1>{});
+    } else {
+      switch (a.gfk[k]) {
+        case LX_GF_POLY_QUAD4: accumulate(IntC<LX_GF_POLY_QUAD4>{}); break;
+        case LX_GF_GAUSSIAN: accumulate(IntC<LX_GF_GAUSSIAN>{}); break;
+        case LX_GF_GAUSSIAN_TARGET: accumulate(IntC<LX_GF_GAUSSIAN_TARGET>{}); break;
+        case LX_GF_STEP: accumulate(IntC<LX_GF_STEP>{}); break;
+        case LX_GF_STAIRCASE: accumulate(IntC<LX_GF_STAIRCASE>{}); break;
+        case LX_GF_TRIANGLE: accumulate(IntC<LX_GF_TRIANGLE>{}); break;
+        default: accumulate(IntC<LX_GF_IDENTITY>{}); break;
       }
     }
   }
