@@ -456,17 +456,21 @@ __global__ __launch_bounds__(256) void adj_growth_kernel(AdjArgs a) {
 
 // One channel, one kernel: the whole pointwise reverse in one pass (grid: blocks per image x B).
 // Reads A, G, U and lambda once; lambda_out may alias lambda (each thread reads before it writes).
+// KIND is the kernel's growth function, so that its value and derivative are selected at compile
+// time instead of per cell (same expressions as growth_rcp / growth_grad: the same values).
+template <int KIND>
 __global__ __launch_bounds__(256) void adj_11_kernel(AdjArgs a) {
   const int64_t b = blockIdx.y;
   const float w = a.w[b * a.w_bs];
   const float gm = a.gf[b * a.gf_bs], gs = a.gf[b * a.gf_bs + 1];
   const float inv_s = 1.f / gs, inv_9s2 = 1.f / (9.f * (gs * gs));
   const float dtw = a.dt[b * a.dt_bs];
-  const int kind = a.gfk[0];
+  constexpr int kind = KIND;
   const float coef = a.average ? w / w : w;      // dG -> dg (weighted mean of one kernel: w / w)
   const float inv_den = a.average ? 1.f / w : 1.f;
   const int64_t base = b * a.cells;
   float acc[4] = {0.f, 0.f, 0.f, 0.f};  // dm, ds, dw, ddt
+#pragma unroll 2
   for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < a.cells; p += int64_t(gridDim.x) * blockDim.x) {
     const float A = a.A[base + p], G = a.G[base + p], U = a.U[base + p];
     const float dz = adj_dz(a, A, G, a.lam[base + p], dtw);
@@ -543,7 +547,18 @@ extern "C" int lx_adjoint_pointwise(const lx_rule* rule, int64_t B, const float*
   const unsigned per_img = unsigned(std::min<int64_t>((a.cells + 1023) / 1024, 256));
   hipStream_t st = (hipStream_t)stream;
   if (rule->C == 1 && rule->K == 1) {
-    hipLaunchKernelGGL(adj_11_kernel, dim3(per_img, unsigned(B)), dim3(256), 0, st, a);
+    const dim3 grid(per_img, unsigned(B));
+    switch (rule->gf[0]) {
+      case LX_GF_POLY_QUAD4: hipLaunchKernelGGL(adj_11_kernel<LX_GF_POLY_QUAD4>, grid, dim3(256), 0, st, a); break;
+      case LX_GF_GAUSSIAN: hipLaunchKernelGGL(adj_11_kernel<LX_GF_GAUSSIAN>, grid, dim3(256), 0, st, a); break;
+      case LX_GF_GAUSSIAN_TARGET:
+        hipLaunchKernelGGL(adj_11_kernel<LX_GF_GAUSSIAN_TARGET>, grid, dim3(256), 0, st, a);
+        break;
+      case LX_GF_STEP: hipLaunchKernelGGL(adj_11_kernel<LX_GF_STEP>, grid, dim3(256), 0, st, a); break;
+      case LX_GF_STAIRCASE: hipLaunchKernelGGL(adj_11_kernel<LX_GF_STAIRCASE>, grid, dim3(256), 0, st, a); break;
+      case LX_GF_TRIANGLE: hipLaunchKernelGGL(adj_11_kernel<LX_GF_TRIANGLE>, grid, dim3(256), 0, st, a); break;
+      default: hipLaunchKernelGGL(adj_11_kernel<LX_GF_IDENTITY>, grid, dim3(256), 0, st, a); break;
+    }
   } else {
     // growth first: it reads lambda, which the state kernel may overwrite in place
     hipLaunchKernelGGL(adj_growth_kernel, dim3(per_img, unsigned(B * rule->K)), dim3(256), 0, st, a);
