@@ -32,15 +32,17 @@ _WORK = {}
 
 
 def _spectra(rule, K, slots):
-    """Half spectra for ``K`` (cached on the tensor's storage, version and slot list)."""
-    key = (K.data_ptr(), K._version, tuple(K.shape), tuple(slots), int(rule.K))
+    """Half spectra for ``K`` (cached on the tensor's storage, version and slot list).  An entry
+    keeps its ``K`` alive, so that no other tensor can be allocated at a cached address while the
+    entry exists (a freed and reused address would otherwise hit a stale entry)."""
+    key = (K.device.index, K.data_ptr(), K._version, tuple(K.shape), tuple(K.stride()), tuple(slots), int(rule.K))
     hit = _SPEC_CACHE.get(key)
     if hit is not None:
         _SPEC_CACHE.move_to_end(key)
-        return hit
+        return hit[1]
     S = K.shape[0]
     Kh = engine.prepare_spectra(rule, K.reshape((S,) + tuple(K.shape[1:])), slots)
-    _SPEC_CACHE[key] = Kh
+    _SPEC_CACHE[key] = (K, Kh)
     while len(_SPEC_CACHE) > 16:
         _SPEC_CACHE.popitem(last=False)
     return Kh

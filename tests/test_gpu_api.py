@@ -397,3 +397,26 @@ def test_update_every_growth_stage_by_stage(state, average):
     assert eu < 2e-6
     assert eg < 1e-5  # the reciprocal forms of the Gaussians and of the quartic
     assert ea < 1e-6
+
+
+def test_spectra_cache_follows_a_new_kernel_at_a_reused_address():
+    """core's half-spectrum cache is keyed on the tensor's address: a spectrum freed and
+    reallocated with other values (torch's allocator hands the same block back) must not hit the
+    old entry."""
+    from leniax_amd import core
+    from leniax_amd.growth_functions import register as gfr
+    N = 64
+    rng = np.random.default_rng(2)
+    A = rng.random((1, 1, N, N), dtype=np.float32)
+    outs = []
+    for r in (0.5, 0.9):
+        kp = [{'k_slug': 'circle_2d', 'k_params': [r, [1.]], 'kf_slug': 'gauss_bump', 'kf_params': [1.],
+               'gf_slug': 'gaussian', 'gf_params': [0.15, 0.015], 'h': 1., 'c_in': 0, 'c_out': 0}]
+        Ko, mp = L.get_kernels_and_mapping(kp, [N, N], 1, 10., np.float32)
+        K = torch.as_tensor(Ko).cuda()
+        _, _, pot = core.fused_update(A, K, mp.get_gf_params(), mp.get_kernels_weight_per_channel(), np.float32(0.1),
+                                      mp.tc_indices(), (gfr['gaussian'],), core.weighted_mean, 'v1')
+        outs.append((_np(pot), L.get_potential_fft(A, Ko, mp.tc_indices())))
+        del K
+    for got, ref in outs:
+        assert float(np.abs(got - ref).max()) < 2e-6
