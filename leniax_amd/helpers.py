@@ -72,6 +72,59 @@ def build_get_field_fn(cin_gfs: List[List[str]], average: bool = True) -> Callab
     return functools.partial(core.get_field, growth_fn_t=growth_fn_t, weighted_fn=weighted_fn)
 
 
+def search_for_init(rng_key, config: Dict, fft: bool = True) -> Tuple[Dict, int]:
+    """leniax/helpers.py:319-398: draw ``nb_init_search`` initial states (``algo.init_slug``) and
+    keep the first run with the most steps whose heuristics held, stopping at the first one that
+    holds for ``max_run_iter`` steps.  Returns ``(best_run, i)``: ``best_run`` = ``{'N',
+    'all_cells' [T, 1, C, *world], 'all_stats'}`` (empty when no run held a step) and ``i`` the
+    index of the last initial state leniax's loop would have tried.
+
+    Every initial state runs at once here (one batched run: the fused kernel where the rule has
+    one), and only the chosen one runs again to keep its frames; the choice follows leniax's
+    sequential loop exactly (first strict maximum, early exit at ``max_run_iter``)."""
+    wp = config['world_params']
+    nb_channels = wp['nb_channels']
+    slug = wp.get('get_state_fn_slug', 'v1')
+    average = wp.get('weighted_average', True)
+    R = wp['R']
+    T = np.float32(wp['T'])
+    world_size = config['render_params']['world_size']
+    kernels_params = config['kernels_params']
+    nb_init_search = config['run_params']['nb_init_search']
+    max_run_iter = config['run_params']['max_run_iter']
+    init_slug = config['algo']['init_slug']
+
+    K, mapping = lx_kernels.get_kernels_and_mapping(kernels_params, world_size, nb_channels, R, fft)
+    gf_params = mapping.get_gf_params()
+    weights = mapping.get_kernels_weight_per_channel()
+    update_fn = build_update_fn(K.shape, mapping, slug, average, fft)
+    compute_stats_fn = lx_stat.build_compute_stats_fn(wp, config['render_params'])
+
+    from . import initializations as lx_init
+    rng_key, noises = lx_init.register[init_slug](rng_key, nb_channels * nb_init_search, world_size, R,
+                                                  kernels_params[0]['gf_params'])
+    # the noises have the world's shape: create_init_cells' centred merge into zeros is the noise itself
+    cells0 = engine.as_device(noises).reshape([nb_init_search, nb_channels] + list(world_size)).contiguous()
+    subkeys = lx_init.split(rng_key, nb_init_search)
+
+    # N of every initial state in one run (leniax/runner.py:161-162 per state)
+    stats, _ = lx_runner.run_scan_mem_optimized(None, cells0[None], engine.as_kernels(K)[None], gf_params[None],
+                                                weights[None], np.array([T], np.float32), max_run_iter, R, update_fn,
+                                                compute_stats_fn)
+    n_all = [int(v) for v in stats['N'][0].cpu().tolist()]
+    best, current_max, i = None, 0, 0
+    for i, n in enumerate(n_all):
+        if current_max < n:
+            current_max, best = n, i
+        if n >= max_run_iter:
+            break
+    if best is None:
+        return {}, i
+    all_cells, _, _, all_stats = lx_runner.run_scan(subkeys[best], cells0[best:best + 1], K, gf_params, weights, T,
+                                                    max_run_iter, R, update_fn, compute_stats_fn)
+    return {'N': all_stats['N'], 'all_cells': all_cells, 'all_stats': all_stats}, i
+
+
 def build_update_fn(kernel_shape, mapping: lx_kernels.KernelMapping, get_state_fn_slug: str = 'v1',
                     average_weight: bool = True, fft: bool = True) -> Callable:
     """leniax/helpers.py:401-427."""

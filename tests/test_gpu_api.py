@@ -420,3 +420,48 @@ def test_spectra_cache_follows_a_new_kernel_at_a_reused_address():
         del K
     for got, ref in outs:
         assert float(np.abs(got - ref).max()) < 2e-6
+
+
+def test_search_for_init_follows_leniax_loop():
+    """leniax/helpers.py:319-398: the batched search picks the run leniax's sequential loop picks
+    (first strict maximum of N, early exit at max_run_iter), checked against the oracle's run_scan
+    of each drawn initial state; the returned frames are the chosen run's."""
+    from leniax_amd import helpers, initializations
+    cfg = {
+        'algo': {'init_slug': 'perlin'},
+        'kernels_params': [{'c_in': 0, 'c_out': 0, 'gf_params': [0.15, 0.015], 'gf_slug': 'poly_quad4', 'h': 1,
+                            'k_params': [1, [1.0]], 'k_slug': 'circle_2d', 'kf_params': [4], 'kf_slug': 'poly_quad'}],
+        'render_params': {'world_size': [64, 64]},
+        'run_params': {'max_run_iter': 48, 'nb_init_search': 4, 'seed': 1},
+        'world_params': {'R': 9, 'T': 10, 'nb_channels': 1, 'nb_dims': 2, 'get_state_fn_slug': 'v1',
+                         'weighted_average': True},
+    }
+    key = initializations.PRNGKey(1)
+    best, i = helpers.search_for_init(key, cfg, fft=True)
+    # the oracle on the same drawn states, in leniax's loop
+    _, noises = initializations.perlin(key, 4, [64, 64], 9, [0.15, 0.015])
+    states = noises.cpu().numpy().reshape(4, 1, 1, 64, 64)
+    Ko, mp = L.get_kernels_and_mapping([dict(k) for k in cfg['kernels_params']], [64, 64], 1, 9, np.float32)
+    rule = L.Rule(mp.gf_slugs(), mp.tc_indices(), True, 'v1')
+    sfn = L.build_compute_stats_fn([64, 64], 9, 10, np.float32)
+    cur, best_o, runs = 0, None, []
+    for j in range(4):
+        cells, _, _, st = L.run_scan(states[j], Ko, mp.get_gf_params(), mp.get_kernels_weight_per_channel(), 10.,
+                                     48, 9, rule, sfn)
+        n = int(st['N'][0])
+        runs.append((n, cells))
+        if cur < n:
+            cur, best_o = n, j
+        if n >= 48:
+            break
+    print('N per init (oracle):', [r[0] for r in runs], 'chosen', best_o, 'i', j, '| device N', best.get('N'), 'i', i)
+    assert i == j
+    if best_o is None:
+        assert best == {}
+        return
+    assert int(best['N'][0]) == runs[best_o][0]
+    got = best['all_cells'].cpu().numpy()
+    assert got.shape == (48, 1, 1, 64, 64)
+    np.testing.assert_array_equal(got[0], runs[best_o][1][0])  # the chosen drawn state itself
+    # a steep growth on a noise state: per-step float32 rounding grows quickly, so the first steps
+    assert float(np.abs(got[:4] - runs[best_o][1][:4]).max()) < 1e-5
