@@ -191,6 +191,28 @@ def run_scan_mem_optimized(rng_key, cells0, K, gf_params, kernels_weight_per_cha
     return stats, final.reshape(cells0.shape)
 
 
+def run_scan_mem_optimized_pmap(rng_key, cells0, K, gf_params, kernels_weight_per_channel, T, max_run_iter: int,
+                                R: float, update_fn: Callable, compute_stats_fn: Callable):
+    """leniax/runner.py:218-268 (pmap over devices of the vmapped runner), on this one GPU: the
+    device axis is one more batch axis.  cells0 ``[N_device, S, N_init, C, ...]``, K
+    ``[N_device, S, 1, C, max_k, ...]``, gf ``[N_device, S, nb_k, 2]``, weights
+    ``[N_device, S, C, nb_k]``, T ``[N_device, S]``; returns stats ``[N_device, S, T, N_init]``
+    (``N`` ``[N_device, S, N_init]``) and the final cells ``[N_device, S, N_init, C, ...]``."""
+    cells0 = engine.as_device(cells0)
+    D, S = cells0.shape[:2]
+
+    def flat(x, dtype=torch.float32):
+        t = engine.as_device(x, dtype)
+        return t.reshape((D * S,) + tuple(t.shape[2:]))
+
+    K = engine.as_kernels(K)
+    stats, final = run_scan_mem_optimized(rng_key, flat(cells0), K.reshape((D * S,) + tuple(K.shape[2:])),
+                                          flat(gf_params), flat(kernels_weight_per_channel),
+                                          flat(T).cpu().numpy(), max_run_iter, R, update_fn, compute_stats_fn)
+    stats = {k: v.reshape((D, S) + tuple(v.shape[1:])) for k, v in stats.items()}
+    return stats, final.reshape((D, S) + tuple(final.shape[1:]))
+
+
 def run(rng_key, cells, K, gf_params, kernels_weight_per_channel, T, max_run_iter: int, R: float,
         update_fn: Callable, compute_stats_fn: Callable, stat_trunc: bool = False):
     """leniax/runner.py:16-116: Python loop, total-mass heuristics, ``N`` = last iteration."""

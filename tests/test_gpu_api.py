@@ -465,3 +465,35 @@ def test_search_for_init_follows_leniax_loop():
     np.testing.assert_array_equal(got[0], runs[best_o][1][0])  # the chosen drawn state itself
     # a steep growth on a noise state: per-step float32 rounding grows quickly, so the first steps
     assert float(np.abs(got[:4] - runs[best_o][1][:4]).max()) < 1e-5
+
+
+def test_run_scan_mem_optimized_pmap_is_the_flat_run():
+    """leniax/runner.py:218-268 on one GPU: the device axis folds into the solutions axis; the
+    result equals run_scan_mem_optimized over the flattened solutions, reshaped back."""
+    from leniax_amd import helpers, kernels, runner, statistics as S
+    N, D, Sn, Ni = 64, 2, 2, 2
+    rng = np.random.default_rng(9)
+    Ks, gfs, ws = [], [], []
+    for s in range(D * Sn):
+        kp = [{'k_slug': 'circle_2d', 'k_params': [1., [1.]], 'kf_slug': 'gauss_bump', 'kf_params': [1.],
+               'gf_slug': 'gaussian', 'gf_params': [0.15 + 0.01 * s, 0.02], 'h': 1., 'c_in': 0, 'c_out': 0}]
+        K, mp = kernels.get_kernels_and_mapping(kp, [N, N], 1, 8.)
+        Ks.append(K)
+        gfs.append(mp.get_gf_params())
+        ws.append(mp.get_kernels_weight_per_channel())
+    cells0 = np.zeros((D * Sn, Ni, 1, N, N), np.float32)
+    cells0[..., 16:48, 16:48] = rng.random((D * Sn, Ni, 1, 32, 32), dtype=np.float32)
+    T = np.array([10., 5., 8., 10.], np.float32)
+    upd = helpers.build_update_fn(Ks[0].shape, mp, 'v1', True)
+    sfn = S.build_compute_stats_fn({'R': 8., 'T': 10.}, {'world_size': [N, N]})
+    K, gf, w = torch.stack(Ks), torch.stack(gfs), torch.stack(ws)
+    st, fin = runner.run_scan_mem_optimized(None, cells0, K, gf, w, T, 12, 8., upd, sfn)
+    stp, finp = runner.run_scan_mem_optimized_pmap(
+        None, cells0.reshape((D, Sn) + cells0.shape[1:]), K.reshape((D, Sn) + tuple(K.shape[1:])),
+        gf.reshape((D, Sn) + tuple(gf.shape[1:])), w.reshape((D, Sn) + tuple(w.shape[1:])), T.reshape(D, Sn), 12, 8.,
+        upd, sfn)
+    assert tuple(finp.shape) == (D, Sn, Ni, 1, N, N)
+    assert tuple(stp['mass'].shape) == (D, Sn, 12, Ni) and tuple(stp['N'].shape) == (D, Sn, Ni)
+    assert torch.equal(finp.reshape(fin.shape), fin)
+    for k in st:
+        assert torch.equal(stp[k].reshape(st[k].shape), st[k]), k
