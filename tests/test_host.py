@@ -218,3 +218,32 @@ def test_compress_round_trips():
     assert code.endswith('!')
     np.testing.assert_allclose(loader.deprecated_decompress_array(code, 3), a, atol=1e-7)
     np.testing.assert_allclose(codec.rle_decode(code, 3), a, atol=1e-7)
+
+
+def test_crop_and_centre_utilities_known_answers():
+    """leniax/utils.py:296-388 (saved-creature format): crop_zero drops the all-zero lines;
+    auto_center moves the mass centroid onto the middle cell; center_and_crop returns a creature
+    that wraps around a border as its own contiguous box; an empty state gives one zero cell."""
+    from leniax_amd import utils as U
+    rng = np.random.default_rng(4)
+    patch = rng.uniform(0.1, 1., (1, 5, 6)).astype(np.float32)
+    w = np.zeros((1, 32, 40), np.float32)
+    w[:, 3:8, 10:16] = patch
+    np.testing.assert_array_equal(U.crop_zero(w), patch)
+    w4 = np.zeros((2, 8, 9, 10), np.float32)
+    w4[1, 2:4, 3:6, 4:9] = 1.
+    assert U.crop_zero(w4).shape == (2, 2, 3, 5)
+    with pytest.raises(ValueError):
+        U.crop_zero(np.zeros((1, 2, 2, 2, 2)))
+    # a symmetric blob (centroid exactly at its centre cell) moves onto the middle cell
+    blob = np.zeros((1, 64, 64), np.float32)
+    blob[0, 9:12, 40:43] = 1.  # centre (10, 41); middle (32, 32)
+    c = U.auto_center(blob)
+    np.testing.assert_array_equal(c, np.roll(blob, (22, -9), axis=(1, 2)))
+    assert c[0, 31:34, 31:34].sum() == 9.
+    # a creature across the right/left border comes back whole
+    big = np.zeros((1, 128, 128), np.float32)
+    big[0, 60:65, 125:128] = patch[0, :, :3]
+    big[0, 60:65, 0:3] = patch[0, :, 3:]
+    np.testing.assert_array_equal(U.center_and_crop(big), patch)
+    np.testing.assert_array_equal(U.center_and_crop(np.zeros((2, 16, 16), np.float32)), np.zeros((2, 1, 1)))
