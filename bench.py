@@ -288,6 +288,7 @@ def direct_rate(worlds=256, n=128, reps=10):
     from leniax_amd import core, kernels
     dev = torch.device('cuda', 0)
     K, _ = kernels.get_kernels_and_mapping(kernel_params(), [n, n], 1, R, fft=False)  # [1, 1, kh, kw]
+    K = torch.as_tensor(K).to(dev)  # resident, as leniax's jnp arrays are
     A = torch.as_tensor(make_worlds(worlds, n)).to(dev)
     pad = ((0, 0), (0, 0)) + tuple((d // 2, d // 2 - 1) if d % 2 == 0 else (d // 2, d // 2) for d in K.shape[2:])
     core.get_potential(A, K, pad)

@@ -60,7 +60,12 @@ def _workspace(rule, B):
 
 
 def _dt_tensor(dt, N):
-    t = engine.as_device(dt if not isinstance(dt, (int, float)) else np.float32(dt)).reshape(-1)
+    if isinstance(dt, (int, float, np.floating, np.integer)) or (isinstance(dt, np.ndarray) and dt.size == 1):
+        # a scalar: filled on the device (no pageable host copy, which would wait for the stream)
+        t = torch.full((1,), float(np.float32(np.asarray(dt).reshape(-1)[0])), dtype=torch.float32,
+                       device=engine.device())
+        return t, 0
+    t = engine.as_device(dt).reshape(-1)
     if t.numel() not in (1, N):
         raise ValueError(f'dt must be a scalar or have one value per world ({N}), got {t.numel()}')
     return t.contiguous(), (1 if t.numel() > 1 else 0)
@@ -134,7 +139,8 @@ def get_potential(state, K, padding=None, tc_indices=None, channel_first=True):
             raise ValueError(f'padding {padding} is not the kernel-centred one (size // 2 before)')
     maxk = Kr.shape[0] // C
     slots = _slots(C, maxk, tc_indices)
-    kern = Kr[list(slots), 0].contiguous()
+    # every slot kept: a view (no index tensor to build and copy to the device per call)
+    kern = Kr[:, 0].contiguous() if list(slots) == list(range(Kr.shape[0])) else Kr[list(slots), 0].contiguous()
     rule = _rule(A.shape[2:], C, len(slots), [s // maxk for s in slots])
     pot = torch.empty((N, len(slots)) + tuple(A.shape[2:]), dtype=torch.float32, device=A.device)
     _lib.call('lx_potential_direct', ctypes.byref(rule), engine.ptr(A), N, engine.ptr(kern), kh, kw,

@@ -15,6 +15,73 @@ namespace {
 
 constexpr int TH = 16, TW = 64, MAXK = 64;
 
+// LDS tile row length: the output tile, the kernel's halo and one spare column (the window's
+// last shift reads it), rounded up to a multiple of 4 so that rows start 16-byte aligned
+__host__ __device__ inline int tile_cols(int kw) { return (TW + kw + 3) & ~3; }
+
+// Stage rows [y0 - qh, y0 - qh + trows) x columns [x0 - qw, x0 - qw + tcols) of an H x W
+// image, wrapped, into tile[trows][tcols]: each thread keeps one column (one modulo per thread,
+// not per element) and walks rows nsplit apart.  tcols <= 256.
+LX_DEV void stage_tile(float* tile, const float* img, int H, int W, int y0, int x0, int qh, int qw, int trows,
+                       int tcols) {
+  const int nsplit = 256 / tcols;
+  const int t = threadIdx.x;
+  if (t < nsplit * tcols) {
+    const int c = t % tcols, r0 = t / tcols;
+    int gx = (x0 + c - qw) % W;
+    gx += gx < 0 ? W : 0;
+    int gy = (y0 + r0 - qh) % H;
+    gy += gy < 0 ? H : 0;
+    // 8 rows per round: the loads are all issued before the first LDS write waits on them.
+    // The row advances by nsplit, i.e. by step = nsplit mod H < H, so one conditional
+    // subtraction keeps gy in [0, H) for any world height.
+    const int step = nsplit % H;
+    for (int rb = r0; rb < trows; rb += 8 * nsplit) {
+      float v[8];
+      int rows[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        rows[u] = rb + u * nsplit;
+        v[u] = rows[u] < trows ? img[int64_t(gy) * W + gx] : 0.f;
+        gy += step;
+        gy -= gy >= H ? H : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (rows[u] < trows) tile[rows[u] * tcols + c] = v[u];
+    }
+  }
+}
+
+// acc[e] += sum_{j < kw} k_j row[e + j], e < 4, j in increasing order (k_j = kr[j], or kr[kw-1-j]
+// when FLIP): taps in blocks of 4 against an 8-wide window read 16 bytes at a time (16 FMAs per
+// two LDS reads, no register shifts), the remaining taps one at a time.
+template <bool FLIP>
+LX_DEV void row_taps(const float* row, const float* kr, int kw, float (&acc)[4]) {
+  float4 lo = *reinterpret_cast<const float4*>(row);
+  int j = 0;
+  for (; j + 4 <= kw; j += 4) {
+    const float4 hi = *reinterpret_cast<const float4*>(row + j + 4);
+    const float w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float kv = FLIP ? kr[kw - 1 - (j + t)] : kr[j + t];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[e] = fmaf(kv, w[e + t], acc[e]);
+    }
+    lo = hi;
+  }
+  float w0 = lo.x, w1 = lo.y, w2 = lo.z, w3 = lo.w;
+  for (; j < kw; ++j) {
+    const float kv = FLIP ? kr[kw - 1 - j] : kr[j];
+    acc[0] = fmaf(kv, w0, acc[0]);
+    acc[1] = fmaf(kv, w1, acc[1]);
+    acc[2] = fmaf(kv, w2, acc[2]);
+    acc[3] = fmaf(kv, w3, acc[3]);
+    w0 = w1; w1 = w2; w2 = w3; w3 = row[j + 4];
+  }
+}
+
 struct DirectArgs {
   const float* A;
   const float* kern;
@@ -24,46 +91,26 @@ struct DirectArgs {
 };
 
 __global__ __launch_bounds__(256) void direct_kernel(DirectArgs a) {
-  extern __shared__ float tile[];  // (TH + kh - 1) rows x (TW + kw) columns
-  const int tcols = TW + a.kw;     // one spare column: the window's last shift reads it
+  extern __shared__ __align__(16) float tile[];  // (TH + kh - 1) rows x tile_cols(kw) columns
+  const int tcols = tile_cols(a.kw);
   const int trows = TH + a.kh - 1;
   const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
   const int64_t bk = blockIdx.z;
   const int64_t b = bk / a.K;
   const int k = int(bk - b * a.K);
   const float* A = a.A + (b * a.C + a.cin[k]) * int64_t(a.H) * a.W;
-  for (int idx = threadIdx.x; idx < trows * tcols; idx += 256) {
-    const int r = idx / tcols, c = idx - r * tcols;
-    int gy = (y0 + r - a.ph) % a.H;
-    gy += gy < 0 ? a.H : 0;
-    int gx = (x0 + c - a.pw) % a.W;
-    gx += gx < 0 ? a.W : 0;
-    tile[idx] = A[int64_t(gy) * a.W + gx];
-  }
+  stage_tile(tile, A, a.H, a.W, y0, x0, a.ph, a.pw, trows, tcols);
   __syncthreads();
   const int ty = threadIdx.x / (TW / 4), tx4 = (threadIdx.x % (TW / 4)) * 4;
   const float* kk = a.kern + int64_t(k) * a.kh * a.kw;
-  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
-  for (int i = 0; i < a.kh; ++i) {
-    const float* row = tile + (ty + i) * tcols + tx4;
-    float w0 = row[0], w1 = row[1], w2 = row[2], w3 = row[3];
-    const float* kr = kk + i * a.kw;
-    for (int j = 0; j < a.kw; ++j) {
-      const float kv = kr[j];
-      acc0 = fmaf(kv, w0, acc0);
-      acc1 = fmaf(kv, w1, acc1);
-      acc2 = fmaf(kv, w2, acc2);
-      acc3 = fmaf(kv, w3, acc3);
-      w0 = w1; w1 = w2; w2 = w3; w3 = row[j + 4];
-    }
-  }
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int i = 0; i < a.kh; ++i) row_taps<false>(tile + (ty + i) * tcols + tx4, kk + i * a.kw, a.kw, acc);
   const int y = y0 + ty;
   if (y < a.H) {
     float* out = a.U + (bk * a.H + y) * int64_t(a.W) + x0 + tx4;
-    const float v[4] = {acc0, acc1, acc2, acc3};
 #pragma unroll
     for (int e = 0; e < 4; ++e)
-      if (x0 + tx4 + e < a.W) out[e] = v[e];
+      if (x0 + tx4 + e < a.W) out[e] = acc[e];
   }
 }
 
@@ -71,8 +118,8 @@ __global__ __launch_bounds__(256) void direct_kernel(DirectArgs a) {
 // correlation of dU[b][k] with the flipped kernel, wrap-padded with kh - 1 - ph before (grid:
 // tiles x B * C).  The tile of each kernel's dU is staged in LDS in turn.
 __global__ __launch_bounds__(256) void direct_adj_kernel(DirectArgs a, const float* dU, float* dA) {
-  extern __shared__ float tile[];
-  const int tcols = TW + a.kw;
+  extern __shared__ __align__(16) float tile[];
+  const int tcols = tile_cols(a.kw);
   const int trows = TH + a.kh - 1;
   const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
   const int64_t bc = blockIdx.z;
@@ -80,42 +127,23 @@ __global__ __launch_bounds__(256) void direct_adj_kernel(DirectArgs a, const flo
   const int c = int(bc - b * a.C);
   const int qh = a.kh - 1 - a.ph, qw = a.kw - 1 - a.pw;
   const int ty = threadIdx.x / (TW / 4), tx4 = (threadIdx.x % (TW / 4)) * 4;
-  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
   for (int k = 0; k < a.K; ++k) {
     if (a.cin[k] != c) continue;
     const float* G = dU + (b * a.K + k) * int64_t(a.H) * a.W;
     __syncthreads();  // the previous kernel's tile is consumed
-    for (int idx = threadIdx.x; idx < trows * tcols; idx += 256) {
-      const int r = idx / tcols, cc = idx - r * tcols;
-      int gy = (y0 + r - qh) % a.H;
-      gy += gy < 0 ? a.H : 0;
-      int gx = (x0 + cc - qw) % a.W;
-      gx += gx < 0 ? a.W : 0;
-      tile[idx] = G[int64_t(gy) * a.W + gx];
-    }
+    stage_tile(tile, G, a.H, a.W, y0, x0, qh, qw, trows, tcols);
     __syncthreads();
     const float* kk = a.kern + int64_t(k) * a.kh * a.kw;
-    for (int i = 0; i < a.kh; ++i) {
-      const float* row = tile + (ty + i) * tcols + tx4;
-      float w0 = row[0], w1 = row[1], w2 = row[2], w3 = row[3];
-      const float* kr = kk + (a.kh - 1 - i) * a.kw;  // flipped rows and columns
-      for (int j = 0; j < a.kw; ++j) {
-        const float kv = kr[a.kw - 1 - j];
-        acc0 = fmaf(kv, w0, acc0);
-        acc1 = fmaf(kv, w1, acc1);
-        acc2 = fmaf(kv, w2, acc2);
-        acc3 = fmaf(kv, w3, acc3);
-        w0 = w1; w1 = w2; w2 = w3; w3 = row[j + 4];
-      }
-    }
+    for (int i = 0; i < a.kh; ++i)  // flipped rows and columns
+      row_taps<true>(tile + (ty + i) * tcols + tx4, kk + (a.kh - 1 - i) * a.kw, a.kw, acc);
   }
   const int y = y0 + ty;
   if (y < a.H) {
     float* out = dA + (bc * a.H + y) * int64_t(a.W) + x0 + tx4;
-    const float v[4] = {acc0, acc1, acc2, acc3};
 #pragma unroll
     for (int e = 0; e < 4; ++e)
-      if (x0 + tx4 + e < a.W) out[e] = v[e];
+      if (x0 + tx4 + e < a.W) out[e] = acc[e];
   }
 }
 
@@ -172,7 +200,7 @@ extern "C" int lx_potential_direct_adjoint(const lx_rule* rule, const float* sta
   const DirectArgs a = direct_args(rule, state_d, kern_d, kh, kw);
   hipStream_t st = (hipStream_t)stream;
   if (dstate_d) {
-    const size_t lds = size_t(TH + kh - 1) * size_t(TW + kw) * sizeof(float);
+    const size_t lds = size_t(TH + kh - 1) * size_t(tile_cols(kw)) * sizeof(float);
     const dim3 grid(unsigned((a.W + TW - 1) / TW), unsigned((a.H + TH - 1) / TH), unsigned(B * rule->C));
     hipLaunchKernelGGL(direct_adj_kernel, grid, dim3(256), lds, st, a, dpot_d, dstate_d);
   }
@@ -196,7 +224,7 @@ extern "C" int lx_potential_direct(const lx_rule* rule, const float* state_d, in
   if (B * rule->K > 65535) return lx_fail(LX_ENOTSUP, "lx_potential_direct: B * K > 65535");
   DirectArgs a = direct_args(rule, state_d, kern_d, kh, kw);
   a.U = potential_d;
-  const size_t lds = size_t(TH + kh - 1) * size_t(TW + kw) * sizeof(float);
+  const size_t lds = size_t(TH + kh - 1) * size_t(tile_cols(kw)) * sizeof(float);
   const dim3 grid(unsigned((a.W + TW - 1) / TW), unsigned((a.H + TH - 1) / TH), unsigned(B * rule->K));
   hipLaunchKernelGGL(direct_kernel, grid, dim3(256), lds, (hipStream_t)stream, a);
   hipError_t e = hipGetLastError();
