@@ -315,6 +315,8 @@ def main():
                     help='after timing, copy this many MiB device-to-device (a known-size kernel for PMC scaling)')
     ap.add_argument('--only-c2', action='store_true',
                     help='time the 3C6K 256^2 step-and-statistics workload only (profiling runs)')
+    ap.add_argument('--only-c4', action='store_true',
+                    help='time the 16 x 96^3 checkpointed adjoint workload only (profiling runs)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: every world batch fits one MI355X and worlds are independent '
@@ -326,6 +328,9 @@ def main():
 
     dev = torch.device('cuda', 0)
     torch.cuda.set_device(dev)
+    if args.only_c4:
+        print(json.dumps(c4_adjoint_rate(reps=1)), flush=True)
+        return
     if args.only_c2:
         print(json.dumps(c2_rate(steps=max(args.steps, 1))), flush=True)
         if args.copy_mb:
