@@ -597,27 +597,30 @@ LX_DEV float growth_rcp(int kind, float m, float s, float inv_s, float inv_9s2, 
 }
 
 // dg/dU, dg/dm, dg/ds for the differentiable growth functions (adjoint).
-LX_DEV void growth_grad(int kind, float m, float s, float U, float& dU, float& dm, float& ds) {
+// inv_s = 1/s and inv_9s2 = 1/(9 s^2) are the kernel's reciprocals (as growth_rcp takes them):
+// the Gaussians and the quartic need no division per cell.
+LX_DEV void growth_grad(int kind, float m, float s, float inv_s, float inv_9s2, float U, float& dU, float& dm,
+                        float& ds) {
   switch (kind) {
     case LX_GF_POLY_QUAD4: {
-      float d = U - m, s2 = s * s;
-      float o = 1.f - (d * d) / (9.f * s2);
+      float d = U - m;
+      float o = 1.f - (d * d) * inv_9s2;
       if (o <= 0.f) { dU = dm = ds = 0.f; return; }
       float o3 = o * o * o;
       float dgdo = 8.f * o3;
-      dU = dgdo * (-2.f * d / (9.f * s2));
+      dU = dgdo * (-2.f * d * inv_9s2);
       dm = -dU;
-      ds = dgdo * (2.f * d * d / (9.f * s2 * s));
+      ds = dgdo * (2.f * d * d * inv_9s2 * inv_s);
       return;
     }
     case LX_GF_GAUSSIAN:
     case LX_GF_GAUSSIAN_TARGET: {
-      float z = (U - m) / s;
-      float e = __expf(-(z * z) / 2.f);
+      float z = (U - m) * inv_s;
+      float e = __expf(-0.5f * (z * z));
       float amp = kind == LX_GF_GAUSSIAN ? 2.f : 1.f;
-      dU = -amp * e * z / s;
+      dU = -amp * e * z * inv_s;
       dm = -dU;
-      ds = amp * e * z * z / s;
+      ds = amp * e * z * z * inv_s;
       return;
     }
     case LX_GF_TRIANGLE: {  // 2 (U - (m - s)) / s on [m - s, m), 2 (U - (m + s)) / (-s) on [m, m + s]

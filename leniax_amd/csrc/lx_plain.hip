@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) void adj_growth_kernel(AdjArgs a) {
     // the value as the forward passes compute it (growth_rcp), so that g - G cancels exactly
     // where the field is one kernel's growth
     const float g = growth_rcp(kind, gm, gs, inv_s, inv_9s2, u);
-    growth_grad(kind, gm, gs, u, dgdu, dgdm, dgds);
+    growth_grad(kind, gm, gs, inv_s, inv_9s2, u, dgdu, dgdm, dgds);
     float dg = 0.f;
 #pragma unroll
     for (int c = 0; c < LX_MAX_C; ++c) {
@@ -470,26 +470,44 @@ __global__ __launch_bounds__(256) void adj_11_kernel(AdjArgs a) {
   const float inv_den = a.average ? 1.f / w : 1.f;
   const int64_t base = b * a.cells;
   float acc[4] = {0.f, 0.f, 0.f, 0.f};  // dm, ds, dw, ddt
-#pragma unroll 2
-  for (int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p < a.cells; p += int64_t(gridDim.x) * blockDim.x) {
-    const float A = a.A[base + p], G = a.G[base + p], U = a.U[base + p];
-    const float dz = adj_dz(a, A, G, a.lam[base + p], dtw);
-    float dG = dtw * dz;
-    if (a.dfield) dG += a.dfield[base + p];
-    float dgdu, dgdm, dgds;
-    const float g = growth_rcp(kind, gm, gs, inv_s, inv_9s2, U);  // as the forward passes
-    growth_grad(kind, gm, gs, U, dgdu, dgdm, dgds);
-    const float dg = dG * coef;
-    a.dU[base + p] = a.dpot_in ? fmaf(dg, dgdu, a.dpot_in[base + p]) : dg * dgdu;
-    acc[0] += dg * dgdm;
-    acc[1] += dg * dgds;
-    acc[2] += dG * (a.average ? (g - G) * inv_den : g);
-    if (a.state_fn == LX_STATE_V2) {
-      a.lam_out[base + p] = dz * (1.f - dtw);
-      acc[3] += dz * (G - A);
-    } else {
-      a.lam_out[base + p] = dz;
-      acc[3] += dz * G;
+  // NB cells per round, all loaded before any store: the stores may alias the loads (lambda_out
+  // is lambda) only at the same cell, so the next cells' loads need not wait for them.  Cells
+  // are taken in the same order as one at a time (the sums round the same way).
+  constexpr int NB = 4;
+  const int64_t S = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t p0 = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; p0 < a.cells; p0 += NB * S) {
+    float vA[NB], vG[NB], vU[NB], vL[NB], vF[NB], vP[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int64_t p = p0 + u * S;
+      const int64_t q = base + (p < a.cells ? p : p0);  // an in-bounds cell for the tail
+      vA[u] = a.A[q]; vG[u] = a.G[q]; vU[u] = a.U[q]; vL[u] = a.lam[q];
+      vF[u] = a.dfield ? a.dfield[q] : 0.f;
+      vP[u] = a.dpot_in ? a.dpot_in[q] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int64_t p = p0 + u * S;
+      if (p >= a.cells) break;
+      const float A = vA[u], G = vG[u], U = vU[u];
+      const float dz = adj_dz(a, A, G, vL[u], dtw);
+      float dG = dtw * dz;
+      if (a.dfield) dG += vF[u];
+      float dgdu, dgdm, dgds;
+      const float g = growth_rcp(kind, gm, gs, inv_s, inv_9s2, U);  // as the forward passes
+      growth_grad(kind, gm, gs, inv_s, inv_9s2, U, dgdu, dgdm, dgds);
+      const float dg = dG * coef;
+      a.dU[base + p] = a.dpot_in ? fmaf(dg, dgdu, vP[u]) : dg * dgdu;
+      acc[0] += dg * dgdm;
+      acc[1] += dg * dgds;
+      acc[2] += dG * (a.average ? (g - G) * inv_den : g);
+      if (a.state_fn == LX_STATE_V2) {
+        a.lam_out[base + p] = dz * (1.f - dtw);
+        acc[3] += dz * (G - A);
+      } else {
+        a.lam_out[base + p] = dz;
+        acc[3] += dz * G;
+      }
     }
   }
   float* const dst[4] = {a.g_gf ? a.g_gf + b * a.gf_bs : nullptr, a.g_gf ? a.g_gf + b * a.gf_bs + 1 : nullptr,
