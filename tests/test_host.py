@@ -99,6 +99,7 @@ def test_empty_batches_are_accepted_with_null_pointers():
         'lx_state': lambda B: L_.lx_state(r, N, N, B, N, 0, N, N),
         'lx_carry_init': lambda B: L_.lx_carry_init(B, N, N),
         'lx_stats': lambda B: L_.lx_stats(r, N, N, N, B, 13., 10., N, N, N),
+        'lx_step_stats': lambda B: L_.lx_step_stats(r, N, B, N, 0, N, 0, N, 0, N, 0, N, N, N, 13., 10., N, N, N, N),
         'lx_check_heuristics': lambda B: L_.lx_check_heuristics(N, 5, B, 1, N, N, N),
         'lx_rfft2': lambda B: L_.lx_rfft2(2, dims, B, N, N, N),
         'lx_irfft2': lambda B: L_.lx_irfft2(2, dims, B, N, N, N, N),
@@ -116,6 +117,15 @@ def test_empty_batches_are_accepted_with_null_pointers():
     for name, fn in calls.items():
         assert fn(0) == _lib.LX_OK, (name, L_.lx_last_error())
         assert fn(-1) != _lib.LX_OK, name
+    # every declared entry point that takes a batch is exercised above
+    batch_free = {'lx_last_error', 'lx_version', 'lx_prepare_spectra', 'lx_workspace_size', 'lx_carry_bytes',
+                  'lx_run_scratch_bytes', 'lx_run_supported', 'lx_run'}
+    assert sorted(set(_declared()) - batch_free - set(calls)) == []
+    assert L_.lx_step_stats(r, N, 0, N, 0, N, 0, N, 0, N, 0, N, N, N, 0., 10., N, N, N, N) != _lib.LX_OK  # R <= 0
+    # the workspace holds the two spectra sets and the row-pair records of lx_step_stats
+    nb = ctypes.c_size_t(0)
+    _lib.call('lx_workspace_size', r, 3, ctypes.byref(nb))
+    assert nb.value == 3 * 2 * 64 * 33 * 8 + 3 * 1 * 32 * 16 * 4
     a = _lib.lx_run_args()
     a.B = 0
     a.max_steps = 10
