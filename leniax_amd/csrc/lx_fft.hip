@@ -25,7 +25,9 @@ namespace {
 
 template <int L>
 struct Lanes {
-  static constexpr int TPR = L <= 96 ? 4 : (L <= 192 ? 8 : 16);  // lanes per transform
+  // lanes per transform: 8 from 96 points up (96 = 8 lanes x 12 points through the DPP network
+  // rather than 4 x 24 through LDS: half the registers, 3 waves per SIMD instead of 2)
+  static constexpr int TPR = L <= 64 ? 4 : (L <= 192 ? 8 : 16);
   static constexpr int P = L / TPR;                                // points per lane
   static constexpr bool XL = P % TPR == 0;                         // cross-lane step through LDS
   static constexpr int XRS = xch_row(P);                           // group scratch row stride
