@@ -467,7 +467,6 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
 #pragma unroll
       for (int m = 0; m < P; ++m) {
         const float U0 = x[m].x * inv_n, U1 = x[m].y * inv_n;
-        if (STATS && own) nU += (U0 > LX_EPS ? 1.f : 0.f) + (U1 > LX_EPS ? 1.f : 0.f);
         if (want_pot && live) {
           po0[TPR * m + t] = U0;
           po0[W + TPR * m + t] = U1;
@@ -483,6 +482,15 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
         }
       }
     };
+    // the count has a loop of its own: folded into the growth loop below, it keeps the kernel
+    // at 156 VGPRs (3 waves per SIMD) instead of 128 for rows of 64 and 128 points
+    if constexpr (STATS) {
+      if (own) {
+#pragma unroll
+        for (int m = 0; m < P; ++m)
+          nU += (x[m].x * inv_n > LX_EPS ? 1.f : 0.f) + (x[m].y * inv_n > LX_EPS ? 1.f : 0.f);
+      }
+    }
     if (a.sel) {
       accumulate(IntC<-1>{});
     } else {
