@@ -23,11 +23,19 @@
 
 namespace {
 
-template <int L>
+// lanes per transform of the row passes: 8 from 96 points up (96 = 8 lanes x 12 points through
+// the DPP network rather than 4 x 24 through LDS: half the registers of rows_inv, 3 waves per
+// SIMD instead of 2)
+constexpr int row_tpr(int L) { return L <= 64 ? 4 : (L <= 192 ? 8 : 16); }
+// lanes per transform of the line passes: as the rows, except the plain forward and inverse
+// passes (modes 1 and 2) at 96 points, which take 4 x 24 through LDS: they stay at 4 waves per
+// SIMD there (101 and 112 VGPRs) and run 10% faster than on the network (the convolution pass,
+// at 173 VGPRs with 4 lanes, is as fast on 8)
+constexpr int line_tpr(int L, int mode) { return L == 96 && mode != 0 ? 4 : row_tpr(L); }
+
+template <int L, int T = row_tpr(L)>
 struct Lanes {
-  // lanes per transform: 8 from 96 points up (96 = 8 lanes x 12 points through the DPP network
-  // rather than 4 x 24 through LDS: half the registers, 3 waves per SIMD instead of 2)
-  static constexpr int TPR = L <= 64 ? 4 : (L <= 192 ? 8 : 16);
+  static constexpr int TPR = T;
   static constexpr int P = L / TPR;                                // points per lane
   static constexpr bool XL = P % TPR == 0;                         // cross-lane step through LDS
   static constexpr int XRS = xch_row(P);                           // group scratch row stride
@@ -68,21 +76,27 @@ int dispatch_len(int L, F&& f) {
 // network, and the lane rows of fft_xlds (row t, entry k = exp(-2 pi i t k / L))
 template <int L>
 __device__ float2 g_tw[L];
-template <int L>
-__device__ __align__(16) float2 g_twl[Lanes<L>::TPR * twl_row(Lanes<L>::P)];
+template <int L, int T>
+__device__ __align__(16) float2 g_twl[T * twl_row(L / T)];
+
+template <int L, int T>
+LX_DEV void init_twl() {
+  constexpr int P = L / T;
+  for (int j = threadIdx.x; j < T * twl_row(P); j += blockDim.x) {
+    const int t = j / twl_row(P), k = j % twl_row(P);
+    g_twl<L, T>[j] = k < P ? twl_entry(L, t, k) : make_float2(0.f, 0.f);
+  }
+}
 
 template <int L>
 __global__ void init_tw_kernel() {
-  using LC = Lanes<L>;
   for (int j = threadIdx.x; j < L; j += blockDim.x) {
     double sn, cs;
     sincospi(2.0 * j / L, &sn, &cs);
     g_tw<L>[j] = make_float2((float)cs, (float)-sn);
   }
-  for (int j = threadIdx.x; j < LC::TPR * twl_row(LC::P); j += blockDim.x) {
-    const int t = j / twl_row(LC::P), k = j % twl_row(LC::P);
-    g_twl<L>[j] = k < LC::P ? twl_entry(L, t, k) : make_float2(0.f, 0.f);
-  }
+  init_twl<L, row_tpr(L)>();
+  if constexpr (line_tpr(L, 1) != row_tpr(L)) init_twl<L, line_tpr(L, 1)>();
 }
 
 constexpr int kMaxDevices = 64;
@@ -111,9 +125,9 @@ unsigned blocks_for(int64_t threads) { return unsigned((threads + 255) / 256); }
 
 // Frequency held by register e of lane t after a forward group transform: with the LDS
 // exchange e = j TPR + k2 holds (J t + j) + P k2; with the DPP network e holds e + P bitrev(t).
-template <int L>
+template <int L, int T = row_tpr(L)>
 LX_DEV int freq_of(int t, int e) {
-  using LC = Lanes<L>;
+  using LC = Lanes<L, T>;
   if constexpr (LC::XL) {
     constexpr int J = LC::P / LC::TPR;
     return J * t + e / LC::TPR + LC::P * (e % LC::TPR);
@@ -124,10 +138,10 @@ LX_DEV int freq_of(int t, int e) {
 
 // Length-L transform over a lane group: natural order (data[TPR m + t] in x[m]) <-> the
 // frequency layout of freq_of; xch is the group's LDS scratch.
-template <int L, bool INV>
+template <int L, bool INV, int T = row_tpr(L)>
 LX_DEV void fft_group(float2* x, int t, int lane, float2* xch) {
-  using LC = Lanes<L>;
-  if constexpr (LC::XL) fft_xlds<LC::P, LC::TPR, INV>(x, t, g_twl<L> + t * twl_row(LC::P), xch);
+  using LC = Lanes<L, T>;
+  if constexpr (LC::XL) fft_xlds<LC::P, LC::TPR, INV>(x, t, g_twl<L, T> + t * twl_row(LC::P), xch);
   else fft_lanes<LC::P, LC::TPR, INV>(x, t, lane, g_tw<L>);
 }
 
@@ -256,7 +270,7 @@ LineArgs line_args() {
 // of 8 (sched_barrier) so that addresses and values are not all held at once.
 template <int L, int MODE>
 __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
-  using LC = Lanes<L>;
+  using LC = Lanes<L, line_tpr(L, MODE)>;
   constexpr int TPR = LC::TPR, P = LC::P;
   __shared__ __align__(16) float2 xch_all[LC::XL ? LC::XCH : 2];
   const int lane = threadIdx.x & 63;
@@ -277,10 +291,10 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
   if constexpr (MODE == 2) {  // inverse only; input in frequency order
 #pragma unroll
     for (int e = 0; e < P; ++e) {
-      y[e] = src[freq_of<L>(t, e) * st];
+      y[e] = src[freq_of<L, TPR>(t, e) * st];
       if ((e & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     }
-    fft_group<L, true>(y, t, lane, xch);
+    fft_group<L, true, TPR>(y, t, lane, xch);
     float2* dst = a.U + img * a.img_elems + off;
     if (live) {
 #pragma unroll
@@ -293,19 +307,19 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
     x[m] = src[(TPR * m + t) * st];
     if ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
   }
-  fft_group<L, false>(x, t, lane, xch);
+  fft_group<L, false, TPR>(x, t, lane, xch);
   if constexpr (MODE == 1) {  // forward only, in place
     float2* dst = const_cast<float2*>(src);
     if (live) {
 #pragma unroll
-      for (int e = 0; e < P; ++e) dst[freq_of<L>(t, e) * st] = x[e];
+      for (int e = 0; e < P; ++e) dst[freq_of<L, TPR>(t, e) * st] = x[e];
     }
     return;
   }
   if (a.FA && live) {
     float2* fa = a.FA + img * a.img_elems + off;
 #pragma unroll
-    for (int e = 0; e < P; ++e) fa[freq_of<L>(t, e) * st] = x[e];
+    for (int e = 0; e < P; ++e) fa[freq_of<L, TPR>(t, e) * st] = x[e];
   }
   const float csign = a.conjk ? -1.f : 1.f;  // multiply by K or by conj(K)
   for (int k = 0; k < a.K; ++k) {
@@ -319,7 +333,7 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
       float2* ds = a.dspec + (b * a.K + k) * a.img_elems + off;
 #pragma unroll
       for (int e = 0; e < P; ++e) {
-        const int o = freq_of<L>(t, e) * st;
+        const int o = freq_of<L, TPR>(t, e) * st;
         ds[o] = cadd(ds[o], cmulc(x[e], fa[o]));
         if ((e & 7) == 7) __builtin_amdgcn_sched_barrier(0);
       }
@@ -327,11 +341,11 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
     const float2* kk = a.Kh + b * a.k_bstride + int64_t(k) * a.img_elems + off;
 #pragma unroll
     for (int e = 0; e < P; ++e) {
-      const float2 kv = kk[freq_of<L>(t, e) * st];
+      const float2 kv = kk[freq_of<L, TPR>(t, e) * st];
       y[e] = cmul(x[e], make_float2(kv.x, csign * kv.y));
       if ((e & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     }
-    fft_group<L, true>(y, t, lane, xch);
+    fft_group<L, true, TPR>(y, t, lane, xch);
     float2* dst = a.U + (b * a.K + k) * a.img_elems + off;
     if (live) {
 #pragma unroll
@@ -615,7 +629,8 @@ int launch_lines(const Geom& g, int axis, LineArgs la, int64_t nimg, hipStream_t
   return dispatch_len(g.L[axis], [&](auto l) {
     constexpr int L = decltype(l)::value;
     if (int rc = ensure_tw<L>(st)) return rc;
-    const dim3 grid(blocks_for(la.nlines * Lanes<L>::TPR));
+    const int tpr = line_tpr(L, la.mode);
+    const dim3 grid(blocks_for(la.nlines * tpr));
     if (la.mode == 0) hipLaunchKernelGGL((lines_kernel<L, 0>), grid, dim3(256), 0, st, la);
     else if (la.mode == 1) hipLaunchKernelGGL((lines_kernel<L, 1>), grid, dim3(256), 0, st, la);
     else hipLaunchKernelGGL((lines_kernel<L, 2>), grid, dim3(256), 0, st, la);
