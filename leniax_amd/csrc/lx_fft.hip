@@ -603,7 +603,9 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
     for (int off2 = TPR / 2; off2 >= 1; off2 >>= 1)
 #pragma unroll
       for (int i = 0; i < 14; ++i) rec[i] += __shfl_xor(rec[i], off2, 64);
-    float* dst = a.part + pr * LX_PAIR_REC;
+    // the pair's index rebuilt from (img, r2), which stay live for the update, rather than pr
+    // kept live over the kernel loop (two registers more at the loop's peak)
+    float* dst = a.part + (img * half_rows + (r2 >> 1)) * LX_PAIR_REC;
 #pragma unroll
     for (int e = 0; e < LX_PAIR_REC; ++e)
       if (e % TPR == t) dst[e] = rec[e];
