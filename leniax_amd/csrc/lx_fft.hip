@@ -354,6 +354,105 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
   }
 }
 
+// The convolution pass (mode 0 without FA or dspec) with its global reads and writes staged
+// through LDS.  A block holds LB = 256 / TPR lines; neighbouring lines are neighbouring half-
+// spectrum columns, so a load or store instruction of the staging loop covers LB columns of
+// 256 / LB points (16 x 8 B = 128 B per point at 16 lanes per line) where the group layout covers
+// only 64 / TPR columns, which left the pass bound by its address unit.  Thread i moves line
+// i % LB, points i / LB + TPR q; group g reads line g from the stage in its transform layout.
+// The stage (LB rows of L + 1 points) shares the exchange scratch, so block barriers separate
+// the staging steps from the transforms.
+// Used at 256 points (16 lanes per line: 3C6K step of 256 worlds of 256^2 0.745 -> 0.695 ms);
+// at 96 points (8 lanes per line, 8 columns per instruction already) the stage's registers
+// cost a wave per SIMD and the 96^3 adjoint ran 1.5% slower, so other lengths keep lines_kernel.
+constexpr bool conv_staged(int L) { return L == 256; }
+
+template <int L>
+__global__ __launch_bounds__(256) void conv_staged_kernel(LineArgs a) {
+  using LC = Lanes<L, line_tpr(L, 0)>;
+  constexpr int TPR = LC::TPR, P = LC::P, LB = 256 / TPR, SRS = L + 1;
+  constexpr int NX = LC::XL ? LC::XCH : 0, NS = LB * SRS;
+  __shared__ __align__(16) float2 xch_all[NX > NS ? NX : NS];
+  __shared__ unsigned chan_mask;
+  const int tid = threadIdx.x, lane = tid & 63, t = tid % TPR, g = tid / TPR;
+  float2* xch = xch_all + (LC::XL ? g * (TPR * LC::XRS) : 0);
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t per_img = int64_t(a.A) * a.WH;
+  const int st = int(a.st);
+  // the line this thread stages (ls) and its first point (pl)
+  const int ls = tid % LB, pl = tid / LB;
+  const int64_t lid_s = blk * LB + ls;
+  const bool live_s = lid_s < a.nlines;
+  const int64_t img_s = live_s ? lid_s / per_img : 0;
+  const int rem_s = live_s ? int(lid_s - img_s * per_img) : 0;
+  const int aa_s = rem_s / a.WH;
+  const int off_s = aa_s * int(a.a_stride) + (rem_s - aa_s * a.WH);
+  const int64_t b_s = img_s / a.C;
+  const int c_s = int(img_s - b_s * a.C);
+  // the line this thread's group transforms
+  const int64_t lid = blk * LB + g;
+  const bool live = lid < a.nlines;
+  const int64_t img = live ? lid / per_img : 0;
+  const int c = int(img - (img / a.C) * a.C);
+  float2* stage = xch_all;
+
+  if (tid == 0) chan_mask = 0u;
+  {
+    const float2* src = a.S + img_s * a.img_elems + off_s;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const int pt = pl + TPR * q;
+      stage[ls * SRS + pt] = live_s ? src[pt * st] : make_float2(0.f, 0.f);
+      if ((q & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  __syncthreads();
+  if (live_s) atomicOr(&chan_mask, 1u << c_s);
+  float2 x[P], y[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) x[m] = stage[g * SRS + TPR * m + t];
+  __syncthreads();
+  const unsigned mask = chan_mask;  // channels of the block's lines (every OR precedes the barrier)
+  fft_group<L, false, TPR>(x, t, lane, xch);
+  const float csign = a.conjk ? -1.f : 1.f;  // multiply by K or by conj(K)
+  for (int k = 0; k < a.K; ++k) {
+    const int ck = a.cin[k];
+    if (!((mask >> ck) & 1u)) continue;  // no line of the block reads kernel k (block-uniform)
+    const bool use_s = live_s && c_s == ck;
+    __syncthreads();  // the previous transform or store step is done with the scratch
+    {
+      const float2* kk = a.Kh + b_s * a.k_bstride + int64_t(k) * a.img_elems + off_s;
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        const int pt = pl + TPR * q;
+        if (use_s) stage[ls * SRS + pt] = kk[pt * st];
+        if ((q & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    __syncthreads();
+    const bool use = live && c == ck;
+#pragma unroll
+    for (int e = 0; e < P; ++e) {
+      const float2 kv = use ? stage[g * SRS + freq_of<L, TPR>(t, e)] : make_float2(0.f, 0.f);
+      y[e] = cmul(x[e], make_float2(kv.x, csign * kv.y));
+    }
+    __syncthreads();
+    fft_group<L, true, TPR>(y, t, lane, xch);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < P; ++m) stage[g * SRS + TPR * m + t] = y[m];
+    __syncthreads();
+    if (use_s) {
+      float2* dst = a.U + (b_s * a.K + k) * a.img_elems + off_s;
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        const int pt = pl + TPR * q;
+        dst[pt * st] = stage[ls * SRS + pt];
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- rows inverse + field + update
 struct RowArgs {
   const float2* U;
@@ -633,8 +732,16 @@ int launch_lines(const Geom& g, int axis, LineArgs la, int64_t nimg, hipStream_t
     if (int rc = ensure_tw<L>(st)) return rc;
     const int tpr = line_tpr(L, la.mode);
     const dim3 grid(blocks_for(la.nlines * tpr));
-    if (la.mode == 0) hipLaunchKernelGGL((lines_kernel<L, 0>), grid, dim3(256), 0, st, la);
-    else if (la.mode == 1) hipLaunchKernelGGL((lines_kernel<L, 1>), grid, dim3(256), 0, st, la);
+    if (la.mode == 0) {
+      bool staged = false;
+      if constexpr (conv_staged(L)) {
+        if (!la.FA && !la.dspec) {
+          hipLaunchKernelGGL(conv_staged_kernel<L>, grid, dim3(256), 0, st, la);
+          staged = true;
+        }
+      }
+      if (!staged) hipLaunchKernelGGL((lines_kernel<L, 0>), grid, dim3(256), 0, st, la);
+    } else if (la.mode == 1) hipLaunchKernelGGL((lines_kernel<L, 1>), grid, dim3(256), 0, st, la);
     else hipLaunchKernelGGL((lines_kernel<L, 2>), grid, dim3(256), 0, st, la);
     return launched("lines");
   });

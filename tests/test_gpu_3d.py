@@ -43,7 +43,8 @@ def _sphere_K(dims, R, b):
     return np.fft.fftn(np.fft.fftshift(kp))[None, None, None].astype(np.complex64)
 
 
-@pytest.mark.parametrize('dims,side,R,steps', [((32, 48, 32), 12, 5, 6), ((96, 96, 96), 24, 10, 4)])
+@pytest.mark.parametrize('dims,side,R,steps', [((32, 48, 32), 12, 5, 6), ((96, 96, 96), 24, 10, 4),
+                                               ((256, 16, 32), 6, 3, 3)])
 def test_3d_steps_match_oracle(dims, side, R, steps):
     from leniax_amd import core
     from leniax_amd.growth_functions import register as gfr

@@ -80,16 +80,22 @@ def test_potential_constant_field_known_answer():
     np.testing.assert_allclose(U[0, 2], 0.54, rtol=1e-6)
 
 
-@pytest.mark.parametrize('shape', [(2, 64, 64), (1, 128, 32), (1, 256, 256), (2, 512, 64)])
+# (3, 256, 256) and (3, 256, 48): 256-point convolution lines staged through LDS, with blocks of
+# 16 lines straddling channel images (129 and 25 lines per image) and a channel no kernel reads
+@pytest.mark.parametrize('shape', [(2, 64, 64), (1, 128, 32), (1, 256, 256), (3, 256, 256), (3, 256, 48),
+                                   (2, 512, 64)])
 def test_potential_field_state_match_oracle(shape):
     from leniax_amd import core
     C, H, W = shape
     rng = np.random.default_rng(H + W)
+    # output channels of kernels 1 and 2: with 3 channels every channel receives a kernel (a
+    # weighted mean over no kernel is 0/0) while channel 1 feeds none
+    c1, c2 = (1, 2) if C >= 3 else (C - 1, 0)
     kp = [{'k_slug': 'circle_2d', 'k_params': [r, b], 'kf_slug': 'gauss_bump', 'kf_params': [1.],
            'gf_slug': g, 'gf_params': [m, s], 'h': h, 'c_in': ci, 'c_out': co}
           for (r, b, g, m, s, h, ci, co) in [(1., [1.], 'gaussian', 0.15, 0.015, 1., 0, 0),
-                                             (0.7, [1., 0.5], 'poly_quad4', 0.25, 0.03, 0.5, 0, C - 1),
-                                             (0.9, [0.3, 1.], 'gaussian', 0.3, 0.05, 0.7, C - 1, 0)]]
+                                             (0.7, [1., 0.5], 'poly_quad4', 0.25, 0.03, 0.5, 0, c1),
+                                             (0.9, [0.3, 1.], 'gaussian', 0.3, 0.05, 0.7, C - 1, c2)]]
     R = min(H, W) / 6.
     Ko, mp = L.get_kernels_and_mapping([dict(k) for k in kp], [H, W], C, R, np.float32)
     A = (rng.random((2, C, H, W), dtype=np.float32) * (rng.random((2, C, H, W)) > 0.5)).astype(np.float32)
