@@ -366,6 +366,10 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
 // at 96 points (8 lanes per line, 8 columns per instruction already) the stage's registers
 // cost a wave per SIMD and the 96^3 adjoint ran 1.5% slower, so other lengths keep lines_kernel.
 constexpr bool conv_staged(int L) { return L == 256; }
+// ... and only for grids of at least 4 blocks per CU: a small grid is latency-bound, and the
+// stage's block barriers serialise each block's loads (one 256^2 world, 9 blocks: 64 forward
+// steps 1.63 -> 2.05 ms staged)
+constexpr unsigned kStagedMinBlocks = 1024;
 
 template <int L>
 __global__ __launch_bounds__(256) void conv_staged_kernel(LineArgs a) {
@@ -735,7 +739,7 @@ int launch_lines(const Geom& g, int axis, LineArgs la, int64_t nimg, hipStream_t
     if (la.mode == 0) {
       bool staged = false;
       if constexpr (conv_staged(L)) {
-        if (!la.FA && !la.dspec) {
+        if (!la.FA && !la.dspec && grid.x >= kStagedMinBlocks) {
           hipLaunchKernelGGL(conv_staged_kernel<L>, grid, dim3(256), 0, st, la);
           staged = true;
         }
