@@ -360,8 +360,9 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
 // 256 / LB points (16 x 8 B = 128 B per point at 16 lanes per line) where the group layout covers
 // only 64 / TPR columns, which left the pass bound by its address unit.  Thread i moves line
 // i % LB, points i / LB + TPR q; group g reads line g from the stage in its transform layout.
-// The stage (LB rows of L + 1 points) shares the exchange scratch, so block barriers separate
-// the staging steps from the transforms.
+// Stage row g lies inside group g's exchange scratch (offset 2g points, which rotates the rows'
+// banks), so a group's transforms touch only its own row: block barriers separate the staging
+// steps (rows written or read by other groups' threads), wave-scope fences the rest.
 // Used at 256 points (16 lanes per line: 3C6K step of 256 worlds of 256^2 0.745 -> 0.695 ms);
 // at 96 points (8 lanes per line, 8 columns per instruction already) the stage's registers
 // cost a wave per SIMD and the 96^3 adjoint ran 1.5% slower, so other lengths keep lines_kernel.
@@ -374,12 +375,13 @@ constexpr unsigned kStagedMinBlocks = 1024;
 template <int L>
 __global__ __launch_bounds__(256) void conv_staged_kernel(LineArgs a) {
   using LC = Lanes<L, line_tpr(L, 0)>;
-  constexpr int TPR = LC::TPR, P = LC::P, LB = 256 / TPR, SRS = L + 1;
-  constexpr int NX = LC::XL ? LC::XCH : 0, NS = LB * SRS;
-  __shared__ __align__(16) float2 xch_all[NX > NS ? NX : NS];
+  constexpr int TPR = LC::TPR, P = LC::P, LB = 256 / TPR, GRS = TPR * LC::XRS;
+  static_assert(LC::XL && L + 2 * (LB - 1) <= GRS, "stage rows fit their groups' exchange scratch");
+  __shared__ __align__(16) float2 xch_all[LC::XCH];
   __shared__ unsigned chan_mask;
   const int tid = threadIdx.x, lane = tid & 63, t = tid % TPR, g = tid / TPR;
-  float2* xch = xch_all + (LC::XL ? g * (TPR * LC::XRS) : 0);
+  float2* xch = xch_all + g * GRS;
+  auto srow = [](int r) { return r * (GRS + 2); };  // stage row r: r GRS + 2 r
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
   const int64_t per_img = int64_t(a.A) * a.WH;
   const int st = int(a.st);
@@ -406,7 +408,7 @@ __global__ __launch_bounds__(256) void conv_staged_kernel(LineArgs a) {
 #pragma unroll
     for (int q = 0; q < P; ++q) {
       const int pt = pl + TPR * q;
-      stage[ls * SRS + pt] = live_s ? src[pt * st] : make_float2(0.f, 0.f);
+      stage[srow(ls) + pt] = live_s ? src[pt * st] : make_float2(0.f, 0.f);
       if ((q & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -414,7 +416,7 @@ __global__ __launch_bounds__(256) void conv_staged_kernel(LineArgs a) {
   if (live_s) atomicOr(&chan_mask, 1u << c_s);
   float2 x[P], y[P];
 #pragma unroll
-  for (int m = 0; m < P; ++m) x[m] = stage[g * SRS + TPR * m + t];
+  for (int m = 0; m < P; ++m) x[m] = stage[srow(g) + TPR * m + t];
   __syncthreads();
   const unsigned mask = chan_mask;  // channels of the block's lines (every OR precedes the barrier)
   fft_group<L, false, TPR>(x, t, lane, xch);
@@ -429,7 +431,7 @@ __global__ __launch_bounds__(256) void conv_staged_kernel(LineArgs a) {
 #pragma unroll
       for (int q = 0; q < P; ++q) {
         const int pt = pl + TPR * q;
-        if (use_s) stage[ls * SRS + pt] = kk[pt * st];
+        if (use_s) stage[srow(ls) + pt] = kk[pt * st];
         if ((q & 7) == 7) __builtin_amdgcn_sched_barrier(0);
       }
     }
@@ -437,21 +439,21 @@ __global__ __launch_bounds__(256) void conv_staged_kernel(LineArgs a) {
     const bool use = live && c == ck;
 #pragma unroll
     for (int e = 0; e < P; ++e) {
-      const float2 kv = use ? stage[g * SRS + freq_of<L, TPR>(t, e)] : make_float2(0.f, 0.f);
+      const float2 kv = use ? stage[srow(g) + freq_of<L, TPR>(t, e)] : make_float2(0.f, 0.f);
       y[e] = cmul(x[e], make_float2(kv.x, csign * kv.y));
     }
-    __syncthreads();
+    wave_lds_fence();  // the group's own row: read before its transform overwrites it
     fft_group<L, true, TPR>(y, t, lane, xch);
-    __syncthreads();
+    wave_lds_fence();
 #pragma unroll
-    for (int m = 0; m < P; ++m) stage[g * SRS + TPR * m + t] = y[m];
+    for (int m = 0; m < P; ++m) stage[srow(g) + TPR * m + t] = y[m];
     __syncthreads();
     if (use_s) {
       float2* dst = a.U + (b_s * a.K + k) * a.img_elems + off_s;
 #pragma unroll
       for (int q = 0; q < P; ++q) {
         const int pt = pl + TPR * q;
-        dst[pt * st] = stage[ls * SRS + pt];
+        dst[pt * st] = stage[srow(ls) + pt];
       }
     }
   }
