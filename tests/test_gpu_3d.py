@@ -43,13 +43,14 @@ def _sphere_K(dims, R, b):
     return np.fft.fftn(np.fft.fftshift(kp))[None, None, None].astype(np.complex64)
 
 
-@pytest.mark.parametrize('dims,side,R,steps', [((32, 48, 32), 12, 5, 6), ((96, 96, 96), 24, 10, 4),
-                                               ((256, 16, 32), 6, 3, 3)])
-def test_3d_steps_match_oracle(dims, side, R, steps):
+# (256, 16, 32) x 61 worlds: 16592 lines of 256 points along D, past the 1024 blocks of 16 lines
+# from which the convolution pass is staged through LDS; x 2 worlds: the group-layout pass
+@pytest.mark.parametrize('dims,side,R,steps,N', [((32, 48, 32), 12, 5, 6, 2), ((96, 96, 96), 24, 10, 4, 2),
+                                                 ((256, 16, 32), 6, 3, 3, 2), ((256, 16, 32), 6, 3, 2, 61)])
+def test_3d_steps_match_oracle(dims, side, R, steps, N):
     from leniax_amd import core
     from leniax_amd.growth_functions import register as gfr
-    print('3-D', dims, flush=True)
-    N = 2
+    print('3-D', dims, N, flush=True)
     A = _cube_world(0, N, dims, side)
     K = _sphere_K(dims, R, [1.0, 0.5])
     gf = np.array([[0.2, 0.03]], np.float32)
