@@ -112,6 +112,26 @@ def test_step_adjoint_matches_oracle(name, state_fn, average, ste):
         assert v < 1e-4, (k, v)
 
 
+def test_potential_adjoint_staged_in_place():
+    """Adjoint of the potential without a kernel gradient, 64 worlds of 256x256 and two kernels:
+    128 images of 129 lines of 256 points, past the 1024 blocks of 16 lines from which the
+    convolution pass is staged through LDS, here in place (each block reads its lines whole
+    before it writes them back).  Against the float64 oracle."""
+    from leniax_amd import autograd as AG
+    N, C, H, W = 64, 1, 256, 256
+    A = _world(30, N, C, H, W, 64)
+    K = _asym_K(31, C, 2, H, W, R=9)
+    tc = (0, 1)
+    Rg = np.random.default_rng(32).normal(size=(N, 2, H, W)).astype(np.float32)
+    tA = _dev(A).requires_grad_()
+    pot = AG.get_potential_fft(tA, _dev(K), tc)
+    (pot * _dev(Rg)).sum().backward()
+    dA_ref, _ = ADJ.potential_vjp(A.astype(np.float64), K.astype(np.complex128), Rg.astype(np.float64), tc)
+    err = _rel(_np(tA.grad), dA_ref)
+    print('potential adjoint, 64 x 256^2, rel err', err)
+    assert err < 2e-5
+
+
 def test_update_with_field_and_potential_cotangents():
     """Losses on the field and the potential outputs (keep_intermediary_data pipelines)."""
     from leniax_amd import autograd as AG
