@@ -256,8 +256,9 @@ def _rule_case(case):
     """(rule, K_half [S, K, ...], gf [B, K, 2], w [B, C, K], cells [B, C, *dims]) of a test case."""
     from leniax_amd import engine, _lib
     rng = np.random.default_rng(11)
-    if case == '3c6k_64':
-        B, dims, C = 3, [64, 64], 3
+    if case.startswith('3c6k'):
+        # 3c6k_256: 43 worlds, 16641 lines of 256 points (the staged convolution pass)
+        B, dims, C = (3, [64, 64], 3) if case == '3c6k_64' else (43, [256, 256], 3)
         cin = [0, 0, 1, 1, 2, 2]
         Kh_l, gfs, ws = [], [], []
         for b in range(B):
@@ -301,7 +302,7 @@ def _rule_case(case):
     return rule, Kh.contiguous(), gf.contiguous(), w.contiguous(), torch.as_tensor(cells).cuda()
 
 
-@pytest.mark.parametrize('case', ['3c6k_64', '1c_48x96', '3d_32x48x32'])
+@pytest.mark.parametrize('case', ['3c6k_64', '3c6k_256', '1c_48x96', '3d_32x48x32'])
 def test_step_stats_equals_update_then_stats(case):
     """lx_step_stats (statistics summed in the inverse row pass) against lx_update + lx_stats over
     8 steps: states, fields and potentials bit for bit, statistics to the order of their sums,
@@ -356,7 +357,10 @@ def test_step_stats_equals_update_then_stats(case):
             if mode == 'one_pass':
                 assert torch.equal(got_outs[t][1], ref_outs[t][1]) and torch.equal(got_outs[t][2], ref_outs[t][2])
         # the statistics' own bound against the oracle (test_compute_stats_matches_oracle): the speeds
-        # and the angle difference centroids, which amplifies the order of the sums
+        # and the angle difference centroids, which amplifies the order of the sums.  The angle speed
+        # (the angle of the centroid's small per-step displacement, / dt) is the worst conditioned:
+        # 1.9e-3 at 43 worlds of 256^2 (3c6k_256), 6e-4 at 64^2; both paths sum in a fixed order,
+        # so the figures repeat from run to run
         err = np.abs(got_rows - ref_rows) / np.maximum(1., np.abs(ref_rows))
         print(mode, 'max stat diff', float(err.max()), 'per stat', np.round(err.max(axis=(0, 1)), 7).tolist())
         assert float(err.max()) < 2e-3
