@@ -562,7 +562,13 @@ extern "C" int lx_adjoint_pointwise(const lx_rule* rule, int64_t B, const float*
     a.gfk[k] = k < rule->K ? rule->gf[k] : 0;
   }
   a.state_fn = rule->state_fn; a.average = rule->average; a.ste = ste != 0;
-  const unsigned per_img = unsigned(std::min<int64_t>((a.cells + 1023) / 1024, 256));
+  // blocks per image: at most 256 and about 1024 blocks over the whole grid.  Every block ends
+  // with float atomics on its world's few gradient scalars, which lie in one L2 channel and are
+  // served one at a time: 256 blocks for each of 16 worlds of 96^3 (16384 atomics) took the
+  // pointwise pass from 54 to 122 us; 64 blocks per world still give 4 blocks per CU.
+  const int64_t imgs = B * std::max(rule->C, rule->K);
+  const int64_t want = std::max<int64_t>(1, 1024 / imgs);
+  const unsigned per_img = unsigned(std::min<int64_t>(std::min<int64_t>((a.cells + 1023) / 1024, 256), want));
   hipStream_t st = (hipStream_t)stream;
   if (rule->C == 1 && rule->K == 1) {
     const dim3 grid(per_img, unsigned(B));
