@@ -7,8 +7,10 @@ stop "mass is 0 or covers more than 90% of the grid", retiring worlds at uneven 
 refilling their slots from a device queue.  Inputs are resident in HBM before timing.
 
 Printed JSON line: whole-job world-steps/s (value), cell-updates/s, the plain per-world
-and plain batched GPU steps for comparison, the roofline of the fused kernel and the CPU
-baseline (the float32 oracle on this host's cores, bounded sample).
+and plain batched GPU steps for comparison, the same search under leniax's stop rule, the
+3-channel 6-kernel 256^2 search (one early-stopped lx_run on the multi-pass path), the
+gradient workloads, the roofline of the fused kernel and the CPU baseline (the float32
+oracle on this host's cores, bounded sample).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -136,18 +138,39 @@ def adjoint_rate(n=256, steps=64, ckpt=16, reps=5):
     return out
 
 
-def c2_rate(worlds=256, n=256, steps=8, seed=0):
-    """Batched 3-channel 6-kernel parameter search on the plain HBM path: `worlds` worlds of
-    n x n, per-world kernels (kernel k reads channel k // 2, writes a random channel; r_k =
-    R_k / 15 with R_k ~ U[10, 15]; 1-3 rings, b ~ U[0, 1]; gaussian growth mu ~ U[0.1, 0.4],
-    sigma ~ U[0.01, 0.08]; weight h ~ U[0.1, 1]; weighted sum), per-channel U[0, 1) 64x64
-    centred patches; one step = the step and its statistics over the batch (lx_step_stats, no
-    field or potential output, as run_scan_mem_optimized), timed beside lx_update + lx_stats."""
+def c2_search(worlds=256, n=256, seed=0):
+    """The batched 3-channel 6-kernel parameter search: `worlds` worlds of n x n, per-world kernels
+    (kernel k reads channel k // 2, writes a random channel; r_k = R_k / 15 with R_k ~ U[10, 15];
+    1-3 rings, b ~ U[0, 1]; gaussian growth mu ~ U[0.1, 0.4], sigma ~ U[0.01, 0.08]; weight
+    h ~ U[0.1, 1]; weighted sum), per-channel U[0, 1) 64x64 centred patches.  Returns the rule,
+    half spectra [worlds, 6, ...], gf, w and the initial states, resident on the device."""
     import torch
     from leniax_amd import _lib, engine, kernels
     dev = torch.device('cuda', 0)
-    rng = np.random.default_rng(seed)
+    kps, cells = c2_world_params(worlds, n, seed)
     Ks, gfs, ws = [], [], []
+    for kp in kps:
+        K, mp = kernels.get_kernels_and_mapping(kp, [n, n], 3, C2_R)
+        Ks.append(K)
+        gfs.append(mp.get_gf_params())
+        ws.append(mp.get_kernels_weight_per_channel())
+    K = torch.cat(Ks)  # [worlds, 3, 2, n, n]
+    rule = engine.make_rule([n, n], 3, [0, 0, 1, 1, 2, 2], [_lib.GROWTH_IDS['gaussian']] * 6, 0, 0)
+    Kh = engine.prepare_spectra(rule, K, list(range(6)))
+    del K, Ks
+    gf = torch.stack(gfs).contiguous()
+    w = torch.stack(ws).contiguous()
+    return rule, Kh, gf, w, torch.as_tensor(cells).to(dev)
+
+
+C2_R, C2_T = 15., 10.
+
+
+def c2_world_params(worlds=256, n=256, seed=0):
+    """leniax kernels_params of each world of the 3C6K search and the initial states [worlds, 3, n, n]
+    (host numpy; one generator for the bench and its parity test)."""
+    rng = np.random.default_rng(seed)
+    kps = []
     for _ in range(worlds):
         kp = []
         for k in range(6):
@@ -159,20 +182,59 @@ def c2_rate(worlds=256, n=256, steps=8, seed=0):
                        'kf_slug': 'gauss_bump', 'kf_params': [1.], 'gf_slug': 'gaussian',
                        'gf_params': [float(rng.uniform(0.1, 0.4)), float(rng.uniform(0.01, 0.08))],
                        'h': float(rng.uniform(0.1, 1.)), 'c_in': k // 2, 'c_out': int(rng.integers(0, 3))})
-        K, mp = kernels.get_kernels_and_mapping(kp, [n, n], 3, 15.)
-        Ks.append(K)
-        gfs.append(mp.get_gf_params())
-        ws.append(mp.get_kernels_weight_per_channel())
-    K = torch.cat(Ks)  # [worlds, 3, 2, n, n]
-    rule = engine.make_rule([n, n], 3, [0, 0, 1, 1, 2, 2], [_lib.GROWTH_IDS['gaussian']] * 6, 0, 0)
-    Kh = engine.prepare_spectra(rule, K, list(range(6)))
-    del K, Ks
-    gf = torch.stack(gfs).contiguous()
-    w = torch.stack(ws).contiguous()
+        kps.append(kp)
     cells = np.zeros((worlds, 3, n, n), np.float32)
     o = (n - 64) // 2
     cells[:, :, o:o + 64, o:o + 64] = rng.random((worlds, 3, 64, 64), dtype=np.float32)
-    a = torch.as_tensor(cells).to(dev)
+    return kps, cells
+
+
+def c2_flops_bytes(n=256):
+    F = 2.5 * n * n * math.log2(n * n) * 9 + 6 * 6 * n * (n // 2 + 1) + 10 * 6 * n * n + 9 * n * n
+    # algorithmic HBM bytes per world-step: state read + write, the world's 6 half spectra
+    Bw = 8 * 3 * n * n + 8 * 6 * n * (n // 2 + 1)
+    return F, Bw
+
+
+def c2_run(worlds=256, n=256, max_steps=512, stop='leniax', reps=1):
+    """The 3C6K search as one lx_run (multi-pass path): every world up to max_steps steps with the
+    per-step statistics, retired when the stop policy says so (uneven times), graph-replayed
+    steps over the live list.  Executed world-steps are counted."""
+    import torch
+    from leniax_amd import engine
+    rule, Kh, gf, w, a0 = c2_search(worlds, n)
+    dt = torch.tensor([np.float32(1) / np.float32(C2_T)], device=a0.device)
+    out = torch.empty_like(a0)
+
+    def run():
+        return engine.run_batched(rule, a0, Kh, True, gf, w, dt, max_steps, stop, 0, C2_R, C2_T, want_stats=True,
+                                  out=out)
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = run()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / reps
+    st = res['steps'].cpu().numpy()
+    executed = int(st.sum())
+    F, Bw = c2_flops_bytes(n)
+    cells = executed * 3 * n * n / sec
+    bound = min(157.3e12 / (F / (3 * n * n)), 8.0e12 / (Bw / (3 * n * n)))
+    return {'config': f'{worlds} worlds of {n}x{n}, 3 channels, 6 kernels, per-world spectra, <= {max_steps} steps, '
+                      f'stop policy {stop}, per-step statistics (lx_run, multi-pass path with retirement)',
+            'ms': sec * 1e3, 'executed_world_steps': executed, 'step_budget': worlds * max_steps,
+            'steps_per_world': {'min': int(st.min()), 'median': int(np.median(st)), 'max': int(st.max())},
+            'world_steps_per_s': executed / sec, 'cell_updates_per_s': cells, 'roofline_frac': cells / bound}
+
+
+def c2_rate(worlds=256, n=256, steps=8, seed=0):
+    """One step of the 3C6K search over all its worlds: the step and its statistics (lx_step_stats,
+    no field or potential output, as run_scan_mem_optimized), timed beside lx_update + lx_stats."""
+    import torch
+    from leniax_amd import _lib, engine
+    dev = torch.device('cuda', 0)
+    rule, Kh, gf, w, a = c2_search(worlds, n, seed)
     bbuf = torch.empty_like(a)
     field = torch.empty_like(a)
     pot = torch.empty((worlds, 6, n, n), device=dev)
@@ -210,9 +272,7 @@ def c2_rate(worlds=256, n=256, steps=8, seed=0):
 
     sec = timed(True)
     sec2 = timed(False)
-    F = 2.5 * n * n * math.log2(n * n) * 9 + 6 * 6 * n * (n // 2 + 1) + 10 * 6 * n * n + 9 * n * n
-    # algorithmic HBM bytes per world-step: state read + write, the world's 6 half spectra
-    Bw = 8 * 3 * n * n + 8 * 6 * n * (n // 2 + 1)
+    F, Bw = c2_flops_bytes(n)
     cells = worlds * 3 * n * n / sec
     bound = min(157.3e12 / (F / (3 * n * n)), 8.0e12 / (Bw / (3 * n * n)))  # cell-updates/s
     return {'config': f'{worlds} worlds of {n}x{n}, 3 channels, 6 kernels, per-world spectra, step + statistics '
@@ -273,12 +333,15 @@ def c4_adjoint_rate(worlds=16, n=96, steps=128, ckpt=32, reps=2):
 
 def measured_traffic():
     """HBM bytes per fused launch from the committed rocprofv3 PMC summary (FETCH_SIZE +
-    WRITE_SIZE, each scaled by the counter's reading for a copy of known size), or None."""
+    WRITE_SIZE, each scaled by the counter's reading for a copy of known size) and where they
+    come from (file, the commit it was measured at, the command), or (None, None)."""
     path = os.path.join(ROOT, 'profiles', 'fused2d_kernel_128_traffic.json')
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as fh:
-        return json.load(fh).get('hbm_bytes_per_launch')
+        d = json.load(fh)
+    return d.get('hbm_bytes_per_launch'), {'file': 'profiles/fused2d_kernel_128_traffic.json',
+                                           'commit': d.get('commit'), 'command': d.get('command')}
 
 
 def direct_rate(worlds=256, n=128, reps=10):
@@ -310,6 +373,7 @@ def main():
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-tune', action='store_true', help='fixed schedules instead of the autotuner')
     ap.add_argument('--only-fused', action='store_true', help='time the fused launch only (profiling runs)')
     ap.add_argument('--copy-mb', type=int, default=0,
                     help='after timing, copy this many MiB device-to-device (a known-size kernel for PMC scaling)')
@@ -317,6 +381,8 @@ def main():
                     help='time the 3C6K 256^2 step-and-statistics workload only (profiling runs)')
     ap.add_argument('--only-c4', action='store_true',
                     help='time the 16 x 96^3 checkpointed adjoint workload only (profiling runs)')
+    ap.add_argument('--only-c2-run', action='store_true',
+                    help='time the 3C6K search as one early-stopped lx_run only (profiling runs)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: every world batch fits one MI355X and worlds are independent '
@@ -328,8 +394,14 @@ def main():
 
     dev = torch.device('cuda', 0)
     torch.cuda.set_device(dev)
+    if not args.no_tune:
+        # schedules picked by the autotuner on the first (warm-up) call of each shape
+        engine.autotune('on', cache=os.environ.get('LENIAX_AMD_TUNE_CACHE') or None)
     if args.only_c4:
         print(json.dumps(c4_adjoint_rate(reps=1)), flush=True)
+        return
+    if args.only_c2_run:
+        print(json.dumps({'leniax_stop': c2_run(stop='leniax'), 'every_step': c2_run(stop='none')}), flush=True)
         return
     if args.only_c2:
         print(json.dumps(c2_rate(steps=max(args.steps, 1))), flush=True)
@@ -351,7 +423,7 @@ def main():
     F = flops_per_world_step()
 
     def fused(stop='mass', max_steps=MAX_STEPS):
-        return engine.run_fused(rule, cells0, Kh, False, gf, w, dt, max_steps, stop, 0, R, T, want_stats=True,
+        return engine.run_batched(rule, cells0, Kh, False, gf, w, dt, max_steps, stop, 0, R, T, want_stats=True,
                                 out=out)
 
     for _ in range(args.warmup):
@@ -384,6 +456,19 @@ def main():
     full_ms = e0.elapsed_time(e1)
     full_ws = B * MAX_STEPS / (full_ms / 1e3)
     log(f'fused full run: {full_ms:.1f} ms for {B * MAX_STEPS} world-steps -> {full_ws / 1e6:.2f} M world-steps/s')
+    # the same search under leniax's own stop rule (its heuristics, leniax/statistics.py:134-205)
+    e0.record(stream)
+    lres = fused(stop='leniax')
+    e1.record(stream)
+    torch.cuda.synchronize()
+    len_ms = e0.elapsed_time(e1)
+    len_steps = lres['steps'].cpu().numpy()
+    leniax_stop = {'ms': len_ms, 'executed_world_steps': int(len_steps.sum()),
+                   'world_steps_per_s': float(len_steps.sum()) / (len_ms / 1e3),
+                   'steps_per_world': {'min': int(len_steps.min()), 'median': int(np.median(len_steps)),
+                                       'max': int(len_steps.max())}}
+    log(f"fused run, leniax stop: {len_ms:.2f} ms, {leniax_stop['executed_world_steps']} world-steps -> "
+        f"{leniax_stop['world_steps_per_s'] / 1e6:.2f} M world-steps/s")
 
     if args.copy_mb:
         src = torch.ones(args.copy_mb * 2 ** 18, device=dev)
@@ -430,7 +515,14 @@ def main():
     log(f"3C6K 256 x 256^2 step + statistics in one pass: {c2['ms_per_step']:.3f} ms/step, "
         f"{c2['world_steps_per_s']:.0f} world-steps/s, {c2['achieved_gbps']:.0f} GB/s algorithmic, "
         f"{c2['roofline_frac']:.3f} of its bound (update + stats as two calls: {c2['two_calls_ms_per_step']:.3f} ms)")
+    c2_stop = c2_run(stop='leniax')
+    c2_full = c2_run(stop='none')
+    for name, r in (('leniax stop', c2_stop), ('every step', c2_full)):
+        log(f"3C6K search, 256 x 256^2, <= 512 steps, {name}: {r['ms']:.1f} ms, {r['executed_world_steps']} "
+            f"world-steps (per world {r['steps_per_world']}), {r['world_steps_per_s']:.0f} world-steps/s, "
+            f"{r['roofline_frac']:.3f} of its bound")
 
+    traffic, traffic_src = measured_traffic()
     flops_per_launch = F * steps_run.sum()
     achieved = flops_per_launch / (launch_ms / 1e3)
     result = {
@@ -456,19 +548,27 @@ def main():
                                   'ms': full_ms},
         'gradient_256x256_64_steps': adj,
         'adjoint_3d_16x96cubed_128_steps': c4,
+        'fused_leniax_stop': leniax_stop,
         'plain_3c6k_256_worlds_256x256': c2,
+        'search_3c6k_256_worlds_leniax_stop': c2_stop,
+        'search_3c6k_256_worlds_every_step': c2_full,
         'direct_potential': dr,
         'plain_single_world_steps_per_s': plain_single,
         'plain_batched_world_steps_per_s': plain_batched,
-        'fused_vs_plain_single_world': full_ws / plain_single,
-        'fused_vs_plain_batched': full_ws / plain_batched,
-        'roofline': {'bound': 'mfma', 'compute': 'fp32 VALU (no MFMA: no dense contraction on the path)',
+        'fused_vs_plain_single_world': value / plain_single,
+        'fused_vs_plain_batched': value / plain_batched,
+        'fused_full_vs_plain_single_world': full_ws / plain_single,
+        'fused_full_vs_plain_batched': full_ws / plain_batched,
+        'roofline': {'bound': 'valu', 'compute': 'fp32 VALU (no MFMA: no dense contraction on the path)',
                      'kernel': 'fused2d_kernel<128> (probe + resume dispatches) and the statistics pass of '
                                'one lx_run',
                      'achieved': achieved / 1e12, 'peak': PEAK_FP32 / 1e12,
-                     'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32, 'traffic': measured_traffic(),
+                     'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32, 'traffic': traffic,
+                     'traffic_source': traffic_src,
                      'flops_per_world_step': F, 'launch_ms': launch_ms},
     }
+    result['schedules'] = {'autotune': not args.no_tune,
+                           'decided': {f'{k} {key}': v for (k, key), v in engine.tune_report()['decided'].items()}}
     if not args.no_cpu:
         log('cpu baseline ...')
         result['cpu_baseline'] = cpu_baseline()

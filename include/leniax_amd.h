@@ -219,12 +219,53 @@ int lx_irfft2(int32_t ndim, const int32_t* dims, int64_t n, const void* X_d, flo
  * and ADDS dL/dgf, dL/dw, dL/ddt into grad_gf_d / grad_w_d / grad_dt_d (laid out and strided like
  * gf_d / w_d / dt_d; each may be NULL).  ste != 0: v1's clip passes gradients straight through, as
  * leniax/core.py:265-270 does; ste == 0: the derivative of the clip itself (1 inside (0, 1)).
- * dfield_d / dpot_in_d (NULL: zero) are cotangents of the step's field and potential outputs. */
+ * dfield_d / dpot_in_d (NULL: zero) are cotangents of the step's field and potential outputs.
+ * work_d: lx_adjoint_workspace_size() bytes, where the gradient sums are reduced in a fixed order
+ * (the gradients are deterministic and do not depend on the batch a world is in). */
+int lx_adjoint_workspace_size(const lx_rule* rule, int64_t B, size_t* bytes);
 int lx_adjoint_pointwise(const lx_rule* rule, int64_t B, const float* state_d, const float* field_d,
                          const float* potential_d, const float* gf_d, int64_t gf_bstride, const float* w_d,
                          int64_t w_bstride, const float* dt_d, int64_t dt_bstride, const float* lambda_d,
                          const float* dfield_d, const float* dpot_in_d, int32_t ste, float* lambda_out_d,
-                         float* dpot_d, float* grad_gf_d, float* grad_w_d, float* grad_dt_d, void* stream);
+                         float* dpot_d, float* grad_gf_d, float* grad_w_d, float* grad_dt_d, void* work_d,
+                         void* stream);
+
+/* The gradient of a whole run (leniax/runner.py:363-444: make_pipeline_fn + make_gradient_fn over
+ * `steps` updates of leniax/core.py:13-49), with a checkpoint every ckpt_every steps and each
+ * segment recomputed in the reverse pass.  lx_grad_run_forward writes the final states cells_d and
+ * keeps the checkpoints in work_d; lx_grad_run_backward, given lambda_d = dL/d(final states) and
+ * the same work_d, ADDS dL/dgf, dL/dw, dL/ddt (laid out and strided like gf / w / dt) and, when
+ * grad_spec_d is set, sum over steps of rfftn(dL/dU_k) conj(rfftn(A_cin k)) per world ([B][K][half],
+ * see lx_adjoint_potential for dL/dK and dL/d(real kernel)), and writes dL/d(initial states) to
+ * grad_cells0_d.  Any gradient pointer may be NULL.  The forward must run with the same
+ * grad_spec_d-or-NULL choice as the backward (it sizes the workspace). */
+typedef struct lx_grad_run_args {
+  const float* cells0_d;   /* [B][C][dims] initial states */
+  const void* K_half_d;    /* lx_prepare_spectra layout */
+  int64_t k_bstride;       /* complex elements between worlds' spectra, 0: shared */
+  const float* gf_d;       /* [K][2] per world, stride gf_bstride floats */
+  int64_t gf_bstride;
+  const float* w_d;        /* [C][K] per world, stride w_bstride floats */
+  int64_t w_bstride;
+  const float* dt_d;       /* dt per world, stride dt_bstride */
+  int64_t dt_bstride;
+  int64_t B;               /* worlds */
+  int32_t steps;           /* updates */
+  int32_t ckpt_every;      /* steps between checkpoints */
+  int32_t ste;             /* v1 clip: 1 straight-through (leniax/core.py:265-270), 0 its true derivative */
+  float* cells_d;          /* [B][C][dims] final states (forward output) */
+  const float* lambda_d;   /* [B][C][dims] dL/d(final states) (backward input) */
+  float* grad_cells0_d;    /* [B][C][dims] dL/d(initial states) or NULL */
+  float* grad_gf_d;        /* added into, or NULL */
+  float* grad_w_d;
+  float* grad_dt_d;
+  void* grad_spec_d;       /* [B][K][half] complex64, added into, or NULL */
+  void* work_d;            /* lx_grad_run_workspace_size() bytes: checkpoints + one segment */
+} lx_grad_run_args;
+int lx_grad_run_workspace_size(const lx_rule* rule, int64_t B, int32_t steps, int32_t ckpt_every, int32_t want_spec,
+                               size_t* bytes);
+int lx_grad_run_forward(const lx_rule* rule, const lx_grad_run_args* args, void* stream);
+int lx_grad_run_backward(const lx_rule* rule, const lx_grad_run_args* args, void* stream);
 
 /* Hermitian extension of n half spectra [n][H][W/2+1] to full spectra [n][H][W], times scale
  * (the full complex K layout of leniax/kernels.py:140 from a real array's half spectrum). */
@@ -241,10 +282,17 @@ int lx_adjoint_potential(const lx_rule* rule, int64_t B, const void* K_half_d, i
                          const float* dpot_d, const void* state_spec_d, void* dspec_d, float* lambda_d,
                          void* work_d, void* stream);
 
-/* Fused persistent batched run (leniax/runner.py:167-215 run_scan_mem_optimized semantics):
- * one launch steps every world up to max_steps, world after world from a device queue,
- * each world resident in registers + LDS, statistics and stop policy evaluated every step.
- * Supported shapes: lx_run_supported(). */
+/* Batched run (leniax/runner.py:167-215 run_scan_mem_optimized and :119-164 run_scan semantics):
+ * every world steps up to max_steps, statistics and stop policy evaluated every step on the device,
+ * worlds retiring at uneven times.  Two paths, chosen by the rule (lx_run_supported):
+ *  - fused (2-D N x N, N in {32, 64, 128}, one channel, one kernel): one persistent launch steps
+ *    world after world from a device queue, each world resident in registers + LDS;
+ *  - multi-pass (every other rule lx_update takes, 2-D and 3-D): the step's HBM passes over a
+ *    device-side list of live worlds; a control kernel after each step retires stopped worlds and
+ *    refills their slots from the queue of worlds not yet started (at most `slots` in flight), and
+ *    the step loop is replayed from HIP graphs of several steps.  This path waits on the stream
+ *    between graph replays to read the live count (it returns once the last replay is enqueued).
+ * The statistics rows after a world's last step are left as the caller put them. */
 typedef struct lx_run_args {
   const float* cells0_d;   /* [B][C][dims] initial states */
   float* cells_d;          /* [B][C][dims] final states (may equal cells0_d) */
@@ -268,16 +316,48 @@ typedef struct lx_run_args {
   float* cells_hist_d;     /* [max_steps][B][C][dims] state before each step, or NULL */
   float* field_hist_d;     /* [max_steps][B][C][dims] or NULL */
   float* potential_hist_d; /* [max_steps][B][K][dims] or NULL */
-  void* scratch_d;         /* lx_run_scratch_bytes(B) bytes of device scratch (queues, parked carries) */
+  void* scratch_d;         /* lx_run_scratch_size() bytes of device scratch */
+  int32_t slots;           /* multi-pass path: worlds in flight at once (0: all B); ignored by the fused path */
 } lx_run_args;
 
 /* With an early-stop policy and more than 256 steps, lx_run is two launches: every world first
  * runs at most max_steps / 8 steps (a probe); the worlds still running then resume together from
  * their parked state and carry, so that the long-lived ones start at once instead of whenever a
  * compute unit frees up.  Results are those of one uninterrupted run. */
-size_t lx_run_scratch_bytes(int64_t B);
+#define LX_RUN_FUSED 1
+#define LX_RUN_PASSES 2
+/* LX_RUN_FUSED or LX_RUN_PASSES: the path lx_run takes for this rule; 0: the rule is invalid or its
+ * world size unsupported. */
 int lx_run_supported(const lx_rule* rule);
+/* Scratch bytes of lx_run for B worlds (slots as in lx_run_args). */
+int lx_run_scratch_size(const lx_rule* rule, int64_t B, int32_t slots, size_t* bytes);
+/* The fused path's scratch alone (= lx_run_scratch_size for a rule with LX_RUN_FUSED). */
+size_t lx_run_scratch_bytes(int64_t B);
 int lx_run(const lx_rule* rule, const lx_run_args* args, void* stream);
+
+/* Autotuner.  The passes have variants that compute the same thing in different schedules:
+ *   "conv_staged"  the convolution pass at 256 points staged through LDS (1) or read in the lane
+ *                  groups' layout (0);
+ *   "line_xcd"     line passes' blocks dealt in XCD-contiguous ranges (1) or in launch order (0);
+ *   "lines96_tpr"  3-D forward / inverse line passes at 96 points: lanes per line, 4 (24 points per
+ *                  lane, transpose through LDS) or 8 (12 points, DPP network);
+ *   "fused_probe"  fused early-stop run: first launch steps every world max_steps / value steps
+ *                  (0: one launch);
+ *   "graph_steps"  multi-pass lx_run: steps per captured graph.
+ * LX_TUNE_OFF (the default, or LENIAX_AMD_TUNE=0): fixed rules.  LX_TUNE_ON: the first call of a
+ * shape times the candidates on its own inputs (outside stream capture) and keeps the fastest, in
+ * memory and in the cache file (LENIAX_AMD_TUNE_CACHE or lx_tune_cache).  LX_TUNE_CACHED: decisions
+ * from memory / the cache file, fixed rules otherwise.  lx_tune_force pins a knob for every shape
+ * (value < 0 unpins).  lx_autotune returns the previous mode; lx_tune_report writes "knob key value"
+ * lines (forced knobs first) and returns the text's length.  No counterpart in leniax, whose
+ * schedules are XLA's (leniax/core.py:76-87). */
+#define LX_TUNE_OFF 0
+#define LX_TUNE_ON 1
+#define LX_TUNE_CACHED 2
+int lx_autotune(int32_t mode);
+int lx_tune_cache(const char* path);
+int lx_tune_force(const char* knob, int32_t value);
+int lx_tune_report(char* buf, size_t n);
 
 #ifdef __cplusplus
 }

@@ -97,6 +97,39 @@ class lx_run_args(ctypes.Structure):
         ('field_hist_d', ctypes.c_void_p),
         ('potential_hist_d', ctypes.c_void_p),
         ('scratch_d', ctypes.c_void_p),
+        ('slots', ctypes.c_int32),
+    ]
+
+
+LX_RUN_FUSED = 1
+LX_RUN_PASSES = 2
+TUNE = {'off': 0, 'on': 1, 'cached': 2}
+KNOBS = ('conv_staged', 'line_xcd', 'lines96_tpr', 'fused_probe', 'graph_steps')
+
+
+class lx_grad_run_args(ctypes.Structure):
+    _fields_ = [
+        ('cells0_d', ctypes.c_void_p),
+        ('K_half_d', ctypes.c_void_p),
+        ('k_bstride', ctypes.c_int64),
+        ('gf_d', ctypes.c_void_p),
+        ('gf_bstride', ctypes.c_int64),
+        ('w_d', ctypes.c_void_p),
+        ('w_bstride', ctypes.c_int64),
+        ('dt_d', ctypes.c_void_p),
+        ('dt_bstride', ctypes.c_int64),
+        ('B', ctypes.c_int64),
+        ('steps', ctypes.c_int32),
+        ('ckpt_every', ctypes.c_int32),
+        ('ste', ctypes.c_int32),
+        ('cells_d', ctypes.c_void_p),
+        ('lambda_d', ctypes.c_void_p),
+        ('grad_cells0_d', ctypes.c_void_p),
+        ('grad_gf_d', ctypes.c_void_p),
+        ('grad_w_d', ctypes.c_void_p),
+        ('grad_dt_d', ctypes.c_void_p),
+        ('grad_spec_d', ctypes.c_void_p),
+        ('work_d', ctypes.c_void_p),
     ]
 
 
@@ -129,8 +162,9 @@ SIGNATURES = {
     'lx_check_heuristics': (ctypes.c_int, [_P, _I32, _I64, _I32, _P, _P, _P]),
     'lx_rfft2': (ctypes.c_int, [_I32, ctypes.POINTER(_I32), _I64, _P, _P, _P]),
     'lx_irfft2': (ctypes.c_int, [_I32, ctypes.POINTER(_I32), _I64, _P, _P, _P, _P]),
+    'lx_adjoint_workspace_size': (ctypes.c_int, [_R, _I64, ctypes.POINTER(ctypes.c_size_t)]),
     'lx_adjoint_pointwise': (ctypes.c_int, [_R, _I64, _P, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P,
-                                            _I32, _P, _P, _P, _P, _P, _P]),
+                                            _I32, _P, _P, _P, _P, _P, _P, _P]),
     'lx_half_to_full': (ctypes.c_int, [_I32, ctypes.POINTER(_I32), _I64, _P, _P, _F, _P]),
     'lx_adjoint_potential': (ctypes.c_int, [_R, _I64, _P, _I64, _P, _P, _P, _P, _P, _P]),
     'lx_run_scratch_bytes': (ctypes.c_size_t, [_I64]),
@@ -141,7 +175,15 @@ SIGNATURES = {
     'lx_build_kernels': (ctypes.c_int, [_I32, ctypes.POINTER(_I32), _P, _I64, ctypes.c_double, _I32, _I32, _P, _P]),
     'lx_build_spectra': (ctypes.c_int, [_I32, ctypes.POINTER(_I32), _P, _I64, ctypes.c_double, _I32, _P, _P, _P]),
     'lx_run_supported': (ctypes.c_int, [_R]),
+    'lx_run_scratch_size': (ctypes.c_int, [_R, _I64, _I32, ctypes.POINTER(ctypes.c_size_t)]),
     'lx_run': (ctypes.c_int, [_R, ctypes.POINTER(lx_run_args), _P]),
+    'lx_autotune': (ctypes.c_int, [_I32]),
+    'lx_tune_cache': (ctypes.c_int, [ctypes.c_char_p]),
+    'lx_tune_force': (ctypes.c_int, [ctypes.c_char_p, _I32]),
+    'lx_tune_report': (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    'lx_grad_run_workspace_size': (ctypes.c_int, [_R, _I64, _I32, _I32, _I32, ctypes.POINTER(ctypes.c_size_t)]),
+    'lx_grad_run_forward': (ctypes.c_int, [_R, ctypes.POINTER(lx_grad_run_args), _P]),
+    'lx_grad_run_backward': (ctypes.c_int, [_R, ctypes.POINTER(lx_grad_run_args), _P]),
 }
 
 _lib = None

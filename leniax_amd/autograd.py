@@ -129,7 +129,7 @@ class _Ctx:
                   engine.ptr(pot), engine.ptr(self.gf), self.gfs, engine.ptr(self.w), self.ws, engine.ptr(self.dt),
                   self.dts, engine.ptr(lam), engine.ptr(dfield), engine.ptr(dpot_in), int(bool(ste)),
                   engine.ptr(lam_out), engine.ptr(dpot), engine.ptr(acc['gf']), engine.ptr(acc['w']),
-                  engine.ptr(acc['dt']), engine.stream_ptr())
+                  engine.ptr(acc['dt']), engine.ptr(adjoint_workspace(self.rule, self.N)), engine.stream_ptr())
         dspec = acc.get('dspec')
         if dspec is not None and FA is None:
             FA = rfftn(A, A.dim() - 2)
@@ -152,6 +152,13 @@ class _Ctx:
 
     def grad_dt(self, g, dt_orig):
         return _grad_dt(g, dt_orig)
+
+
+def adjoint_workspace(rule, N):
+    """Device workspace of lx_adjoint_pointwise (its per-block gradient sums) for N worlds."""
+    nb = ctypes.c_size_t(0)
+    _lib.call('lx_adjoint_workspace_size', ctypes.byref(rule), N, ctypes.byref(nb))
+    return torch.empty(max(int(nb.value), 256), dtype=torch.uint8, device=engine.device())
 
 
 def _grad_gf(g_pad, gf_orig):
@@ -608,7 +615,7 @@ class _Field(torch.autograd.Function):
                   engine.ptr(pot), engine.ptr(gf2), core._per_world_stride(gf2, N, 'gf'), engine.ptr(w),
                   core._per_world_stride(w, N, 'weights'), engine.ptr(zero_dt), 0, engine.ptr(field),
                   engine.ptr(g.contiguous()), None, 1, engine.ptr(scratch), engine.ptr(dpot), engine.ptr(ggf),
-                  engine.ptr(gw), None, engine.stream_ptr())
+                  engine.ptr(gw), None, engine.ptr(adjoint_workspace(rule, N)), engine.stream_ptr())
         return dpot, _grad_gf(ggf, gf), gw, None, None
 
 
@@ -646,7 +653,8 @@ class _State(torch.autograd.Function):
         gdt = torch.zeros_like(dt_t)
         _lib.call('lx_adjoint_pointwise', ctypes.byref(rule), N, engine.ptr(A), engine.ptr(G), engine.ptr(G),
                   engine.ptr(gf), 0, engine.ptr(w), 0, engine.ptr(dt_t), dts, engine.ptr(g.contiguous()), None, None,
-                  int(bool(ste)), engine.ptr(dA), engine.ptr(dG), None, None, engine.ptr(gdt), engine.stream_ptr())
+                  int(bool(ste)), engine.ptr(dA), engine.ptr(dG), None, None, engine.ptr(gdt),
+                  engine.ptr(adjoint_workspace(rule, N)), engine.stream_ptr())
         return dA, dG, _grad_dt(gdt, dt), None, None
 
 

@@ -90,27 +90,53 @@ extern "C" int lx_prepare_spectra(const lx_rule* rule, const void* K_full_d, int
   return LX_OK;
 }
 
-// ------------------------------------------------------------------ fused run
+// ------------------------------------------------------------------ batched run
+namespace {
+// the world size the step passes take on every axis (2^k in [16, 512], 3 * 2^k in [48, 384])
+bool passes_size_ok(const lx_rule* rule) {
+  for (int i = 0; i < rule->ndim; ++i) {
+    int n = rule->dims[i];
+    if (n % 3 == 0) n /= 3;
+    if (n < 16 || n > 512 || (n & (n - 1)) || (rule->dims[i] % 3 == 0 && (n < 16 || n > 128))) return false;
+  }
+  return true;
+}
+}  // namespace
+
 extern "C" int lx_run_supported(const lx_rule* rule) {
   if (lx_check_rule(rule)) return 0;
-  return lx_fused2d_supported(rule);
+  if (lx_fused2d_supported(rule)) return LX_RUN_FUSED;
+  return passes_size_ok(rule) ? LX_RUN_PASSES : 0;
 }
 
 extern "C" size_t lx_run_scratch_bytes(int64_t B) { return lx_run_scratch_layout(B < 0 ? 0 : B).total; }
+
+extern "C" int lx_run_scratch_size(const lx_rule* rule, int64_t B, int32_t slots, size_t* bytes) {
+  int rc = lx_check_rule(rule);
+  if (rc) return rc;
+  if (!bytes || B < 0 || slots < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  if (lx_fused2d_supported(rule)) {
+    *bytes = lx_run_scratch_layout(B).total;
+    return LX_OK;
+  }
+  return lx_passes_scratch(rule, B, slots, bytes);
+}
 
 extern "C" int lx_run(const lx_rule* rule, const lx_run_args* a, void* stream) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
   if (!a) return lx_fail(LX_EINVAL, "args is NULL");
-  if (a->B < 0 || a->max_steps < 0) return lx_fail(LX_EINVAL, "B and max_steps must be >= 0");
+  if (a->B < 0 || a->max_steps < 0 || a->slots < 0) return lx_fail(LX_EINVAL, "B, max_steps and slots must be >= 0");
   if (a->stop_policy < LX_STOP_NONE || a->stop_policy > LX_STOP_MASS)
     return lx_fail(LX_EINVAL, "unknown stop policy %d", a->stop_policy);
   if (!(a->R > 0.f) || !(a->T > 0.f)) return lx_fail(LX_EINVAL, "R and T must be positive");
-  if (!lx_fused2d_supported(rule))
-    return lx_fail(LX_ENOTSUP, "lx_run: no fused kernel for ndim=%d dims=%dx%d C=%d K=%d (use lx_update)",
-                   rule->ndim, rule->dims[0], rule->dims[1], rule->C, rule->K);
+  const int path = lx_run_supported(rule);
+  if (!path)
+    return lx_fail(LX_ENOTSUP, "lx_run: world size %dx%dx%d unsupported (2^k in [16, 512], 3*2^k in [48, 384])",
+                   rule->dims[0], rule->dims[1], rule->ndim == 3 ? rule->dims[2] : 1);
   if (a->B == 0) return LX_OK;  // no world: pointers may be NULL
   if (!a->cells0_d || !a->cells_d || !a->K_half_d || !a->gf_d || !a->w_d || !a->dt_d || !a->scratch_d)
     return lx_fail(LX_EINVAL, "NULL device pointer in lx_run_args");
-  return lx_fused2d_launch(rule, a, (hipStream_t)stream);
+  if (path == LX_RUN_FUSED) return lx_fused2d_launch(rule, a, (hipStream_t)stream);
+  return lx_passes_run(rule, a, (hipStream_t)stream);
 }

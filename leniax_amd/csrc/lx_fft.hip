@@ -16,8 +16,13 @@
 // FFT of P points, a twiddle multiply and the cross-lane step (a transpose through LDS when
 // P is a multiple of TPR, else a radix-2 network across the group through DPP).
 // Workspace of a step: S [B][C][half] and U [B][K][half] complex64, half = prod(dims[:-1]) (W/2+1).
+#include <stdio.h>
+
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <type_traits>
+#include <vector>
 
 #include "lx_internal.h"
 
@@ -101,18 +106,34 @@ __global__ void init_tw_kernel() {
 
 constexpr int kMaxDevices = 64;
 
+// The tables of length L on the current device, built once per process and device, before the
+// first kernel that reads them.  The build runs on an internal stream and is waited for, so that
+// every later call, on any stream or thread, finds the tables complete.  It is refused while the
+// caller's stream is being captured into a graph: a build recorded there would run only when the
+// graph does, leaving the eager calls in between with empty tables.
 template <int L>
 int ensure_tw(hipStream_t stream) {
-  static bool ready[kMaxDevices] = {};
+  static std::atomic<bool> ready[kMaxDevices];
+  static std::mutex mu;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
     return lx_fail(LX_EDEVICE, "hipGetDevice failed or device index out of range");
-  if (!ready[dev]) {
-    hipLaunchKernelGGL(init_tw_kernel<L>, dim3(1), dim3(256), 0, stream);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return lx_fail(LX_EDEVICE, "twiddle init: %s", hipGetErrorString(e));
-    ready[dev] = true;
-  }
+  if (ready[dev].load(std::memory_order_acquire)) return LX_OK;
+  std::lock_guard<std::mutex> lock(mu);
+  if (ready[dev].load(std::memory_order_relaxed)) return LX_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (stream && hipStreamIsCapturing(stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+    return lx_fail(LX_EINVAL, "transform length %d is first used inside a stream capture: make one call of the "
+                   "same shape outside the capture first (it builds the twiddle tables)", L);
+  hipStream_t own = nullptr;
+  if (hipStreamCreateWithFlags(&own, hipStreamNonBlocking) != hipSuccess)
+    return lx_fail(LX_EDEVICE, "twiddle init: stream creation failed");
+  hipLaunchKernelGGL(init_tw_kernel<L>, dim3(1), dim3(256), 0, own);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(own);
+  hipStreamDestroy(own);
+  if (e != hipSuccess) return lx_fail(LX_EDEVICE, "twiddle init: %s", hipGetErrorString(e));
+  ready[dev].store(true, std::memory_order_release);
   return LX_OK;
 }
 
@@ -192,13 +213,35 @@ int make_geom(int ndim, const int32_t* dims, Geom& g) {
   return LX_OK;
 }
 
+// ---------------------------------------------------------------- batched-run indirection
+// The kernels of lx_run's multi-pass path (LIST = true) step the worlds of a live list: slot s of
+// the list holds world list[s], slots >= *nlive are empty.  The per-step temporaries (spectra S,
+// U, the row-pair records) are indexed by slot; what persists (the state, carry, parameters,
+// spectra, statistics) by world.  World b's state after t of its steps lives in X0 (t even) or X1
+// (t odd), t = carry[b].t: worlds advance in lockstep inside a step but start at different steps.
+struct RunView {
+  const int32_t* list;
+  const int32_t* nlive;
+  const lx_carry* carry;
+  float* X0;
+  float* X1;
+  int64_t world_elems;  // C * cells floats per world
+};
+
+// the state buffer holding world b's current state
+LX_DEV float* run_state(const RunView& v, int64_t b, bool next) {
+  const bool odd = ((v.carry[b].t & 1) != 0) != next;
+  return (odd ? v.X1 : v.X0) + b * v.world_elems;
+}
+
 // ---------------------------------------------------------------- rows forward
 // Group = one pair of rows packed as z = a + i b; after the transform Z goes through the group's
 // scratch in frequency order and comes out as the two half spectra, A[k] = (Z[k] + conj Z[W-k])/2,
 // B[k] = (Z[k] - conj Z[W-k])/2i, stored with consecutive lanes on consecutive k.
-template <int W>
+// LIST: npairs is the pairs of one slot (C rows / 2); the state comes from the slot's world.
+template <int W, bool LIST>
 __global__ __launch_bounds__(256) void rows_fwd_kernel(const float* __restrict__ A, float2* __restrict__ S,
-                                                       int64_t npairs) {
+                                                       int64_t npairs, RunView rv) {
   using LC = Lanes<W>;
   constexpr int TPR = LC::TPR, P = LC::P, WH = W / 2 + 1;
   __shared__ __align__(16) float2 xch_all[LC::XCH];
@@ -206,9 +249,18 @@ __global__ __launch_bounds__(256) void rows_fwd_kernel(const float* __restrict__
   const int t = threadIdx.x % TPR;
   float2* xch = xch_all + (threadIdx.x / TPR) * (TPR * LC::XRS);
   const int64_t pr = (int64_t(blockIdx.x) * 256 + threadIdx.x) / TPR;
-  const bool live = pr < npairs;
+  int64_t total = npairs;
+  if constexpr (LIST) {
+    total = int64_t(*rv.nlive) * npairs;
+    if (int64_t(blockIdx.x) * (256 / TPR) >= total) return;  // the block's slots are all empty
+  }
+  const bool live = pr < total;
   const int64_t row0 = live ? pr * 2 : 0;  // row pairs never straddle images: rows per image are even
   const float* a0 = A + row0 * W;
+  if constexpr (LIST) {
+    const int64_t slot = live ? pr / npairs : 0;
+    a0 = run_state(rv, rv.list[slot], false) + (row0 - slot * npairs * 2) * W;
+  }
   const float* a1 = a0 + W;
   float2 x[P];
 #pragma unroll
@@ -254,6 +306,11 @@ struct LineArgs {
   float2* dspec;
   int C_fa;
   int acc_cin[LX_MAX_K];
+  // batched run (LIST kernels): nlines is the lines of one slot's slot_imgs images; the spectra
+  // of a slot's lines are those of world list[slot]
+  const int32_t* list;
+  const int32_t* nlive;
+  int xcd;  // blocks in XCD-contiguous order (xcd_block) or in launch order
 };
 
 LineArgs line_args() {
@@ -262,28 +319,36 @@ LineArgs line_args() {
   a.WH = 0; a.A = 1; a.a_stride = 0; a.st = 0; a.img_elems = 0; a.C = 1; a.K = 0;
   for (int k = 0; k < LX_MAX_K; ++k) { a.cin[k] = -1; a.acc_cin[k] = 0; }
   a.mode = 0; a.conjk = 0; a.FA = nullptr; a.FAin = nullptr; a.dspec = nullptr; a.C_fa = 1;
+  a.list = nullptr; a.nlive = nullptr; a.xcd = 1;
   return a;
 }
 
 // One launch per mode (MODE 0: conv, 1: forward in place, 2: inverse), so that each keeps
 // only its own registers.  Offsets inside an image are 32-bit; loads are issued in batches
 // of 8 (sched_barrier) so that addresses and values are not all held at once.
-template <int L, int MODE>
+template <int L, int MODE, bool LIST = false, int T = line_tpr(L, MODE)>
 __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
-  using LC = Lanes<L, line_tpr(L, MODE)>;
+  using LC = Lanes<L, T>;
   constexpr int TPR = LC::TPR, P = LC::P;
   __shared__ __align__(16) float2 xch_all[LC::XL ? LC::XCH : 2];
   const int lane = threadIdx.x & 63;
   const int t = threadIdx.x % TPR;
   float2* xch = xch_all + (LC::XL ? (threadIdx.x / TPR) * (TPR * LC::XRS) : 0);
-  const int64_t lid = (xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x) / TPR;
-  const bool live = lid < a.nlines;
+  const int64_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : int64_t(blockIdx.x);
+  const int64_t lid = (blk * 256 + threadIdx.x) / TPR;
+  int64_t nlines = a.nlines;
+  if constexpr (LIST) {
+    nlines *= *a.nlive;
+    if (blk * (256 / TPR) >= nlines) return;  // the block's slots are all empty
+  }
+  const bool live = lid < nlines;
   const int64_t per_img = int64_t(a.A) * a.WH;
   const int64_t img = live ? lid / per_img : 0;
   const int rem = live ? int(lid - img * per_img) : 0;
   const int aa = rem / a.WH;
   const int off = aa * int(a.a_stride) + (rem - aa * a.WH);  // line base inside its image
   const int64_t b = img / a.C;
+  const int64_t bw = LIST && MODE == 0 ? int64_t(a.list[b]) : b;  // the world whose spectra apply
   const int c = int(img - b * a.C);
   const int st = int(a.st);
   const float2* src = a.S + img * a.img_elems + off;
@@ -338,7 +403,7 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
         if ((e & 7) == 7) __builtin_amdgcn_sched_barrier(0);
       }
     }
-    const float2* kk = a.Kh + b * a.k_bstride + int64_t(k) * a.img_elems + off;
+    const float2* kk = a.Kh + bw * a.k_bstride + int64_t(k) * a.img_elems + off;
 #pragma unroll
     for (int e = 0; e < P; ++e) {
       const float2 kv = kk[freq_of<L, TPR>(t, e) * st];
@@ -372,7 +437,7 @@ constexpr bool conv_staged(int L) { return L == 256; }
 // steps 1.63 -> 2.05 ms staged)
 constexpr unsigned kStagedMinBlocks = 1024;
 
-template <int L>
+template <int L, bool LIST = false>
 __global__ __launch_bounds__(256) void conv_staged_kernel(LineArgs a) {
   using LC = Lanes<L, line_tpr(L, 0)>;
   constexpr int TPR = LC::TPR, P = LC::P, LB = 256 / TPR, GRS = TPR * LC::XRS;
@@ -382,22 +447,28 @@ __global__ __launch_bounds__(256) void conv_staged_kernel(LineArgs a) {
   const int tid = threadIdx.x, lane = tid & 63, t = tid % TPR, g = tid / TPR;
   float2* xch = xch_all + g * GRS;
   auto srow = [](int r) { return r * (GRS + 2); };  // stage row r: r GRS + 2 r
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : int64_t(blockIdx.x);
+  int64_t nlines = a.nlines;
+  if constexpr (LIST) {
+    nlines *= *a.nlive;
+    if (blk * LB >= nlines) return;  // the block's slots are all empty (uniform: before any barrier)
+  }
   const int64_t per_img = int64_t(a.A) * a.WH;
   const int st = int(a.st);
   // the line this thread stages (ls) and its first point (pl)
   const int ls = tid % LB, pl = tid / LB;
   const int64_t lid_s = blk * LB + ls;
-  const bool live_s = lid_s < a.nlines;
+  const bool live_s = lid_s < nlines;
   const int64_t img_s = live_s ? lid_s / per_img : 0;
   const int rem_s = live_s ? int(lid_s - img_s * per_img) : 0;
   const int aa_s = rem_s / a.WH;
   const int off_s = aa_s * int(a.a_stride) + (rem_s - aa_s * a.WH);
   const int64_t b_s = img_s / a.C;
+  const int64_t bw_s = LIST ? int64_t(a.list[b_s]) : b_s;  // the world whose spectra apply
   const int c_s = int(img_s - b_s * a.C);
   // the line this thread's group transforms
   const int64_t lid = blk * LB + g;
-  const bool live = lid < a.nlines;
+  const bool live = lid < nlines;
   const int64_t img = live ? lid / per_img : 0;
   const int c = int(img - (img / a.C) * a.C);
   float2* stage = xch_all;
@@ -427,7 +498,7 @@ __global__ __launch_bounds__(256) void conv_staged_kernel(LineArgs a) {
     const bool use_s = live_s && c_s == ck;
     __syncthreads();  // the previous transform or store step is done with the scratch
     {
-      const float2* kk = a.Kh + b_s * a.k_bstride + int64_t(k) * a.img_elems + off_s;
+      const float2* kk = a.Kh + bw_s * a.k_bstride + int64_t(k) * a.img_elems + off_s;
 #pragma unroll
       for (int q = 0; q < P; ++q) {
         const int pt = pl + TPR * q;
@@ -489,13 +560,19 @@ struct RowArgs {
   float* part;
   const lx_stat_carry* carry;
   int ndim, d0, d1;  // rows = d0 d1 (d0 = 1 for 2-D worlds)
+  // batched run (LIST kernels): npairs is the pairs of one slot; state from / to rv's buffers, the
+  // carry is rv.carry; with hist_B > 0, field and pot (and cells_hist, when set) are the run's
+  // histories [max_steps][hist_B][C or K][dims], written at the world's step
+  RunView rv;
+  float* cells_hist;
+  int64_t hist_B;
 };
 
 // UNI: every wave's row pairs lie in one image (the host checks that the pairs per image are a
 // multiple of the pairs per wave), so the world's weights and growth parameters are the same for
 // the whole wave: the wave copies them into LDS once instead of loading them from global memory
 // in each kernel iteration, and its branches on them are uniform.
-template <int W, bool STATS, bool UNI>
+template <int W, bool STATS, bool UNI, bool LIST = false>
 __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
   using LC = Lanes<W>;
   constexpr int TPR = LC::TPR, P = LC::P, WH = W / 2 + 1;
@@ -505,28 +582,38 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
   const int t = threadIdx.x % TPR;
   float2* xch = xch_all + (threadIdx.x / TPR) * (TPR * LC::XRS);
   const int64_t pr = (int64_t(blockIdx.x) * 256 + threadIdx.x) / TPR;
-  const bool live = pr < a.npairs;
+  int64_t npairs = a.npairs;
+  if constexpr (LIST) {
+    npairs *= *a.rv.nlive;
+    if (int64_t(blockIdx.x) * (256 / TPR) >= npairs) return;  // the block's slots are all empty
+  }
+  const bool live = pr < npairs;
   const int64_t half_rows = a.rows / 2;
-  const int64_t img = live ? pr / half_rows : 0;
+  const int64_t img = live ? pr / half_rows : 0;  // image of the pair (LIST: of its slot)
   const int64_t r2 = live ? (pr - img * half_rows) * 2 : 0;
-  int64_t b = img / a.C;
+  int64_t b = img / a.C;  // world (LIST: slot)
   int c = int(img - b * a.C);
+  int64_t bw = b;         // world
+  if constexpr (LIST) bw = a.rv.list[b];
   const float inv_n = 1.f / float(a.rows * W);
-  const float* pw = a.sel ? nullptr : a.w + b * a.w_bs;
-  const float* pg = a.sel ? nullptr : a.gf + b * a.gf_bs;
+  const float* pw = a.sel ? nullptr : a.w + bw * a.w_bs;
+  const float* pg = a.sel ? nullptr : a.gf + bw * a.gf_bs;
   if constexpr (UNI) {
     b = __builtin_amdgcn_readfirstlane(int(b));
+    bw = LIST ? int64_t(__builtin_amdgcn_readfirstlane(int(bw))) : b;
     c = __builtin_amdgcn_readfirstlane(c);
     if (!a.sel) {
       float* ws = wpar[threadIdx.x >> 6];
       const int nw = a.C * a.K;
       for (int i = lane; i < nw + 2 * a.K; i += 64)
-        ws[i] = i < nw ? a.w[b * a.w_bs + i] : a.gf[b * a.gf_bs + (i - nw)];
+        ws[i] = i < nw ? a.w[bw * a.w_bs + i] : a.gf[bw * a.gf_bs + (i - nw)];
       wave_lds_fence();
       pw = ws;
       pg = ws + nw;
     }
   }
+  // the world's step (LIST: history rows and the state buffers follow it)
+  const int64_t tw = LIST ? int64_t(a.rv.carry[bw].t) : 0;
   float G0[P], G1[P];
 #pragma unroll
   for (int m = 0; m < P; ++m) G0[m] = G1[m] = 0.f;
@@ -578,7 +665,9 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
     fft_group<W, true>(x, t, lane, xch);
     const float gm = a.sel ? 0.f : pg[2 * k], gs = a.sel ? 1.f : pg[2 * k + 1];
     const float inv_s = 1.f / gs, inv_9s2 = 1.f / (9.f * (gs * gs));
-    float* po0 = want_pot ? a.pot + ((b * a.K + k) * a.rows + r2) * W : nullptr;
+    float* po0 = !want_pot ? nullptr
+                 : LIST ? a.pot + (((tw * a.hist_B + bw) * a.K + k) * a.rows + r2) * W
+                        : a.pot + ((b * a.K + k) * a.rows + r2) * W;
     // the growth function is chosen once per kernel, outside the loop over the lane's points
     // (KIND < 0: selection mode, identity)
     auto accumulate = [&](auto kind_c) {
@@ -625,14 +714,26 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
     }
   }
   if (!a.update || !live) return;
-  const float dtw = a.sel ? 1.f : a.dt[b * a.dt_bs];
+  const float dtw = a.sel ? 1.f : a.dt[bw * a.dt_bs];
   const int64_t off = (img * a.rows + r2) * int64_t(W);
+  const float* Ain = a.A + off;
+  float* Aout = a.A_out + off;
+  float* fld = a.field ? a.field + off : nullptr;
+  float* chist = nullptr;
+  if constexpr (LIST) {
+    const int64_t in_world = (int64_t(c) * a.rows + r2) * W;
+    Ain = run_state(a.rv, bw, false) + in_world;
+    Aout = run_state(a.rv, bw, true) + in_world;
+    const int64_t hist_off = (tw * a.hist_B + bw) * a.rv.world_elems + in_world;
+    fld = a.field ? a.field + hist_off : nullptr;
+    chist = a.cells_hist ? a.cells_hist + hist_off : nullptr;
+  }
   // statistics: per row sums of A, A z, A z^2, G+, G+ z (z the re-centred column) and counts
   float sAr0 = 0.f, sAr1 = 0.f, sAz = 0.f, sAzz = 0.f, sGr0 = 0.f, sGr1 = 0.f, sGz = 0.f, nA = 0.f, nG = 0.f;
   int sh[3] = {0, 0, 0};  // carried shift reduced into [0, size) along (d0, d1, W)
   if (STATS) {
     auto modp = [](int v, int m) { v %= m; return v < 0 ? v + m : v; };
-    const lx_carry& cc = a.carry[b].c;
+    const lx_carry& cc = LIST ? a.rv.carry[bw] : a.carry[b].c;
     if (a.ndim == 3) sh[0] = modp(cc.shift[0], a.d0);
     sh[1] = modp(cc.shift[a.ndim - 2], a.d1);
     sh[2] = modp(cc.shift[a.ndim - 1], W);
@@ -642,8 +743,9 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
     const int col = TPR * m + t;
     float g0 = G0[m], g1 = G1[m];
     if (a.average) { g0 = (g0 / den); g1 = (g1 / den); }
-    if (a.field) { a.field[off + col] = g0; a.field[off + W + col] = g1; }
-    const float v0 = a.A[off + col], v1 = a.A[off + W + col];
+    if (fld) { fld[col] = g0; fld[W + col] = g1; }
+    const float v0 = Ain[col], v1 = Ain[W + col];
+    if (LIST && chist) { chist[col] = v0; chist[W + col] = v1; }
     if (STATS) {
       int zc = col - sh[2];
       zc += zc < 0 ? W : 0;
@@ -672,8 +774,8 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
         n1 = n1 < 0.f ? 0.f : (n1 > 1.f ? 1.f : n1);
       }
     }
-    a.A_out[off + col] = n0;
-    a.A_out[off + W + col] = n1;
+    Aout[col] = n0;
+    Aout[W + col] = n1;
   }
   if constexpr (STATS) {
     // re-centred row coordinates of the pair's two rows: (x, y) in 3-D, y in 2-D
@@ -723,34 +825,154 @@ int launch_rows_fwd(const Geom& g, const float* A, float2* S, int64_t nimg, hipS
   return dispatch_len(g.W, [&](auto w) {
     constexpr int W = decltype(w)::value;
     if (int rc = ensure_tw<W>(st)) return rc;
-    hipLaunchKernelGGL(rows_fwd_kernel<W>, dim3(blocks_for(npairs * Lanes<W>::TPR)), dim3(256), 0, st, A, S,
-                       npairs);
+    hipLaunchKernelGGL((rows_fwd_kernel<W, false>), dim3(blocks_for(npairs * Lanes<W>::TPR)), dim3(256), 0, st, A,
+                       S, npairs, RunView{nullptr, nullptr, nullptr, nullptr, nullptr, 0});
     return launched("rows_fwd");
   });
+}
+
+// ---------------------------------------------------------------- line pass variants and their choice
+// A line pass has schedule variants that compute the same transforms: the convolution pass at 256
+// points staged through LDS or in the lane groups' layout, blocks in XCD-contiguous or launch order,
+// and at 96 points 4 or 8 lanes per line.  The fixed rule (tuning off) is the measured default:
+// staged from 1024 blocks up, XCD order, 4 lanes for the forward / inverse passes at 96 points.
+struct LineVariant {
+  int staged, xcd, tpr;
+};
+
+// launches one variant of a line pass over `lines` lines (LIST: the slots' lines of a batched run)
+template <bool LIST>
+int launch_lines_v(LineArgs la, int L, int64_t lines, LineVariant v, hipStream_t st) {
+  la.xcd = v.xcd;
+  return dispatch_len(L, [&](auto l) {
+    constexpr int LL = decltype(l)::value;
+    auto go = [&](auto tc) {
+      constexpr int T = decltype(tc)::value;
+      const dim3 grid(blocks_for(lines * T));
+      if (la.mode == 0) {
+        if constexpr (conv_staged(LL) && T == line_tpr(LL, 0)) {
+          if (v.staged) {
+            hipLaunchKernelGGL((conv_staged_kernel<LL, LIST>), grid, dim3(256), 0, st, la);
+            return launched("conv (staged)");
+          }
+        }
+        hipLaunchKernelGGL((lines_kernel<LL, 0, LIST, T>), grid, dim3(256), 0, st, la);
+      } else if (la.mode == 1) {
+        hipLaunchKernelGGL((lines_kernel<LL, 1, LIST, T>), grid, dim3(256), 0, st, la);
+      } else {
+        hipLaunchKernelGGL((lines_kernel<LL, 2, LIST, T>), grid, dim3(256), 0, st, la);
+      }
+      return launched("lines");
+    };
+    if constexpr (LL == 96) {
+      if (v.tpr == 4) return go(IntC<4>{});
+    }
+    return go(IntC<row_tpr(LL)>{});
+  });
+}
+
+bool staged_eligible(const LineArgs& la, int L) { return la.mode == 0 && L == 256 && !la.FA && !la.dspec; }
+
+LineVariant default_variant(const LineArgs& la, int L, int64_t lines) {
+  LineVariant v;
+  v.tpr = L == 96 ? (la.mode == 0 ? 8 : 4) : row_tpr(L);
+  v.staged = staged_eligible(la, L) && blocks_for(lines * v.tpr) >= kStagedMinBlocks;
+  v.xcd = 1;
+  return v;
+}
+
+void lines_key(char* key, size_t n, const LineArgs& la, int L, int64_t lines, bool list) {
+  snprintf(key, n, "L=%d:mode=%d:lines=%lld:K=%d:%s", L, la.mode, (long long)lines, la.mode == 0 ? la.K : 0,
+           list ? "list" : "plain");
+}
+
+// The variant of a line pass: forced knobs, then a recorded decision, then (tuning on) the fastest
+// candidate timed by run(v) -- which launches v on buffers it may overwrite -- then the fixed rule.
+template <class Run>
+LineVariant choose_lines(const LineArgs& la, int L, int64_t lines, bool list, bool can_time, hipStream_t st,
+                         Run&& run) {
+  char key[160];
+  lines_key(key, sizeof(key), la, L, lines, list);
+  LineVariant d = default_variant(la, L, lines), v = d;
+  int fs = -1, fx = -1, ft = -1, val = 0;
+  const bool hs = lx_tune_lookup("conv_staged", key, &val) != 0;
+  if (hs) fs = val;
+  const bool hx = lx_tune_lookup("line_xcd", key, &val) != 0;
+  if (hx) fx = val;
+  const bool ht = lx_tune_lookup("lines96_tpr", key, &val) != 0;
+  if (ht) ft = val;
+  if (hs) v.staged = staged_eligible(la, L) && fs != 0;
+  if (hx) v.xcd = fx != 0;
+  if (ht && L == 96) v.tpr = ft == 4 ? 4 : 8;
+  const bool all_known = (hs || !staged_eligible(la, L)) && hx && (ht || L != 96);
+  if (all_known || !can_time || !lx_tune_should_time(st)) return v;
+  // time every candidate the known values leave open
+  LineVariant best = v;
+  float best_ms = -1.f;
+  for (int s = 0; s <= 1; ++s) {
+    if (s && !staged_eligible(la, L)) continue;
+    if (hs && s != v.staged) continue;
+    for (int x = 0; x <= 1; ++x) {
+      if (hx && x != v.xcd) continue;
+      for (int t : {4, 8, 16}) {
+        if (L == 96 ? (t != 4 && t != 8) : t != row_tpr(L)) continue;
+        if (ht && L == 96 && t != v.tpr) continue;
+        if (s && t != line_tpr(L, 0)) continue;
+        LineVariant c{s, x, t};
+        const float ms = lx_time_ms(st, 3, [&] { return run(c); });
+        if (ms >= 0.f && (best_ms < 0.f || ms < best_ms)) {
+          best_ms = ms;
+          best = c;
+        }
+      }
+    }
+  }
+  if (best_ms < 0.f) return d;
+  if (staged_eligible(la, L)) lx_tune_record("conv_staged", key, best.staged);
+  lx_tune_record("line_xcd", key, best.xcd);
+  if (L == 96) lx_tune_record("lines96_tpr", key, best.tpr);
+  return best;
 }
 
 // la: mode-specific fields filled by the caller; geometry of `axis` and the line count set here
 int launch_lines(const Geom& g, int axis, LineArgs la, int64_t nimg, hipStream_t st) {
   la.WH = g.WH; la.A = g.A[axis]; la.a_stride = g.a_stride[axis]; la.st = g.st[axis]; la.img_elems = g.half;
   la.nlines = nimg * g.A[axis] * int64_t(g.WH);
-  return dispatch_len(g.L[axis], [&](auto l) {
-    constexpr int L = decltype(l)::value;
-    if (int rc = ensure_tw<L>(st)) return rc;
-    const int tpr = line_tpr(L, la.mode);
-    const dim3 grid(blocks_for(la.nlines * tpr));
-    if (la.mode == 0) {
-      bool staged = false;
-      if constexpr (conv_staged(L)) {
-        if (!la.FA && !la.dspec && grid.x >= kStagedMinBlocks) {
-          hipLaunchKernelGGL(conv_staged_kernel<L>, grid, dim3(256), 0, st, la);
-          staged = true;
-        }
+  const int L = g.L[axis];
+  if (int rc = dispatch_len(L, [&](auto l) { return ensure_tw<decltype(l)::value>(st); })) return rc;
+  // timing is safe where the pass can run again on its own inputs: a convolution into another
+  // buffer with no accumulated kernel gradient, an out-of-place inverse; in-place passes are timed
+  // on a copy of their images
+  const bool in_place = la.mode == 1 || la.S == la.U;
+  const bool idem = !in_place && !la.dspec;
+  float2* tmp = nullptr;
+  auto run = [&](LineVariant v) {
+    LineArgs t = la;
+    if (tmp) { t.S = tmp; t.U = tmp; }
+    return launch_lines_v<false>(t, L, la.nlines, v, st);
+  };
+  const bool can_time = idem || (in_place && la.mode != 0);
+  if (can_time && !idem && lx_tune_should_time(st)) {
+    const size_t bytes = size_t(nimg) * size_t(g.half) * sizeof(float2);
+    char key[160];
+    lines_key(key, sizeof(key), la, L, la.nlines, false);
+    int val = 0;
+    const bool known = lx_tune_lookup("line_xcd", key, &val) && (L != 96 || lx_tune_lookup("lines96_tpr", key, &val));
+    if (!known && hipMalloc(reinterpret_cast<void**>(&tmp), bytes) == hipSuccess) {
+      if (hipMemcpyAsync(tmp, la.S, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+        hipFree(tmp);
+        tmp = nullptr;
       }
-      if (!staged) hipLaunchKernelGGL((lines_kernel<L, 0>), grid, dim3(256), 0, st, la);
-    } else if (la.mode == 1) hipLaunchKernelGGL((lines_kernel<L, 1>), grid, dim3(256), 0, st, la);
-    else hipLaunchKernelGGL((lines_kernel<L, 2>), grid, dim3(256), 0, st, la);
-    return launched("lines");
-  });
+    } else {
+      tmp = nullptr;
+    }
+  }
+  const LineVariant v = choose_lines(la, L, la.nlines, false, can_time && (idem || tmp), st, run);
+  if (tmp) {
+    hipStreamSynchronize(st);
+    hipFree(tmp);
+  }
+  return launch_lines_v<false>(la, L, la.nlines, v, st);
 }
 
 int launch_rows_inv(const Geom& g, RowArgs ra, int64_t nimg, hipStream_t st) {
@@ -783,6 +1005,8 @@ RowArgs row_args(const lx_rule* rule) {
   }
   ra.state_fn = rule->state_fn; ra.average = rule->average; ra.update = 0; ra.sel = 0;
   ra.part = nullptr; ra.carry = nullptr;
+  ra.rv = RunView{nullptr, nullptr, nullptr, nullptr, nullptr, 0};
+  ra.cells_hist = nullptr; ra.hist_B = 0;
   ra.ndim = rule->ndim;
   ra.d0 = rule->ndim == 3 ? rule->dims[0] : 1;
   ra.d1 = rule->dims[rule->ndim - 2];
@@ -1058,4 +1282,450 @@ extern "C" int lx_adjoint_potential(const lx_rule* rule, int64_t B, const void* 
   ra.U = X; ra.A = lambda_d; ra.A_out = lambda_d;  // lambda_c += sum over kernels fed by c (in place, per element)
   ra.update = 1; ra.sel = 1; ra.average = 0; ra.state_fn = LX_STATE_SIMPLE;
   return launch_rows_inv(g, ra, B * rule->C, st);
+}
+
+// ====================================================================== lx_run, multi-pass path
+// leniax/runner.py:167-215 (run_scan_mem_optimized) and :119-164 (run_scan) for every rule the
+// step passes take, with the per-step statistics, the stop heuristics evaluated on the device every
+// step (leniax/statistics.py:134-205) and the retirement of worlds at uneven times.
+//
+// A world that stops frees its slot; a device-side control kernel compacts the live list after each
+// step and refills the free slots from a queue of worlds not yet started, so that at most `slots`
+// worlds are in flight (their S, U and record scratch) while all B are resident.  The step's
+// kernels run over the live list (the LIST variants above); blocks of empty slots return at once.
+// G steps are captured as one HIP graph and replayed; the host reads the live count after each
+// replay (one replay of lag) and, once the queue is drained and the count falls, replays a graph
+// captured for fewer slots (a power-of-two fraction of `slots`), whose grids are that much smaller.
+namespace {
+
+// steps per captured graph (even: step i of a graph reads list i & 1).  The fixed rule: 8 steps when
+// a step moves more than 2^22 cells (its launches are a small part of it), 32 below (small worlds
+// or batches, where the graph's launches are what it saves); the knob "graph_steps" overrides it.
+int steps_per_graph(int64_t cells_in_flight) {
+  int v = 0;
+  if (lx_tune_lookup("graph_steps", "run", &v) && v > 0) return v + (v & 1);
+  return cells_in_flight > (int64_t(1) << 22) ? 8 : 32;
+}
+
+struct RunCtl {
+  int32_t* list0;
+  int32_t* list1;
+  int32_t* nlive;  // [2]: live slots of list0, list1
+  int32_t* head;   // next world of the queue
+  int32_t* done;   // [slots]: the slot's world stopped this step
+  lx_carry* carry; // [B]
+};
+
+struct PassesLayout {
+  size_t X1, carry, work, list0, list1, done, ctl, total;
+  size_t S, U, part;  // inside work
+};
+
+int64_t cells_per_world(const lx_rule* r) {
+  int64_t n = 1;
+  for (int i = 0; i < r->ndim; ++i) n *= r->dims[i];
+  return n;
+}
+
+PassesLayout passes_layout(const lx_rule* r, const Geom& g, int64_t B, int64_t slots) {
+  auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+  PassesLayout l;
+  size_t o = 0;
+  l.X1 = o;    o = up(o + size_t(B) * r->C * cells_per_world(r) * sizeof(float));
+  l.carry = o; o = up(o + size_t(B) * sizeof(lx_carry));
+  l.work = o;
+  l.S = o;     o = up(o + size_t(slots) * r->C * g.half * sizeof(float2));
+  l.U = o;     o = up(o + size_t(slots) * r->K * g.half * sizeof(float2));
+  l.part = o;  o = up(o + size_t(slots) * r->C * (g.rows / 2) * LX_PAIR_REC * sizeof(float));
+  l.list0 = o; o = up(o + size_t(slots) * 4);
+  l.list1 = o; o = up(o + size_t(slots) * 4);
+  l.done = o;  o = up(o + size_t(slots) * 4);
+  l.ctl = o;   o = up(o + 64);
+  l.total = o;
+  return l;
+}
+
+// every world's carry cleared; the first slots take worlds 0, 1, ... and the queue starts after them
+__global__ void run_start_kernel(RunCtl c, int64_t B, int slots) {
+  const int64_t n = B > slots ? B : slots;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    if (i < B) {
+      lx_carry z;
+      for (int d = 0; d < 3; ++d) { z.shift[d] = 0; z.centroid[d] = 0.f; }
+      z.angle = 0.f;
+      for (int ch = 0; ch < LX_MAX_C; ++ch) z.init_cm[ch] = 0.f;
+      z.prev_mass = z.prev_sign = 0.f;
+      z.mono = z.vol = z.n = z.t = z.pad = 0;
+      z.cont = 1;
+      c.carry[i] = z;
+    }
+    if (i < slots) c.list0[i] = int32_t(i < B ? i : 0);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int32_t first = int32_t(B < slots ? B : slots);
+    c.nlive[0] = first;
+    c.nlive[1] = 0;
+    *c.head = first;
+  }
+}
+
+struct RunStatArgs {
+  RunCtl c;
+  int par;             // the list this step reads
+  const float* part;   // [slots][C][pairs] row-pair records
+  int64_t pairs;
+  int C, ndim;
+  int32_t dims[3];
+  float R2, R, dt;
+  int stop_policy, min_steps, max_steps;
+  float cells_total;
+  float* stats;        // [B][max_steps][11 + C] or NULL
+  int32_t* n_out;      // [B] or NULL
+  int32_t* steps_out;  // [B] or NULL
+};
+
+// One block per slot: the step's statistics of the slot's world (leniax/statistics.py:64-126), its
+// carry, the heuristics and the stop decision (a stop, or the world's last step, frees the slot).
+__global__ __launch_bounds__(kPairThreads) void run_stats_kernel(RunStatArgs a) {
+  const int slot = blockIdx.x;
+  const int32_t* list = a.par ? a.c.list1 : a.c.list0;
+  if (slot >= a.c.nlive[a.par]) return;  // block-uniform
+  const int64_t b = list[slot];
+  const lx_sums s = lx_block_pair_sums(a.part + int64_t(slot) * a.C * a.pairs * LX_PAIR_REC, a.C, a.pairs);
+  if (threadIdx.x != 0) return;
+  lx_carry cr = a.c.carry[b];
+  float* out = a.stats ? a.stats + (b * a.max_steps + cr.t) * (LX_NSTAT_BASE + a.C) : nullptr;
+  const int stop = lx_stats_finish(s, cr, a.ndim, a.dims, a.C, a.R2, a.R, a.dt, a.stop_policy, a.min_steps,
+                                   a.cells_total, out);
+  const int done = stop || cr.t >= a.max_steps;
+  a.c.carry[b] = cr;
+  a.c.done[slot] = done;
+  if (done) {
+    if (a.n_out) a.n_out[b] = cr.n;
+    if (a.steps_out) a.steps_out[b] = cr.t;
+  }
+}
+
+// One block: the next step's live list, the slots whose world goes on (in order) and then worlds
+// from the queue while slots are free.
+constexpr int kCtlThreads = 1024;
+__global__ __launch_bounds__(kCtlThreads) void run_control_kernel(RunCtl c, int par, int slots, int64_t B) {
+  __shared__ int scan[kCtlThreads];
+  __shared__ int64_t head_s;
+  const int tid = threadIdx.x;
+  const int32_t* in = par ? c.list1 : c.list0;
+  int32_t* out = par ? c.list0 : c.list1;
+  const int n = c.nlive[par];
+  const int per = (n + kCtlThreads - 1) / kCtlThreads;
+  const int s0 = min(tid * per, n), s1 = min(s0 + per, n);
+  int cnt = 0;
+  for (int s = s0; s < s1; ++s) cnt += c.done[s] ? 0 : 1;
+  scan[tid] = cnt;
+  if (tid == 0) head_s = *c.head;
+  __syncthreads();
+  for (int off = 1; off < kCtlThreads; off <<= 1) {  // inclusive scan of the survivor counts
+    const int v = tid >= off ? scan[tid - off] : 0;
+    __syncthreads();
+    scan[tid] += v;
+    __syncthreads();
+  }
+  int o = scan[tid] - cnt;
+  const int kept = scan[kCtlThreads - 1];
+  for (int s = s0; s < s1; ++s)
+    if (!c.done[s]) out[o++] = in[s];
+  const int64_t h = head_s;
+  const int64_t room = slots - kept, left = B - h;
+  const int take = int(room < left ? room : left);
+  for (int i = tid; i < take; i += kCtlThreads) out[kept + i] = int32_t(h + i);
+  if (tid == 0) {
+    *c.head = int32_t(h + take);
+    c.nlive[par ^ 1] = kept + take;
+  }
+}
+
+__global__ void zero_i32_kernel(int32_t* p, int64_t n) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// after the run: worlds that ran an odd number of steps hold their final state in X1
+__global__ void run_finish_kernel(const lx_carry* __restrict__ carry, const float* __restrict__ X1,
+                                  float* __restrict__ X0, int64_t B, int64_t elems) {
+  const int64_t total = B * elems;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x)
+    if (carry[i / elems].t & 1) X0[i] = X1[i];
+}
+
+struct PassesRun {
+  const lx_rule* rule;
+  const lx_run_args* a;
+  Geom g;
+  RunCtl ctl;
+  RunView rv[2];       // per list parity
+  float2* S;
+  float2* U;
+  float* part;
+  int slots;
+  int gsteps;  // steps per graph
+};
+
+// A line pass of the step over the live list, grids sized for `bucket` slots.  With `tune`, only the
+// choice of its variant is made (timing the candidates, which write nothing but the slots' scratch
+// spectra), outside any capture; otherwise the chosen variant is launched.
+int run_lines(const PassesRun& r, const RunView& rv, int bucket, int axis, LineArgs la, int slot_imgs, bool tune,
+              hipStream_t st) {
+  const Geom& g = r.g;
+  la.WH = g.WH; la.A = g.A[axis]; la.a_stride = g.a_stride[axis]; la.st = g.st[axis]; la.img_elems = g.half;
+  la.nlines = int64_t(slot_imgs) * g.A[axis] * g.WH;  // per slot: the kernels scale it by the live count
+  la.list = rv.list; la.nlive = rv.nlive;
+  const int L = g.L[axis];
+  const int64_t lines = la.nlines * bucket;
+  const LineVariant v = choose_lines(la, L, lines, true, tune, st,
+                                     [&](LineVariant c) { return launch_lines_v<true>(la, L, lines, c, st); });
+  return tune ? LX_OK : launch_lines_v<true>(la, L, lines, v, st);
+}
+
+// the variants of the step's line passes for `bucket` slots, chosen before a capture
+int tune_run_lines(const PassesRun& r, int bucket, hipStream_t st) {
+  const lx_rule* rule = r.rule;
+  const int C = rule->C, K = rule->K;
+  if (!lx_tune_should_time(st)) return LX_OK;
+  int rc = LX_OK;
+  if (r.g.naxes == 2) {
+    LineArgs la = line_args();
+    la.S = r.S; la.mode = 1;
+    rc = run_lines(r, r.rv[0], bucket, 1, la, C, true, st);
+  }
+  if (!rc) {
+    LineArgs la = line_args();
+    la.S = r.S; la.U = r.U; la.Kh = static_cast<const float2*>(r.a->K_half_d); la.k_bstride = r.a->k_bstride;
+    la.C = C; la.K = K;
+    for (int k = 0; k < LX_MAX_K; ++k) la.cin[k] = k < K ? rule->cin[k] : -1;
+    rc = run_lines(r, r.rv[0], bucket, 0, la, C, true, st);
+  }
+  if (!rc && r.g.naxes == 2) {
+    LineArgs la = line_args();
+    la.S = r.U; la.U = r.U; la.mode = 2;
+    rc = run_lines(r, r.rv[0], bucket, 1, la, K, true, st);
+  }
+  return rc;
+}
+
+// one step over the live list of parity `par`, grids sized for `bucket` slots
+int enqueue_step(const PassesRun& r, int par, int bucket, hipStream_t st) {
+  const lx_rule* rule = r.rule;
+  const lx_run_args* a = r.a;
+  const Geom& g = r.g;
+  const int C = rule->C, K = rule->K;
+  const RunView& rv = r.rv[par];
+  const int64_t slot_pairs = int64_t(C) * (g.rows / 2);
+  int rc = dispatch_len(g.W, [&](auto w) {
+    constexpr int W = decltype(w)::value;
+    hipLaunchKernelGGL((rows_fwd_kernel<W, true>), dim3(blocks_for(bucket * slot_pairs * Lanes<W>::TPR)), dim3(256),
+                       0, st, nullptr, r.S, slot_pairs, rv);
+    return launched("rows_fwd (run)");
+  });
+  auto lines = [&](int axis, LineArgs la, int slot_imgs) {
+    return run_lines(r, rv, bucket, axis, la, slot_imgs, false, st);
+  };
+  if (!rc && g.naxes == 2) {
+    LineArgs la = line_args();
+    la.S = r.S; la.mode = 1;
+    rc = lines(1, la, C);
+  }
+  if (!rc) {
+    LineArgs la = line_args();
+    la.S = r.S; la.U = r.U; la.Kh = static_cast<const float2*>(a->K_half_d); la.k_bstride = a->k_bstride;
+    la.C = C; la.K = K;
+    for (int k = 0; k < LX_MAX_K; ++k) la.cin[k] = k < K ? rule->cin[k] : -1;
+    rc = lines(0, la, C);
+  }
+  if (!rc && g.naxes == 2) {
+    LineArgs la = line_args();
+    la.S = r.U; la.U = r.U; la.mode = 2;
+    rc = lines(1, la, K);
+  }
+  if (rc) return rc;
+  RowArgs ra = row_args(rule);
+  ra.U = r.U; ra.gf = a->gf_d; ra.gf_bs = a->gf_bstride; ra.w = a->w_d; ra.w_bs = a->w_bstride;
+  ra.dt = a->dt_d; ra.dt_bs = a->dt_bstride;
+  ra.update = 1; ra.part = r.part;
+  ra.rv = rv;
+  ra.field = a->field_hist_d; ra.pot = a->potential_hist_d; ra.cells_hist = a->cells_hist_d; ra.hist_B = a->B;
+  ra.rows = g.rows;
+  ra.npairs = slot_pairs;
+  rc = dispatch_len(g.W, [&](auto w) {
+    constexpr int W = decltype(w)::value;
+    const dim3 grid(blocks_for(bucket * slot_pairs * Lanes<W>::TPR));
+    const bool uni = (ra.rows / 2) % (64 / Lanes<W>::TPR) == 0;
+    if (uni) hipLaunchKernelGGL((rows_inv_kernel<W, true, true, true>), grid, dim3(256), 0, st, ra);
+    else hipLaunchKernelGGL((rows_inv_kernel<W, true, false, true>), grid, dim3(256), 0, st, ra);
+    return launched("rows_inv (run)");
+  });
+  if (rc) return rc;
+  RunStatArgs sa;
+  sa.c = r.ctl; sa.par = par; sa.part = r.part; sa.pairs = g.rows / 2; sa.C = C; sa.ndim = rule->ndim;
+  for (int i = 0; i < 3; ++i) sa.dims[i] = i < rule->ndim ? rule->dims[i] : 1;
+  sa.R2 = (float)((double)a->R * (double)a->R); sa.R = a->R; sa.dt = (float)(1.0 / (double)a->T);
+  sa.stop_policy = a->stop_policy; sa.min_steps = a->min_steps; sa.max_steps = a->max_steps;
+  sa.cells_total = float(cells_per_world(rule) * C);
+  sa.stats = a->stats_d; sa.n_out = a->n_d; sa.steps_out = a->steps_d;
+  hipLaunchKernelGGL(run_stats_kernel, dim3(unsigned(bucket)), dim3(kPairThreads), 0, st, sa);
+  hipLaunchKernelGGL(run_control_kernel, dim3(1), dim3(kCtlThreads), 0, st, r.ctl, par, r.slots, a->B);
+  return launched("run statistics / control");
+}
+
+// twiddle tables of every length the step uses, built before a capture starts
+int warm_tables(const Geom& g, hipStream_t st) {
+  auto warm = [&](int L) { return dispatch_len(L, [&](auto l) { return ensure_tw<decltype(l)::value>(st); }); };
+  if (int rc = warm(g.W)) return rc;
+  for (int ax = 0; ax < g.naxes; ++ax)
+    if (int rc = warm(g.L[ax])) return rc;
+  return LX_OK;
+}
+
+hipStream_t capture_stream() {
+  static hipStream_t s[kMaxDevices] = {};
+  static std::mutex mu;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!s[dev] && hipStreamCreateWithFlags(&s[dev], hipStreamNonBlocking) != hipSuccess) s[dev] = nullptr;
+  return s[dev];
+}
+
+// pinned host slots the live counts are copied into (two per device)
+int32_t* live_count_slots() {
+  static int32_t* p[kMaxDevices] = {};
+  static std::mutex mu;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!p[dev] && hipHostMalloc(reinterpret_cast<void**>(&p[dev]), 2 * sizeof(int32_t), hipHostMallocDefault) !=
+                     hipSuccess)
+    p[dev] = nullptr;
+  return p[dev];
+}
+
+struct GraphSet {
+  std::vector<hipGraphExec_t> exec;  // by bucket index k: slots >> k
+  ~GraphSet() {
+    for (hipGraphExec_t e : exec)
+      if (e) hipGraphExecDestroy(e);
+  }
+};
+
+int capture(const PassesRun& r, int bucket, hipGraphExec_t* out, hipStream_t st) {
+  if (int rc = tune_run_lines(r, bucket, st)) return rc;  // on the caller's stream, before the capture
+  hipStream_t cs = capture_stream();
+  if (!cs) return lx_fail(LX_EDEVICE, "lx_run: no capture stream");
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return lx_fail(LX_EDEVICE, "lx_run: hipStreamBeginCapture failed");
+  int rc = LX_OK;
+  for (int i = 0; i < r.gsteps && !rc; ++i) rc = enqueue_step(r, i & 1, bucket, cs);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(cs, &graph);
+  if (rc) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess || !graph) return lx_fail(LX_EDEVICE, "lx_run: capture failed: %s", hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (ei != hipSuccess) return lx_fail(LX_EDEVICE, "lx_run: hipGraphInstantiate: %s", hipGetErrorString(ei));
+  return LX_OK;
+}
+
+}  // namespace
+
+int lx_passes_scratch(const lx_rule* rule, int64_t B, int32_t slots, size_t* bytes) {
+  Geom g;
+  if (int rc = make_geom(rule->ndim, rule->dims, g)) return rc;
+  const int64_t cap = slots > 0 && slots < B ? slots : B;
+  *bytes = passes_layout(rule, g, B, cap < 1 ? 1 : cap).total;
+  return LX_OK;
+}
+
+int lx_passes_run(const lx_rule* rule, const lx_run_args* a, hipStream_t st) {
+  if (a->max_steps == 0) {  // no step: the initial states are the final ones
+    const int64_t elems = int64_t(rule->C) * cells_per_world(rule);
+    if (a->cells_d != a->cells0_d &&
+        hipMemcpyAsync(a->cells_d, a->cells0_d, size_t(a->B * elems) * sizeof(float), hipMemcpyDeviceToDevice,
+                       st) != hipSuccess)
+      return lx_fail(LX_EDEVICE, "lx_run: copy of the initial states failed");
+    if (a->n_d) hipLaunchKernelGGL(zero_i32_kernel, dim3(blocks_for(a->B)), dim3(256), 0, st, a->n_d, a->B);
+    if (a->steps_d) hipLaunchKernelGGL(zero_i32_kernel, dim3(blocks_for(a->B)), dim3(256), 0, st, a->steps_d, a->B);
+    return launched("run (no step)");
+  }
+  PassesRun r;
+  r.rule = rule;
+  r.a = a;
+  if (int rc = make_geom(rule->ndim, rule->dims, r.g)) return rc;
+  if (a->B > INT32_MAX) return lx_fail(LX_ENOTSUP, "lx_run: more than 2^31 - 1 worlds");
+  const int64_t cap = a->slots > 0 && a->slots < a->B ? a->slots : a->B;
+  if (cap > (1 << 20)) return lx_fail(LX_ENOTSUP, "lx_run: more than 2^20 worlds in flight (set slots)");
+  r.slots = int(cap);
+  r.gsteps = steps_per_graph(cap * rule->C * cells_per_world(rule));
+  const PassesLayout l = passes_layout(rule, r.g, a->B, cap);
+  char* scr = static_cast<char*>(a->scratch_d);
+  r.ctl.list0 = reinterpret_cast<int32_t*>(scr + l.list0);
+  r.ctl.list1 = reinterpret_cast<int32_t*>(scr + l.list1);
+  r.ctl.nlive = reinterpret_cast<int32_t*>(scr + l.ctl);
+  r.ctl.head = r.ctl.nlive + 2;
+  r.ctl.done = reinterpret_cast<int32_t*>(scr + l.done);
+  r.ctl.carry = reinterpret_cast<lx_carry*>(scr + l.carry);
+  r.S = reinterpret_cast<float2*>(scr + l.S);
+  r.U = reinterpret_cast<float2*>(scr + l.U);
+  r.part = reinterpret_cast<float*>(scr + l.part);
+  const int64_t elems = int64_t(rule->C) * cells_per_world(rule);
+  float* X0 = a->cells_d;
+  float* X1 = reinterpret_cast<float*>(scr + l.X1);
+  for (int p = 0; p < 2; ++p)
+    r.rv[p] = RunView{p ? r.ctl.list1 : r.ctl.list0, r.ctl.nlive + p, r.ctl.carry, X0, X1, elems};
+  int rc = warm_tables(r.g, st);
+  if (rc) return rc;
+  if (a->cells_d != a->cells0_d &&
+      hipMemcpyAsync(a->cells_d, a->cells0_d, size_t(a->B * elems) * sizeof(float), hipMemcpyDeviceToDevice, st) !=
+          hipSuccess)
+    return lx_fail(LX_EDEVICE, "lx_run: copy of the initial states failed");
+  const unsigned sgrid = blocks_for(std::max<int64_t>(a->B, cap));
+  hipLaunchKernelGGL(run_start_kernel, dim3(std::min(sgrid, 4096u)), dim3(256), 0, st, r.ctl, a->B, r.slots);
+  if ((rc = launched("run start"))) return rc;
+
+  int nb = 1;  // buckets: slots, slots / 2, ... down to 1
+  while ((r.slots >> nb) >= 1 && nb < 31) ++nb;
+  GraphSet gs;
+  gs.exec.assign(nb, nullptr);
+  int32_t* host_n = live_count_slots();
+  if (!host_n) return lx_fail(LX_EDEVICE, "lx_run: pinned host memory unavailable");
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  for (int i = 0; i < 2; ++i)
+    if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess)
+      return lx_fail(LX_EDEVICE, "lx_run: event creation failed");
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() { for (int i = 0; i < 2; ++i) if (e[i]) hipEventDestroy(e[i]); }
+  } evg{ev};
+  // a bound on the replays: list scheduling finishes within (total work) / slots + the longest world
+  const int64_t bound_steps = (a->B * int64_t(a->max_steps) + cap - 1) / cap + a->max_steps;
+  const int64_t max_graphs = bound_steps / r.gsteps + 4;
+  int64_t seen = cap;  // the live count the host last read (an upper bound of the next replays' counts)
+  for (int64_t i = 0;; ++i) {
+    if (i > max_graphs) return lx_fail(LX_EDEVICE, "lx_run: the live list did not drain (internal error)");
+    int k = 0;
+    while (k + 1 < nb && (int64_t(r.slots) >> (k + 1)) >= seen) ++k;
+    if (!gs.exec[k] && (rc = capture(r, r.slots >> k, &gs.exec[k], st))) return rc;
+    if (hipGraphLaunch(gs.exec[k], st) != hipSuccess) return lx_fail(LX_EDEVICE, "lx_run: hipGraphLaunch failed");
+    // the live count after this replay (step parity 0 again: the graph has an even number of steps)
+    if (hipMemcpyAsync(host_n + (i & 1), r.ctl.nlive, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipEventRecord(ev[i & 1], st) != hipSuccess)
+      return lx_fail(LX_EDEVICE, "lx_run: live count copy failed");
+    if (i == 0) continue;
+    if (hipEventSynchronize(ev[(i - 1) & 1]) != hipSuccess) return lx_fail(LX_EDEVICE, "lx_run: stream failed");
+    seen = host_n[(i - 1) & 1];
+    if (seen == 0) break;
+  }
+  hipLaunchKernelGGL(run_finish_kernel, dim3(std::min(blocks_for(a->B * elems), 65536u)), dim3(256), 0, st,
+                     r.ctl.carry, X1, X0, a->B, elems);
+  return launched("run finish");
 }

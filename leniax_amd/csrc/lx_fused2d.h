@@ -20,6 +20,8 @@
 // LDS: one complex buffer of N*N/2 (+padding) elements reused by both transposes, and
 // one TPR x (P + 2) exchange scratch per lane group.
 #pragma once
+#include <stdio.h>
+
 #include "lx_internal.h"
 
 namespace lx_fused {
@@ -533,27 +535,52 @@ int launch_k(const FusedParams& p, int64_t B, hipStream_t stream) {
   if (grid < 1) grid = 1;
   // counters: [0] probe queue head, [1] resume queue head, [2] survivor count
   int32_t* ctr = p.queue;
-  hipLaunchKernelGGL(zero_i32<0>, dim3(1), dim3(64), 0, stream, ctr);
-  const bool probe = p.stop_policy != LX_STOP_NONE && p.max_steps > 256;
-  FusedParams p1 = p;
-  p1.queue = ctr;
-  p1.surv_count = ctr + 2;
-  p1.resume = 0;
-  p1.limit = probe ? p.max_steps / 8 : p.max_steps;
-  hipLaunchKernelGGL((fused2d_kernel<N, GF, HIST, V1>), dim3((unsigned)grid), dim3(F2<N>::NT), 0, stream, p1);
-  if (probe) {
-    FusedParams p2 = p1;
-    p2.queue = ctr + 1;
-    p2.resume = 1;
-    p2.limit = p.max_steps;
-    hipLaunchKernelGGL((fused2d_kernel<N, GF, HIST, V1>), dim3((unsigned)grid), dim3(F2<N>::NT), 0, stream, p2);
+  // div > 0: a probe launch of max_steps / div steps per world, then the survivors resume together
+  auto launch_with = [&](int div) -> int {
+    const bool probe = div > 0;
+    hipLaunchKernelGGL(zero_i32<0>, dim3(1), dim3(64), 0, stream, ctr);
+    FusedParams p1 = p;
+    p1.queue = ctr;
+    p1.surv_count = ctr + 2;
+    p1.resume = 0;
+    p1.limit = probe ? p.max_steps / div : p.max_steps;
+    hipLaunchKernelGGL((fused2d_kernel<N, GF, HIST, V1>), dim3((unsigned)grid), dim3(F2<N>::NT), 0, stream, p1);
+    if (probe) {
+      FusedParams p2 = p1;
+      p2.queue = ctr + 1;
+      p2.resume = 1;
+      p2.limit = p.max_steps;
+      hipLaunchKernelGGL((fused2d_kernel<N, GF, HIST, V1>), dim3((unsigned)grid), dim3(F2<N>::NT), 0, stream, p2);
+    }
+    if (p.stats && B > 0)
+      hipLaunchKernelGGL(stats_pass_kernel<0>, dim3((unsigned)B), dim3(64), 0, stream, p.stats, p.steps_out, B,
+                         p.max_steps, p.R2, p.R, p.dt_stats);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? LX_OK : lx_fail(LX_EDEVICE, "fused launch failed: %s", hipGetErrorString(e));
+  };
+  // the probe (worlds per CU per residency): worth it when worlds retire at uneven times and the run
+  // is long.  Fixed rule: max_steps / 8.  Tuned: the fastest of no probe and / 4, / 8, / 16, timed on
+  // the call's own inputs when they are not overwritten by it.
+  if (p.stop_policy == LX_STOP_NONE || p.max_steps <= 256) return launch_with(0);
+  char key[128];
+  snprintf(key, sizeof(key), "N=%d:B=%lld:steps=%d:stop=%d:min=%d", N, (long long)B, p.max_steps, p.stop_policy,
+           p.min_steps);
+  int div = 8;
+  if (!lx_tune_lookup("fused_probe", key, &div)) {
+    div = 8;
+    if (p.cells != p.cells0 && lx_tune_should_time(stream)) {
+      float best = -1.f;
+      for (int c : {0, 4, 8, 16}) {
+        const float ms = lx_time_ms(stream, 2, [&] { return launch_with(c); });
+        if (ms >= 0.f && (best < 0.f || ms < best)) {
+          best = ms;
+          div = c;
+        }
+      }
+      if (best >= 0.f) lx_tune_record("fused_probe", key, div);
+    }
   }
-  if (p.stats && B > 0)
-    hipLaunchKernelGGL(stats_pass_kernel<0>, dim3((unsigned)B), dim3(64), 0, stream, p.stats, p.steps_out, B,
-                       p.max_steps, p.R2, p.R, p.dt_stats);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return lx_fail(LX_EDEVICE, "fused launch failed: %s", hipGetErrorString(e));
-  return LX_OK;
+  return launch_with(div);
 }
 
 template <int N, bool HIST, bool V1>

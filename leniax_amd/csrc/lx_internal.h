@@ -226,6 +226,80 @@ LX_DEV lx_sums lx_sums_of_record(const float* r) {
   return s;
 }
 
+// The world sums of one step from a world's row-pair records (lx_step_stats' inverse row pass:
+// C x pairs records of LX_PAIR_REC floats, channel-major), summed by a block of kPairThreads
+// threads: thread i takes records i, i + kPairThreads, ... in order, then the waves butterfly and
+// thread 0 adds the wave sums in order (deterministic).  The result is complete in thread 0.
+constexpr int kPairThreads = 256;
+LX_DEV lx_sums lx_block_pair_sums(const float* __restrict__ rec, int C, int64_t pairs) {
+  constexpr int NV = 14 + LX_MAX_C;  // the record's 14 sums, then the channel masses
+  __shared__ float red[kPairThreads / 64][NV];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  float v[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = 0.f;
+  const int64_t per_world = int64_t(C) * pairs;
+  const float4* base = reinterpret_cast<const float4*>(rec);
+  static_assert(LX_PAIR_REC == 16, "four float4 per record");
+  for (int64_t j = tid; j < per_world; j += kPairThreads) {
+    const float4 q0 = base[j * 4], q1 = base[j * 4 + 1], q2 = base[j * 4 + 2], q3 = base[j * 4 + 3];
+    const float r[14] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y};
+#pragma unroll
+    for (int i = 0; i < 14; ++i) v[i] += r[i];
+    const int ch = int(j / pairs);
+#pragma unroll
+    for (int c2 = 0; c2 < LX_MAX_C; ++c2) v[14 + c2] += c2 == ch ? r[0] : 0.f;
+  }
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const float w = wave_sum(v[i]);
+    if (lane == 0) red[wave][i] = w;
+  }
+  __syncthreads();
+  lx_sums s;
+  if (tid == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      float acc = 0.f;
+      for (int wv = 0; wv < kPairThreads / 64; ++wv) acc += red[wv][i];
+      v[i] = acc;
+    }
+  }
+  s.sA = v[0]; s.sG = v[1]; s.nA = v[2]; s.nG = v[3]; s.nU = v[4];
+  for (int d = 0; d < 3; ++d) { s.sAy[d] = v[5 + d]; s.sGy[d] = v[8 + d]; s.sAyy[d] = v[11 + d]; }
+  for (int ch = 0; ch < LX_MAX_C; ++ch) s.sAc[ch] = ch < C ? v[14 + ch] : 0.f;
+  return s;
+}
+
+// Autotuner (lx_tune.hip): knob values by launch shape.  lx_tune_lookup gives a forced or recorded
+// value (1) or nothing (0); lx_tune_should_time says whether a launcher may time its candidates now
+// (tuning on, stream not capturing).
+int lx_tune_mode();
+int lx_tune_lookup(const char* knob, const char* key, int* value);
+void lx_tune_record(const char* knob, const char* key, int value);
+bool lx_tune_should_time(hipStream_t st);
+
+// Milliseconds per call of enqueue() on stream st: one untimed call, then `reps` timed ones between
+// two events (the caller's stream waits for them; < 0 on a HIP error).
+template <class F>
+float lx_time_ms(hipStream_t st, int reps, F&& enqueue) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1.f;
+  float ms = -1.f;
+  if (enqueue() == 0 && hipEventRecord(e0, st) == hipSuccess) {
+    int rc = 0;
+    for (int i = 0; i < reps && !rc; ++i) rc = enqueue();
+    if (!rc && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+        hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+      ms /= float(reps);
+    else
+      ms = -1.f;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return ms;
+}
+
 // Error reporting (thread-local message), lx_api.hip
 int lx_fail(int code, const char* fmt, ...);
 int lx_check_rule(const lx_rule* rule);
@@ -234,6 +308,11 @@ int lx_check_rule(const lx_rule* rule);
 // (lx_plain.hip): pairs records per (world, channel), carry [B] lx_stat_carry, stats [B][11 + C]
 int lx_pair_stats_launch(const lx_rule* rule, const float* part, int64_t B, int64_t pairs, float R, float T,
                          void* carry_d, float* stats_d, hipStream_t st);
+
+// lx_run on the multi-pass path (lx_fft.hip): any rule lx_update takes, with retirement and
+// refill from a device-side list and the step loop captured as HIP graphs
+int lx_passes_scratch(const lx_rule* rule, int64_t B, int32_t slots, size_t* bytes);
+int lx_passes_run(const lx_rule* rule, const lx_run_args* a, hipStream_t stream);
 
 // Fused persistent 2-D run (lx_fused2d.hip)
 int lx_fused2d_supported(const lx_rule* rule);

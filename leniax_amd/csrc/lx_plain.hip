@@ -286,49 +286,14 @@ __global__ void stats_finish_kernel(StatArgs a) {
 }
 
 // The step's statistics from the row-pair records of lx_step_stats' inverse row pass: one
-// 256-thread block per world; thread i sums records i, i + 256, ... of the world's C x pairs in
-// order, then butterflies within each wave and the four wave sums in order (deterministic), and
-// thread 0 finishes the step (leniax/statistics.py:64-126) as stats_finish_kernel does.
-constexpr int kPairThreads = 256;
+// kPairThreads block per world sums the records in a fixed order (lx_block_pair_sums) and thread 0
+// finishes the step (leniax/statistics.py:64-126) as stats_finish_kernel does.
 __global__ __launch_bounds__(kPairThreads) void pair_stats_kernel(StatArgs a, const float* __restrict__ part,
                                                                   int64_t pairs) {
-  constexpr int NV = 14 + LX_MAX_C;  // the record's 14 sums, then the channel masses
-  __shared__ float red[kPairThreads / 64][NV];
   const int64_t b = blockIdx.x;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  float v[NV];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) v[i] = 0.f;
-  const int64_t per_world = int64_t(a.C) * pairs;
-  const float4* base = reinterpret_cast<const float4*>(part + b * per_world * LX_PAIR_REC);
-  static_assert(LX_PAIR_REC == 16, "four float4 per record");
-  for (int64_t j = tid; j < per_world; j += kPairThreads) {
-    const float4 q0 = base[j * 4], q1 = base[j * 4 + 1], q2 = base[j * 4 + 2], q3 = base[j * 4 + 3];
-    const float r[14] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, q3.x, q3.y};
-#pragma unroll
-    for (int i = 0; i < 14; ++i) v[i] += r[i];
-    const int ch = int(j / pairs);
-#pragma unroll
-    for (int c2 = 0; c2 < LX_MAX_C; ++c2) v[14 + c2] += c2 == ch ? r[0] : 0.f;
-  }
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    const float w = wave_sum(v[i]);
-    if (lane == 0) red[wave][i] = w;
-  }
-  __syncthreads();
-  if (tid != 0) return;
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    float acc = 0.f;
-    for (int wv = 0; wv < kPairThreads / 64; ++wv) acc += red[wv][i];
-    v[i] = acc;
-  }
+  const lx_sums s = lx_block_pair_sums(part + b * int64_t(a.C) * pairs * LX_PAIR_REC, a.C, pairs);
+  if (threadIdx.x != 0) return;
   lx_stat_carry* sc = a.carry + b;
-  lx_sums s;
-  s.sA = v[0]; s.sG = v[1]; s.nA = v[2]; s.nG = v[3]; s.nU = v[4];
-  for (int d = 0; d < 3; ++d) { s.sAy[d] = v[5 + d]; s.sGy[d] = v[8 + d]; s.sAyy[d] = v[11 + d]; }
-  for (int ch = 0; ch < LX_MAX_C; ++ch) s.sAc[ch] = ch < a.C ? v[14 + ch] : 0.f;
   int32_t dd[3];
   for (int i = 0; i < 3; ++i) dd[i] = a.dims[i];
   const int64_t n = int64_t(a.ndim == 3 ? a.dims[0] : 1) * a.dims[a.ndim - 2] * a.dims[a.ndim - 1];
@@ -366,6 +331,10 @@ struct AdjArgs {
   float* g_gf;
   float* g_w;
   float* g_dt;
+  float* part_g;  // per-block partial sums of the growth / weight (and one-kernel) gradients
+  float* part_s;  // per-block partial sums of the dt gradient (state kernel)
+  float* ysum_g;
+  float* ysum_s;
   int64_t cells;
   int C, K;
   int cin[LX_MAX_K];
@@ -379,9 +348,10 @@ LX_DEV float adj_dz(const AdjArgs& a, float A, float G, float lam, float dtw) {
   return (z > 0.f && z < 1.f) ? lam : 0.f;
 }
 
-// block-wide sums of NV per-thread values, each added atomically into dst[i] (skipped where NULL)
+// block-wide sums of NV per-thread values, written to the block's slot of the partial sums:
+// part[(blockIdx.y * gridDim.x + blockIdx.x) * NV + i] (summed later in a fixed order)
 template <int NV>
-LX_DEV void block_atomic_sum(float (&v)[NV], float* const (&dst)[NV]) {
+LX_DEV void block_partial(float (&v)[NV], float* part) {
   __shared__ float red[NV][4];
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
 #pragma unroll
@@ -390,9 +360,10 @@ LX_DEV void block_atomic_sum(float (&v)[NV], float* const (&dst)[NV]) {
     if (ln == 0) red[i][wv] = s;
   }
   __syncthreads();
+  float* dst = part + (int64_t(blockIdx.y) * gridDim.x + blockIdx.x) * NV;
 #pragma unroll
   for (int i = 0; i < NV; ++i)
-    if (threadIdx.x == i && dst[i]) atomicAdd(dst[i], red[i][0] + red[i][1] + red[i][2] + red[i][3]);
+    if (threadIdx.x == i) dst[i] = red[i][0] + red[i][1] + red[i][2] + red[i][3];
 }
 
 // grid (blocks per image, B * K): dL/dU_k and the growth / weight gradients; reads lambda before the
@@ -445,13 +416,7 @@ __global__ __launch_bounds__(256) void adj_growth_kernel(AdjArgs a) {
     acc[0] += dg * dgdm;
     acc[1] += dg * dgds;
   }
-  float* const dst[2 + LX_MAX_C] = {
-      a.g_gf ? a.g_gf + b * a.gf_bs + 2 * k : nullptr, a.g_gf ? a.g_gf + b * a.gf_bs + 2 * k + 1 : nullptr,
-      a.g_w && 0 < a.C ? a.g_w + b * a.w_bs + 0 * a.K + k : nullptr,
-      a.g_w && 1 < a.C ? a.g_w + b * a.w_bs + 1 * a.K + k : nullptr,
-      a.g_w && 2 < a.C ? a.g_w + b * a.w_bs + 2 * a.K + k : nullptr,
-      a.g_w && 3 < a.C ? a.g_w + b * a.w_bs + 3 * a.K + k : nullptr};
-  block_atomic_sum<2 + LX_MAX_C>(acc, dst);
+  block_partial<2 + LX_MAX_C>(acc, a.part_g);
 }
 
 // One channel, one kernel: the whole pointwise reverse in one pass (grid: blocks per image x B).
@@ -510,9 +475,7 @@ __global__ __launch_bounds__(256) void adj_11_kernel(AdjArgs a) {
       }
     }
   }
-  float* const dst[4] = {a.g_gf ? a.g_gf + b * a.gf_bs : nullptr, a.g_gf ? a.g_gf + b * a.gf_bs + 1 : nullptr,
-                         a.g_w ? a.g_w + b * a.w_bs : nullptr, a.g_dt ? a.g_dt + b * a.dt_bs : nullptr};
-  block_atomic_sum<4>(acc, dst);
+  block_partial<4>(acc, a.part_g);
 }
 
 // grid (blocks per image, B * C): direct part of dL/dA and dL/ddt
@@ -533,23 +496,107 @@ __global__ __launch_bounds__(256) void adj_state_kernel(AdjArgs a) {
       acc[0] += dz * G;
     }
   }
-  float* const dst[1] = {a.g_dt ? a.g_dt + b * a.dt_bs : nullptr};
-  block_atomic_sum<1>(acc, dst);
+  block_partial<1>(acc, a.part_s);
+}
+
+// The gradient scalars from the blocks' partial sums, in a fixed order: first each image's sum over
+// its blocks (ysum[y][i], y the grid row), then each gradient element adds the sums of the worlds
+// and channels that feed it (every element has one writer).  Deterministic, and the same for a
+// world whatever the batch it is in: the blocks per image depend on the world size alone.
+__global__ void adj_ysum_kernel(const float* __restrict__ part, float* __restrict__ ysum, int64_t Y, int NV,
+                                int per_img) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= Y * NV) return;
+  const int64_t y = i / NV;
+  const int s = int(i - y * NV);
+  float acc = 0.f;
+  for (int blk = 0; blk < per_img; ++blk) acc += part[(y * per_img + blk) * NV + s];
+  ysum[i] = acc;
+}
+
+__global__ void adj_targets_kernel(AdjArgs a, int64_t B, int one) {
+  const int C = a.C, K = a.K;
+  const int64_t ngf = a.g_gf ? (a.gf_bs ? B : 1) * 2 * K : 0;
+  const int64_t nw = a.g_w ? (a.w_bs ? B : 1) * int64_t(C) * K : 0;
+  const int64_t ndt = a.g_dt ? (a.dt_bs ? B : 1) : 0;
+  const int NG = 2 + LX_MAX_C;  // slots of a growth row (dm, ds, dw per channel), as block_partial writes it
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < ngf + nw + ndt;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    float acc = 0.f;
+    if (i < ngf) {
+      const int64_t bt = i / (2 * K);
+      const int e = int(i - bt * 2 * K), k = e / 2, j = e % 2;
+      const int64_t b0 = a.gf_bs ? bt : 0, b1 = a.gf_bs ? bt + 1 : B;
+      for (int64_t b = b0; b < b1; ++b) acc += one ? a.ysum_g[b * 4 + j] : a.ysum_g[(b * K + k) * NG + j];
+      a.g_gf[bt * a.gf_bs + e] += acc;
+    } else if (i < ngf + nw) {
+      const int64_t r = i - ngf;
+      const int64_t bt = r / (int64_t(C) * K);
+      const int e = int(r - bt * C * K), c = e / K, k = e % K;
+      const int64_t b0 = a.w_bs ? bt : 0, b1 = a.w_bs ? bt + 1 : B;
+      for (int64_t b = b0; b < b1; ++b) acc += one ? a.ysum_g[b * 4 + 2] : a.ysum_g[(b * K + k) * NG + 2 + c];
+      a.g_w[bt * a.w_bs + e] += acc;
+    } else {
+      const int64_t bt = i - ngf - nw;
+      const int64_t b0 = a.dt_bs ? bt : 0, b1 = a.dt_bs ? bt + 1 : B;
+      for (int64_t b = b0; b < b1; ++b) {
+        if (one) {
+          acc += a.ysum_g[b * 4 + 3];
+        } else {
+          for (int c = 0; c < C; ++c) acc += a.ysum_s[b * C + c];
+        }
+      }
+      a.g_dt[bt * a.dt_bs] += acc;
+    }
+  }
+}
+
+// blocks per image of the pointwise adjoint: about 4 cells per thread, at most 256
+unsigned adj_blocks_per_image(int64_t cells) {
+  return unsigned(std::min<int64_t>(std::max<int64_t>((cells + 1023) / 1024, 1), 256));
+}
+
+// partial-sum layout in the workspace: growth (or one-kernel) rows, state rows, then their sums
+struct AdjWork {
+  size_t part_g, part_s, ysum_g, ysum_s, total;
+};
+AdjWork adj_work_layout(const lx_rule* r, int64_t B) {
+  const int64_t per = adj_blocks_per_image(cells_of(r));
+  const bool one = r->C == 1 && r->K == 1;
+  const int64_t yg = one ? B : B * r->K, ys = one ? 0 : B * r->C;
+  const int ng = one ? 4 : 2 + LX_MAX_C, ns = one ? 0 : 1;  // row widths of block_partial<4> / <2 + LX_MAX_C>
+  AdjWork l;
+  size_t o = 0;
+  l.part_g = o; o += size_t(yg * per * ng) * sizeof(float);
+  l.part_s = o; o += size_t(ys * per * ns) * sizeof(float);
+  l.ysum_g = o; o += size_t(yg * ng) * sizeof(float);
+  l.ysum_s = o; o += size_t(ys * ns) * sizeof(float);
+  l.total = (o + 255) / 256 * 256;
+  return l;
 }
 
 }  // namespace
+
+extern "C" int lx_adjoint_workspace_size(const lx_rule* rule, int64_t B, size_t* bytes) {
+  int rc = lx_check_rule(rule);
+  if (rc) return rc;
+  if (!bytes || B < 0) return lx_fail(LX_EINVAL, "bad arguments");
+  *bytes = adj_work_layout(rule, B).total;
+  return LX_OK;
+}
 
 extern "C" int lx_adjoint_pointwise(const lx_rule* rule, int64_t B, const float* state_d, const float* field_d,
                                     const float* potential_d, const float* gf_d, int64_t gf_bstride, const float* w_d,
                                     int64_t w_bstride, const float* dt_d, int64_t dt_bstride, const float* lambda_d,
                                     const float* dfield_d, const float* dpot_in_d, int32_t ste, float* lambda_out_d,
-                                    float* dpot_d, float* grad_gf_d, float* grad_w_d, float* grad_dt_d,
+                                    float* dpot_d, float* grad_gf_d, float* grad_w_d, float* grad_dt_d, void* work_d,
                                     void* stream) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
   if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
   if (B == 0) return LX_OK;
-  if (!state_d || !field_d || !potential_d || !gf_d || !w_d || !dt_d || !lambda_d || !lambda_out_d || !dpot_d)
+  if (!state_d || !field_d || !potential_d || !gf_d || !w_d || !dt_d || !lambda_d || !lambda_out_d || !dpot_d ||
+      !work_d)
     return lx_fail(LX_EINVAL, "NULL pointer");
   if (B * std::max(rule->C, rule->K) > 65535) return lx_fail(LX_ENOTSUP, "lx_adjoint_pointwise: B * max(C, K) > 65535");
   AdjArgs a;
@@ -562,15 +609,18 @@ extern "C" int lx_adjoint_pointwise(const lx_rule* rule, int64_t B, const float*
     a.gfk[k] = k < rule->K ? rule->gf[k] : 0;
   }
   a.state_fn = rule->state_fn; a.average = rule->average; a.ste = ste != 0;
-  // blocks per image: at most 256 and about 1024 blocks over the whole grid.  Every block ends
-  // with float atomics on its world's few gradient scalars, which lie in one L2 channel and are
-  // served one at a time: 256 blocks for each of 16 worlds of 96^3 (16384 atomics) took the
-  // pointwise pass from 54 to 122 us; 64 blocks per world still give 4 blocks per CU.
-  const int64_t imgs = B * std::max(rule->C, rule->K);
-  const int64_t want = std::max<int64_t>(1, 1024 / imgs);
-  const unsigned per_img = unsigned(std::min<int64_t>(std::min<int64_t>((a.cells + 1023) / 1024, 256), want));
+  const AdjWork l = adj_work_layout(rule, B);
+  char* wk = static_cast<char*>(work_d);
+  a.part_g = reinterpret_cast<float*>(wk + l.part_g);
+  a.part_s = reinterpret_cast<float*>(wk + l.part_s);
+  a.ysum_g = reinterpret_cast<float*>(wk + l.ysum_g);
+  a.ysum_s = reinterpret_cast<float*>(wk + l.ysum_s);
+  // blocks per image from the world size alone: each block leaves its partial gradient sums in the
+  // workspace (no atomics), summed afterwards in a fixed order
+  const unsigned per_img = adj_blocks_per_image(a.cells);
   hipStream_t st = (hipStream_t)stream;
-  if (rule->C == 1 && rule->K == 1) {
+  const bool one = rule->C == 1 && rule->K == 1;
+  if (one) {
     const dim3 grid(per_img, unsigned(B));
     switch (rule->gf[0]) {
       case LX_GF_POLY_QUAD4: hipLaunchKernelGGL(adj_11_kernel<LX_GF_POLY_QUAD4>, grid, dim3(256), 0, st, a); break;
@@ -587,6 +637,16 @@ extern "C" int lx_adjoint_pointwise(const lx_rule* rule, int64_t B, const float*
     // growth first: it reads lambda, which the state kernel may overwrite in place
     hipLaunchKernelGGL(adj_growth_kernel, dim3(per_img, unsigned(B * rule->K)), dim3(256), 0, st, a);
     hipLaunchKernelGGL(adj_state_kernel, dim3(per_img, unsigned(B * rule->C)), dim3(256), 0, st, a);
+  }
+  if (grad_gf_d || grad_w_d || grad_dt_d) {
+    const int64_t yg = one ? B : B * rule->K;
+    const int ng = one ? 4 : 2 + LX_MAX_C;
+    hipLaunchKernelGGL(adj_ysum_kernel, dim3(grid_for(yg * ng, 256)), dim3(256), 0, st, a.part_g, a.ysum_g, yg, ng,
+                       int(per_img));
+    if (!one)
+      hipLaunchKernelGGL(adj_ysum_kernel, dim3(grid_for(B * rule->C, 256)), dim3(256), 0, st, a.part_s, a.ysum_s,
+                         B * rule->C, 1, int(per_img));
+    hipLaunchKernelGGL(adj_targets_kernel, dim3(1), dim3(256), 0, st, a, B, one ? 1 : 0);
   }
   return launched("adjoint pointwise");
 }

@@ -4,6 +4,7 @@ Tensors must be contiguous; float32 state, complex64 spectra.  All calls are
 enqueued on torch's current HIP stream.
 """
 import ctypes
+import os
 
 import numpy as np
 import torch
@@ -102,9 +103,16 @@ def carry_init(B):
     return buf
 
 
-def run_fused(rule, cells0, K_half, k_per_world, gf, w, dt, max_steps, stop='none', min_steps=0, R=1., T=1.,
-              want_stats=True, want_hist=False, out=None):
-    """One lx_run launch over B worlds.  gf [B or 1, K, 2], w [B or 1, C, K], dt [B or 1]."""
+def run_path(rule):
+    """'fused', 'passes' or None: the path lx_run takes for this rule."""
+    p = _lib.lib().lx_run_supported(ctypes.byref(rule))
+    return {_lib.LX_RUN_FUSED: 'fused', _lib.LX_RUN_PASSES: 'passes'}.get(p)
+
+
+def run_batched(rule, cells0, K_half, k_per_world, gf, w, dt, max_steps, stop='none', min_steps=0, R=1., T=1.,
+                want_stats=True, want_hist=False, out=None, slots=0):
+    """One lx_run over B worlds (the fused persistent launch or the multi-pass run with retirement,
+    whichever the rule takes).  gf [B or 1, K, 2], w [B or 1, C, K], dt [B or 1]."""
     B = cells0.shape[0]
     dev = cells0.device
     nst = NSTAT_BASE + rule.C
@@ -112,7 +120,9 @@ def run_fused(rule, cells0, K_half, k_per_world, gf, w, dt, max_steps, stop='non
     stats = torch.zeros((B, max_steps, nst), dtype=torch.float32, device=dev) if want_stats else None
     n = torch.zeros(B, dtype=torch.int32, device=dev)
     steps = torch.zeros(B, dtype=torch.int32, device=dev)
-    scratch = torch.empty(max(int(_lib.lib().lx_run_scratch_bytes(B)), 16), dtype=torch.uint8, device=dev)
+    nbytes = ctypes.c_size_t(0)
+    _lib.call('lx_run_scratch_size', ctypes.byref(rule), B, int(slots), ctypes.byref(nbytes))
+    scratch = torch.empty(max(int(nbytes.value), 16), dtype=torch.uint8, device=dev)
     hist = None
     if want_hist:
         dims = tuple(cells0.shape[2:])
@@ -142,6 +152,7 @@ def run_fused(rule, cells0, K_half, k_per_world, gf, w, dt, max_steps, stop='non
     if hist is not None:
         a.cells_hist_d, a.field_hist_d, a.potential_hist_d = (h.data_ptr() for h in hist)
     a.scratch_d = scratch.data_ptr()
+    a.slots = int(slots)
     _lib.call('lx_run', ctypes.byref(rule), ctypes.byref(a), stream_ptr())
     return {'cells': cells, 'stats': stats, 'n': n, 'steps': steps, 'hist': hist}
 
@@ -151,3 +162,36 @@ def stats_to_dict(stats_bt, C):
     d = {k: stats_bt[..., i] for i, k in enumerate(STAT_KEYS)}
     d['channel_mass'] = stats_bt[..., NSTAT_BASE:NSTAT_BASE + C]
     return d
+
+
+def autotune(mode='on', cache=None):
+    """Schedule autotuning of the HIP passes: 'off' (fixed rules), 'on' (time the variants on the
+    first call of each shape, keep the fastest) or 'cached' (decisions already taken only).  cache:
+    a file the decisions are read from and appended to.  Returns the previous mode."""
+    if cache is not None:
+        _lib.call('lx_tune_cache', os.fsencode(cache))
+    prev = _lib.lib().lx_autotune(_lib.TUNE[mode])
+    if prev < 0:
+        raise _lib.LxError('lx_autotune', prev, _lib.lib().lx_last_error().decode())
+    return {v: k for k, v in _lib.TUNE.items()}[prev]
+
+
+def tune_force(knob, value):
+    """Pin a schedule knob (see include/leniax_amd.h) for every shape; value None unpins it."""
+    _lib.call('lx_tune_force', knob.encode(), -1 if value is None else int(value))
+
+
+def tune_report():
+    """The schedule decisions so far: {'forced': {knob: value}, 'decided': {(knob, key): value}}."""
+    L = _lib.lib()
+    n = L.lx_tune_report(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    L.lx_tune_report(buf, n + 1)
+    out = {'forced': {}, 'decided': {}}
+    for line in buf.value.decode().splitlines():
+        parts = line.split()
+        if parts[0] == 'forced':
+            out['forced'][parts[1]] = int(parts[2])
+        else:
+            out['decided'][(parts[0], parts[1])] = int(parts[2])
+    return out

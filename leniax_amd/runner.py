@@ -1,12 +1,12 @@
 """Scan runners (leniax/runner.py) on the GPU.
 
 * ``run_scan`` / ``run_scan_mem_optimized`` with the standard update function
-  and the standard statistics run as ONE persistent HIP launch (lx_run) when the
-  rule has a fused kernel (2-D N x N, one channel, one kernel, N <= 128): every
-  world stays on-chip for all steps, statistics and heuristics are evaluated
-  every step.  Other rules step with the plain HIP step and its statistics in
-  one pass (lx_step_stats) from a host loop, then scan the heuristics on the
-  device.
+  and the standard statistics are ONE lx_run: a persistent HIP launch when the
+  rule has a fused kernel (2-D N x N, one channel, one kernel, N <= 128: every
+  world stays on-chip for all steps), else the multi-pass step over a device-side
+  list of live worlds, replayed from HIP graphs.  Statistics and heuristics are
+  evaluated on the device every step.
+* Custom update or statistics functions step through them one call per step.
 * ``run`` keeps leniax's Python loop (different heuristics, optional truncation).
 """
 import ctypes
@@ -47,7 +47,8 @@ def _dt_of(T):
 
 
 def _scan_plain(rule, slots, cells0, K_half, kbs, gf, w, dt, max_run_iter, R, T, keep):
-    """Host loop of plain HIP steps over B worlds (all arrays already flattened per world)."""
+    """One lx_step_stats call per step over B worlds (arrays flattened per world), then the heuristic
+    scan: the step-by-step form of lx_run's multi-pass path, which the tests compare it with."""
     B = cells0.shape[0]
     dims = tuple(cells0.shape[2:])
     dev = cells0.device
@@ -103,16 +104,14 @@ def run_scan(rng_key, cells0, K, gf_params, kernels_weight_per_channel, T, max_r
     w = engine.as_device(kernels_weight_per_channel)
     dt = engine.as_device(_dt_of(T)).reshape(-1)
     Rs, Ts = compute_stats_fn.R, compute_stats_fn.T
-    if _lib.lib().lx_run_supported(ctypes.byref(rule)):
-        res = engine.run_fused(rule, cells0, Kh, kbs != 0, gf.reshape((1,) + tuple(gf.shape)),
-                               w.reshape((1,) + tuple(w.shape)), dt, max_run_iter, 'none', 0, Rs, Ts,
-                               want_stats=True, want_hist=True)
-        rows = res['stats'].transpose(0, 1)  # [T, B, nst]
-        n = res['n'].to(torch.float32)
-        cells_h, field_h, pot_h = res['hist']
-    else:
-        rows, n, _, (cells_h, field_h, pot_h) = _scan_plain(rule, slots, cells0, Kh, kbs, gf, w, dt, max_run_iter,
-                                                            Rs, Ts, keep=True)
+    if engine.run_path(rule) is None:
+        raise ValueError(f'world size {tuple(cells0.shape[2:])} is not supported by the HIP step')
+    res = engine.run_batched(rule, cells0, Kh, kbs != 0, gf.reshape((1,) + tuple(gf.shape)),
+                             w.reshape((1,) + tuple(w.shape)), dt, max_run_iter, 'none', 0, Rs, Ts,
+                             want_stats=True, want_hist=True)
+    rows = res['stats'].transpose(0, 1)  # [T, B, nst]
+    n = res['n'].to(torch.float32)
+    cells_h, field_h, pot_h = res['hist']
     stats = lx_stat.stats_rows_to_dict(rows, rule.C)
     stats['N'] = n
     return cells_h, field_h, pot_h, stats
@@ -141,6 +140,43 @@ def _run_scan_generic(rng_key, cells0, K, gf_params, w, T, max_run_iter, R, upda
     return torch.stack(ys_c), torch.stack(ys_f), torch.stack(ys_p), stats
 
 
+def _run_scan_mem_optimized_generic(cells0, K, gf_params, w, T, max_run_iter, update_fn, compute_stats_fn):
+    """run_scan_mem_optimized for a custom update or statistics function (e.g. the direct-convolution
+    potential of fft=False): each solution's N_init worlds step as one batch through ``update_fn``,
+    solution after solution, as leniax's vmap evaluates them (leniax/runner.py:167-215)."""
+    S, N_init = cells0.shape[:2]
+    T = np.asarray(T.detach().cpu() if isinstance(T, torch.Tensor) else T, dtype=np.float32).reshape(-1)
+    gf_params = engine.as_device(gf_params)
+    w = engine.as_device(w)
+    all_stats, finals = [], []
+    for s in range(S):
+        st, final = _scan_generic_mem(cells0[s], K[s], gf_params[s], w[s], T[s if T.size > 1 else 0], max_run_iter,
+                                      update_fn, compute_stats_fn)
+        all_stats.append(st)
+        finals.append(final)
+    stats = {k: torch.stack([st[k] for st in all_stats]) for k in all_stats[0]}
+    return stats, torch.stack(finals)
+
+
+def _scan_generic_mem(cells, K, gf_params, w, T, max_run_iter, update_fn, compute_stats_fn):
+    """leniax/runner.py:295-334 without the kept frames: statistics [T, N] and the final state."""
+    N, nd = cells.shape[0], cells.dim() - 2
+    dt = np.float32(1) / np.float32(T)
+    dev = cells.device
+    shift = torch.zeros((N, nd), dtype=torch.int32, device=dev)
+    centroid = torch.zeros((nd, N), dtype=torch.float32, device=dev)
+    angle = torch.zeros((N,), dtype=torch.float32, device=dev)
+    ys = []
+    for _ in range(max_run_iter):
+        new_cells, field, potential = update_fn(None, cells, K, gf_params, w, dt)
+        st, shift, centroid, angle = compute_stats_fn(cells, field, potential, shift, centroid, angle)
+        ys.append(st)
+        cells = new_cells
+    stats = lx_stat.stats_list_to_dict(ys)
+    stats['N'] = lx_stat.check_heuristics(stats).sum(dim=0)
+    return stats, cells
+
+
 def run_scan_mem_optimized(rng_key, cells0, K, gf_params, kernels_weight_per_channel, T, max_run_iter: int,
                            R: float, update_fn: Callable, compute_stats_fn: Callable):
     """leniax/runner.py:167-215 (vmapped over solutions S).
@@ -159,7 +195,10 @@ def run_scan_mem_optimized(rng_key, cells0, K, gf_params, kernels_weight_per_cha
         Kf = K.reshape((S,) + tuple(K.shape[-(cells0.dim() - 1):]))  # [S, C, maxk, *world]
     fr = _fused_rule(update_fn, compute_stats_fn, cells0[0], Kf)
     if fr is None:
-        raise NotImplementedError('run_scan_mem_optimized needs the standard update and statistics functions')
+        if isinstance(K, lx_kernels.HalfSpectra):
+            raise ValueError('device-built half spectra need the standard FFT update function')
+        return _run_scan_mem_optimized_generic(cells0, K, gf_params, kernels_weight_per_channel, T, max_run_iter,
+                                               update_fn, compute_stats_fn)
     rule, slots = fr
     B = S * N_init
     if isinstance(K, lx_kernels.HalfSpectra):
@@ -176,14 +215,12 @@ def run_scan_mem_optimized(rng_key, cells0, K, gf_params, kernels_weight_per_cha
     dt = engine.as_device(_dt_of(T)).reshape(-1).repeat_interleave(N_init).contiguous()
     flat = cells0.reshape((B,) + tuple(cells0.shape[2:])).contiguous()
     Rs, Ts = compute_stats_fn.R, compute_stats_fn.T
-    if _lib.lib().lx_run_supported(ctypes.byref(rule)):
-        res = engine.run_fused(rule, flat, Kh, True, gf, w, dt, max_run_iter, 'none', 0, Rs, Ts, want_stats=True)
-        rows = res['stats']  # [B, T, nst]
-        n = res['n'].to(torch.float32)
-        final = res['cells']
-    else:
-        rows, n, final, _ = _scan_plain(rule, slots, flat, Kh, kbs, gf, w, dt, max_run_iter, Rs, Ts, keep=False)
-        rows = rows.transpose(0, 1)
+    if engine.run_path(rule) is None:
+        raise ValueError(f'world size {tuple(cells0.shape[3:])} is not supported by the HIP step')
+    res = engine.run_batched(rule, flat, Kh, True, gf, w, dt, max_run_iter, 'none', 0, Rs, Ts, want_stats=True)
+    rows = res['stats']  # [B, T, nst]
+    n = res['n'].to(torch.float32)
+    final = res['cells']
     nst = rows.shape[-1]
     rows = rows.reshape(S, N_init, max_run_iter, nst).transpose(1, 2)  # [S, T, N_init, nst]
     stats = lx_stat.stats_rows_to_dict(rows, rule.C)

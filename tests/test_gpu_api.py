@@ -479,6 +479,90 @@ def test_search_for_init_follows_leniax_loop():
     assert float(np.abs(got[:4] - runs[best_o][1][:4]).max()) < 1e-5
 
 
+def test_search_for_init_direct_potential_follows_leniax_loop():
+    """leniax/helpers.py:319-398 with fft=False: the update function is the direct-convolution one,
+    which run_scan_mem_optimized steps through its generic per-solution loop; the N of every drawn
+    state, the chosen index and the chosen run's first frames match the oracle's loop over the
+    oracle's direct potential (leniax/core.py:105-146) with the same host-built kernel."""
+    from leniax_amd import helpers, initializations, kernels
+    kp = [{'c_in': 0, 'c_out': 0, 'gf_params': [0.15, 0.015], 'gf_slug': 'poly_quad4', 'h': 1,
+           'k_params': [1, [1.0]], 'k_slug': 'circle_2d', 'kf_params': [4], 'kf_slug': 'poly_quad'}]
+    cfg = {
+        'algo': {'init_slug': 'perlin'},
+        'kernels_params': kp,
+        'render_params': {'world_size': [48, 48]},
+        'run_params': {'max_run_iter': 40, 'nb_init_search': 3, 'seed': 1},
+        'world_params': {'R': 7, 'T': 10, 'nb_channels': 1, 'nb_dims': 2, 'get_state_fn_slug': 'v1',
+                         'weighted_average': True},
+    }
+    key = initializations.PRNGKey(2)
+    best, i = helpers.search_for_init(key, cfg, fft=False)
+    _, noises = initializations.perlin(key, 3, [48, 48], 7, [0.15, 0.015])
+    states = noises.cpu().numpy().reshape(3, 1, 1, 48, 48)
+    Kd, mp = kernels.get_kernels_and_mapping([dict(k) for k in kp], [48, 48], 1, 7, fft=False)
+    Kd = np.asarray(Kd.cpu() if hasattr(Kd, 'cpu') else Kd, np.float32)
+    pad = L.conv_padding(Kd.shape)
+    sfn = L.build_compute_stats_fn([48, 48], 7, 10, np.float32)
+    gf = np.asarray(mp.get_gf_params(), np.float32)
+    w = np.asarray(mp.get_kernels_weight_per_channel(), np.float32)
+    cur, best_o, runs = 0, None, []
+    for j in range(3):
+        A = states[j]
+        carry = L.init_stats_carry(1, 2)
+        frames, st_all = [], []
+        for _ in range(40):
+            U = L.get_potential(A, Kd, pad, None)
+            G = L.get_field(U, gf, w, ['poly_quad4'], True)
+            st, *carry = sfn(A, G, U, *carry)
+            frames.append(A)
+            st_all.append(st)
+            A = L.get_state(A, G, np.float32(1) / np.float32(10))
+        stats = {k: np.stack([s[k] for s in st_all]) for k in st_all[0]}
+        n = int(L.check_heuristics(stats).sum(axis=0)[0])
+        runs.append((n, np.stack(frames)))
+        if cur < n:
+            cur, best_o = n, j
+        if n >= 40:
+            break
+    print('N per init (oracle):', [r[0] for r in runs], 'chosen', best_o, 'i', j, '| device N', best.get('N'), 'i', i)
+    assert i == j
+    if best_o is None:
+        assert best == {}
+        return
+    assert int(best['N'][0]) == runs[best_o][0]
+    got = best['all_cells'].cpu().numpy()
+    assert got.shape == (40, 1, 1, 48, 48)
+    np.testing.assert_array_equal(got[0], runs[best_o][1][0])
+    assert float(np.abs(got[:4] - runs[best_o][1][:4]).max()) < 1e-5
+
+
+def test_run_scan_mem_optimized_generic_update_matches_per_solution_scans():
+    """A custom update function (not the standard FFT triple) through run_scan_mem_optimized:
+    statistics and final states equal run_scan of each solution on its own."""
+    from leniax_amd import core, helpers, kernels, runner, statistics as S
+    kp = [{'k_slug': 'circle_2d', 'k_params': [1., [1.]], 'kf_slug': 'gauss_bump', 'kf_params': [1.],
+           'gf_slug': 'gaussian', 'gf_params': [0.15, 0.02], 'h': 1., 'c_in': 0, 'c_out': 0}]
+    Kd, mp = kernels.get_kernels_and_mapping([dict(k) for k in kp], [32, 32], 1, 6., fft=False)
+    upd = helpers.build_update_fn(Kd.shape, mp, 'v1', True, fft=False)
+    sfn = S.build_compute_stats_fn({'R': 6., 'T': 10., 'nb_dims': 2, 'nb_channels': 1}, {'world_size': [32, 32]})
+    rng = np.random.default_rng(4)
+    cells = np.zeros((2, 3, 1, 32, 32), np.float32)
+    cells[..., 10:22, 10:22] = rng.uniform(size=(2, 3, 1, 12, 12))
+    Kt = torch.as_tensor(np.asarray(Kd.cpu() if hasattr(Kd, 'cpu') else Kd)).cuda()
+    gf = torch.as_tensor(mp.get_gf_params()).cuda()
+    w = torch.as_tensor(mp.get_kernels_weight_per_channel()).cuda()
+    T = np.array([10., 8.], np.float32)
+    stats, final = runner.run_scan_mem_optimized(None, torch.as_tensor(cells).cuda(), torch.stack([Kt, Kt]),
+                                                 torch.stack([gf, gf]), torch.stack([w, w]), T, 12, 6., upd, sfn)
+    assert final.shape == cells.shape and stats['N'].shape == (2, 3)
+    for s in range(2):
+        c_h, _, _, st = runner.run_scan(None, torch.as_tensor(cells[s]).cuda(), Kt, gf, w, T[s], 12, 6., upd, sfn)
+        assert torch.equal(stats['mass'][s], st['mass'])
+        assert torch.equal(stats['N'][s], st['N'])
+        nxt = upd(None, c_h[-1], Kt, gf, w, np.float32(1) / T[s])[0]
+        assert torch.equal(final[s], nxt)
+
+
 def test_run_scan_mem_optimized_pmap_is_the_flat_run():
     """leniax/runner.py:218-268 on one GPU: the device axis folds into the solutions axis; the
     result equals run_scan_mem_optimized over the flattened solutions, reshaped back."""

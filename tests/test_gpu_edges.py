@@ -34,7 +34,7 @@ def _fused_setup(N, R, kernels_params=None):
 def test_empty_batch_through_the_host_interface():
     """core.update / get_potential_fft / the fused runner over zero worlds give empty results."""
     from leniax_amd import core, engine
-    print('empty batch: core.update, get_potential_fft, run_fused', flush=True)
+    print('empty batch: core.update, get_potential_fft, run_batched', flush=True)
     R = 13.
     K, mp = L.get_kernels_and_mapping(_bump(R), [64, 64], 1, R, np.float32)
     A = np.zeros((0, 1, 64, 64), np.float32)
@@ -47,7 +47,7 @@ def test_empty_batch_through_the_host_interface():
     assert tuple(new.shape) == (0, 1, 64, 64) and tuple(field.shape) == (0, 1, 64, 64)
     eng, rule, Kh, gf, w, _, _ = _fused_setup(64, R)
     dt = torch.tensor([0.1], device=eng.device())
-    res = eng.run_fused(rule, torch.zeros((0, 1, 64, 64), device=eng.device()), Kh, False, gf, w, dt, 16,
+    res = eng.run_batched(rule, torch.zeros((0, 1, 64, 64), device=eng.device()), Kh, False, gf, w, dt, 16,
                         'leniax', 0, R, 10.)
     torch.cuda.synchronize()
     assert tuple(res['cells'].shape) == (0, 1, 64, 64) and res['n'].numel() == 0
@@ -60,7 +60,7 @@ def test_fused_zero_steps_returns_the_initial_state(N):
     rng = np.random.default_rng(N)
     c0 = torch.as_tensor(rng.random((3, 1, N, N), dtype=np.float32)).to(eng.device())
     dt = torch.tensor([0.1], device=eng.device())
-    res = eng.run_fused(rule, c0, Kh, False, gf, w, dt, 0, 'none', 0, N / 10., 10., want_stats=False)
+    res = eng.run_batched(rule, c0, Kh, False, gf, w, dt, 0, 'none', 0, N / 10., 10., want_stats=False)
     torch.cuda.synchronize()
     assert torch.equal(res['cells'], c0)
     assert res['steps'].tolist() == [0, 0, 0] and res['n'].tolist() == [0, 0, 0]
@@ -80,8 +80,8 @@ def test_empty_world_among_live_ones():
     c0 = torch.as_tensor(batch).to(eng.device())
     dt = torch.tensor([np.float32(1) / np.float32(T)], device=eng.device())
     steps = 64
-    res = eng.run_fused(rule, c0, Kh, False, gf, w, dt, steps, 'none', 0, R, T)
-    res_m = eng.run_fused(rule, c0, Kh, False, gf, w, dt, steps, 'mass', 0, R, T, want_stats=False)
+    res = eng.run_batched(rule, c0, Kh, False, gf, w, dt, steps, 'none', 0, R, T)
+    res_m = eng.run_batched(rule, c0, Kh, False, gf, w, dt, steps, 'mass', 0, R, T, want_stats=False)
     torch.cuda.synchronize()
     n = res['n'].tolist()
     st = _np(res['stats'])

@@ -40,7 +40,7 @@ def _run_both(kernels_params, cells0, R, T, steps, hist=True):
     gf = torch.as_tensor(gf_np).reshape(1, 1, 2).to(dev)
     w = torch.as_tensor(w_np).reshape(1, 1, 1).to(dev)
     dt = torch.tensor([np.float32(1) / np.float32(T)], device=dev)
-    res = engine.run_fused(rule, c0, Kh, False, gf, w, dt, steps, 'none', 0, R, T, want_stats=True, want_hist=hist)
+    res = engine.run_batched(rule, c0, Kh, False, gf, w, dt, steps, 'none', 0, R, T, want_stats=True, want_hist=hist)
     torch.cuda.synchronize()
     return (o_cells, o_field, o_pot, o_stats), res
 
@@ -149,9 +149,9 @@ def test_fused_probe_and_resume_equal_one_uninterrupted_run(stop):
     w = torch.ones(1, 1, 1, device=dev)
     dt = torch.tensor([np.float32(1) / np.float32(T)], device=dev)
     print('probe/resume', stop, flush=True)
-    full = engine.run_fused(rule, c0, Kh, False, gf, w, dt, steps, 'none', 0, 8., T, want_stats=True,
+    full = engine.run_batched(rule, c0, Kh, False, gf, w, dt, steps, 'none', 0, 8., T, want_stats=True,
                             want_hist=True)
-    part = engine.run_fused(rule, c0, Kh, False, gf, w, dt, steps, stop, 64, 8., T, want_stats=True)
+    part = engine.run_batched(rule, c0, Kh, False, gf, w, dt, steps, stop, 64, 8., T, want_stats=True)
     torch.cuda.synchronize()
     st = part['steps'].cpu().numpy()
     print('  steps per world: min', st.min(), 'max', st.max(), 'probed out (>', steps // 8, '):', int((st > steps // 8).sum()))
@@ -200,8 +200,8 @@ def test_fused_every_growth_and_state_function_stage_by_stage(slug, gfp, state):
     w = torch.ones(1, 1, 1, device=dev)
     dt32 = np.float32(1) / np.float32(T)
     dt = torch.tensor([dt32], device=dev)
-    res = engine.run_fused(rule, c0, Kh, False, gf, w, dt, steps, 'none', 0, R, T, want_stats=True, want_hist=True)
-    res2 = engine.run_fused(rule, c0, Kh, False, gf, w, dt, steps, 'none', 0, R, T, want_stats=True)
+    res = engine.run_batched(rule, c0, Kh, False, gf, w, dt, steps, 'none', 0, R, T, want_stats=True, want_hist=True)
+    res2 = engine.run_batched(rule, c0, Kh, False, gf, w, dt, steps, 'none', 0, R, T, want_stats=True)
     torch.cuda.synchronize()
     cells_h, field_h, pot_h = (h.cpu().numpy() for h in res['hist'])
     final = res['cells'].cpu().numpy()

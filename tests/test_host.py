@@ -59,11 +59,42 @@ def test_argument_errors_are_reported_without_a_device():
     with pytest.raises(_lib.LxError, match='gf'):
         _lib.call('lx_workspace_size', ctypes.byref(r), 1, ctypes.byref(ctypes.c_size_t()))
     r.gf[0] = 1
-    assert _lib.lib().lx_run_supported(ctypes.byref(r)) == 1
+    assert _lib.lib().lx_run_supported(ctypes.byref(r)) == _lib.LX_RUN_FUSED
     r.C = 2
     r.K = 2
     r.cin[1] = 1
+    assert _lib.lib().lx_run_supported(ctypes.byref(r)) == _lib.LX_RUN_PASSES  # the multi-pass path
+    r.dims[0] = 100  # no FFT of 100 points
     assert _lib.lib().lx_run_supported(ctypes.byref(r)) == 0
+    r.dims[0] = 48
+    assert _lib.lib().lx_run_supported(ctypes.byref(r)) == _lib.LX_RUN_PASSES
+    r.ndim = 3
+    r.dims[2] = 24
+    assert _lib.lib().lx_run_supported(ctypes.byref(r)) == 0
+
+
+def test_run_scratch_sizes():
+    """lx_run_scratch_size: the fused path's queues and parked carries; the multi-pass path's second
+    state buffer, carries and the per-slot spectra and records of the worlds in flight."""
+    from leniax_amd import _lib
+    if not _lib.available():
+        pytest.skip('libleniax_amd.so not built')
+    nb = ctypes.c_size_t(0)
+    r = _rule(_lib, [64, 64])
+    _lib.call('lx_run_scratch_size', ctypes.byref(r), 100, 0, ctypes.byref(nb))
+    assert nb.value == _lib.lib().lx_run_scratch_bytes(100)
+    r = _rule(_lib, [64, 64], C=2, K=3)
+    r.cin[1], r.cin[2] = 1, 1
+    sizes = {}
+    for slots in (0, 10, 100, 1000):
+        _lib.call('lx_run_scratch_size', ctypes.byref(r), 100, slots, ctypes.byref(nb))
+        sizes[slots] = nb.value
+    assert sizes[0] == sizes[100] == sizes[1000] > sizes[10]
+    per_slot = (2 + 3) * 64 * 33 * 8 + 2 * 32 * 16 * 4
+    assert sizes[0] - sizes[10] >= 90 * per_slot
+    assert sizes[10] >= 100 * 2 * 64 * 64 * 4 + 10 * per_slot  # the second state buffer of all worlds
+    with pytest.raises(_lib.LxError):
+        _lib.call('lx_run_scratch_size', ctypes.byref(r), 100, -1, ctypes.byref(nb))
 
 
 def _rule(_lib, dims, C=1, K=1):
@@ -107,7 +138,7 @@ def test_empty_batches_are_accepted_with_null_pointers():
         'lx_kernel_spectrum': lambda B: L_.lx_kernel_spectrum(2, dims, B, N, N, N, N),
         'lx_adjoint_potential': lambda B: L_.lx_adjoint_potential(r, B, N, 0, N, N, N, N, N, N),
         'lx_adjoint_pointwise': lambda B: L_.lx_adjoint_pointwise(r, B, N, N, N, N, 0, N, 0, N, 0, N, N, N, 0, N, N,
-                                                                  N, N, N, N),
+                                                                  N, N, N, N, N),
         'lx_random_bits': lambda B: L_.lx_random_bits(key, B, N, N),
         'lx_init_uniform': lambda B: L_.lx_init_uniform(key, B, 64 * 64, N, N, N),
         'lx_init_perlin': lambda B: L_.lx_init_perlin(key, B, 64, 64, 8, 8, N, 1, N, N, N),
@@ -117,10 +148,34 @@ def test_empty_batches_are_accepted_with_null_pointers():
     for name, fn in calls.items():
         assert fn(0) == _lib.LX_OK, (name, L_.lx_last_error())
         assert fn(-1) != _lib.LX_OK, name
+    # the run-level gradient: an empty batch is valid, a negative one or no step is not
+    g = _lib.lx_grad_run_args()
+    g.steps, g.ckpt_every = 8, 4
+    nbytes = ctypes.c_size_t(0)
+    run_calls = {
+        'lx_grad_run_forward': lambda: L_.lx_grad_run_forward(r, ctypes.byref(g), N),
+        'lx_grad_run_backward': lambda: L_.lx_grad_run_backward(r, ctypes.byref(g), N),
+        'lx_grad_run_workspace_size': lambda: L_.lx_grad_run_workspace_size(r, g.B, g.steps, g.ckpt_every, 1,
+                                                                           ctypes.byref(nbytes)),
+    }
+    for name, fn in run_calls.items():
+        g.B = 0
+        assert fn() == _lib.LX_OK, (name, L_.lx_last_error())
+        g.B = -1
+        assert fn() != _lib.LX_OK, name
+        g.B, g.steps = 0, 0
+        assert fn() != _lib.LX_OK, name
+        g.steps = 8
+    # checkpoints (2 of 8 steps at 4) + a segment of 4 steps (5 states, 4 fields, potentials, spectra)
+    g.B = 3
+    _lib.call('lx_grad_run_workspace_size', r, 3, 8, 4, 1, ctypes.byref(nbytes))
+    state = 3 * 64 * 64 * 4
+    assert nbytes.value >= (2 + 5 + 4 + 4 + 1 + 1) * state + 4 * 3 * 64 * 33 * 8
     # every declared entry point that takes a batch is exercised above
     batch_free = {'lx_last_error', 'lx_version', 'lx_prepare_spectra', 'lx_workspace_size', 'lx_carry_bytes',
-                  'lx_run_scratch_bytes', 'lx_run_supported', 'lx_run'}
-    assert sorted(set(_declared()) - batch_free - set(calls)) == []
+                  'lx_adjoint_workspace_size', 'lx_autotune', 'lx_tune_cache', 'lx_tune_force', 'lx_tune_report',
+                  'lx_run_scratch_bytes', 'lx_run_scratch_size', 'lx_run_supported', 'lx_run'}
+    assert sorted(set(_declared()) - batch_free - set(calls) - set(run_calls)) == []
     assert L_.lx_step_stats(r, N, 0, N, 0, N, 0, N, 0, N, 0, N, N, N, 0., 10., N, N, N, N) != _lib.LX_OK  # R <= 0
     # the workspace holds the two spectra sets and the row-pair records of lx_step_stats
     nb = ctypes.c_size_t(0)
@@ -134,11 +189,18 @@ def test_empty_batches_are_accepted_with_null_pointers():
     assert L_.lx_run(r, ctypes.byref(a), N) == _lib.LX_OK, L_.lx_last_error()
     a.B = -1
     assert L_.lx_run(r, ctypes.byref(a), N) != _lib.LX_OK
+    r2 = _rule(_lib, [64, 64], C=2, K=2)  # the multi-pass path
+    r2.cin[1] = 1
+    a.B = 0
+    assert L_.lx_run(ctypes.byref(r2), ctypes.byref(a), N) == _lib.LX_OK, L_.lx_last_error()
+    a.slots = -1
+    assert L_.lx_run(ctypes.byref(r2), ctypes.byref(a), N) != _lib.LX_OK
 
 
 def test_unsupported_sizes_are_rejected_before_the_batch_is_looked_at():
-    """World sizes the FFT does not cover (2^k in [16, 512], 3 2^k in [48, 384]) and worlds the
-    fused run does not cover are errors even for an empty batch."""
+    """World sizes the FFT does not cover (2^k in [16, 512], 3 2^k in [48, 384]) are errors even
+    for an empty batch; lx_run takes every size the step passes take (the fused kernel or the
+    multi-pass path)."""
     from leniax_amd import _lib
     if not _lib.available():
         pytest.skip('libleniax_amd.so not built')
@@ -158,12 +220,12 @@ def test_unsupported_sizes_are_rejected_before_the_batch_is_looked_at():
     a.B = 0
     a.R = 13.
     a.T = 10.
-    for dims, ok in (([32, 32], True), ([128, 128], True), ([256, 256], False), ([64, 128], False)):
+    for dims, ok in (([32, 32], True), ([128, 128], True), ([256, 256], True), ([64, 128], True),
+                     ([100, 100], False), ([1024, 64], False), ([96, 80], False)):
         rc = L_.lx_run(ctypes.byref(_rule(_lib, dims)), ctypes.byref(a), N)
         assert (rc == _lib.LX_OK) == ok, (dims, L_.lx_last_error())
-    r = _rule(_lib, [64, 64], C=2, K=2)
-    r.cin[1] = 1
-    assert L_.lx_run(ctypes.byref(r), ctypes.byref(a), N) == _lib.LX_ENOTSUP
+        if not ok:
+            assert rc == _lib.LX_ENOTSUP and 'world size' in L_.lx_last_error().decode()
 
 
 def test_circle_2d_matches_oracle():
@@ -257,3 +319,29 @@ def test_crop_and_centre_utilities_known_answers():
     big[0, 60:65, 0:3] = patch[0, :, 3:]
     np.testing.assert_array_equal(U.center_and_crop(big), patch)
     np.testing.assert_array_equal(U.center_and_crop(np.zeros((2, 16, 16), np.float32)), np.zeros((2, 1, 1)))
+
+
+def test_tuner_controls_without_a_device(tmp_path):
+    """lx_autotune / lx_tune_force / lx_tune_report / lx_tune_cache: modes, unknown knobs, forced
+    values in the report, decisions read back from a cache file.  No device call."""
+    from leniax_amd import _lib, engine
+    if not _lib.available():
+        pytest.skip('libleniax_amd.so not built')
+    L_ = _lib.lib()
+    assert L_.lx_autotune(7) == _lib.LX_EINVAL
+    prev = engine.autotune('cached')
+    try:
+        assert engine.autotune('cached') == 'cached'
+        with pytest.raises(_lib.LxError, match='unknown knob'):
+            engine.tune_force('no_such_knob', 1)
+        engine.tune_force('fused_probe', 16)
+        assert engine.tune_report()['forced'] == {'fused_probe': 16}
+        engine.tune_force('fused_probe', None)
+        assert engine.tune_report()['forced'] == {}
+        cache = tmp_path / 'schedules.txt'
+        cache.write_text('line_xcd L=256:mode=0:lines=123:K=6:plain 0\n')
+        engine.autotune('cached', cache=str(cache))
+        assert engine.tune_report()['decided'] == {('line_xcd', 'L=256:mode=0:lines=123:K=6:plain'): 0}
+    finally:
+        _lib.call('lx_tune_cache', None)
+        engine.autotune(prev)

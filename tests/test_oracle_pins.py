@@ -377,3 +377,29 @@ def test_planted_sphere_ring_by_squared_distance():
             np.int64), bs.shape[0] - 1)]
         return (k / k.sum())[np.newaxis]
     _fails(pin_sphere_nd, bad)
+
+
+# ---------------------------------------------------------------- runner heuristics (leniax/runner.py:84-104)
+def pin_min_mass_heuristic(fn):
+    """mass >= epsilon (leniax/statistics.py:254-266), inclusive."""
+    got = fn(1e-7, np.asarray([0., 5e-8, 1e-7, 1.]))
+    assert list(np.asarray(got, bool)) == [False, False, True, True], got
+
+
+def pin_max_mass_heuristic(fn):
+    """mass <= 3 initial mass (leniax/statistics.py:269-281), inclusive."""
+    got = fn(np.asarray([1., 1., 1., 2.]), np.asarray([2.9, 3., 3.1, 6.]))
+    assert list(np.asarray(got, bool)) == [True, True, False, True], got
+
+
+def test_runner_heuristics_pinned():
+    pin_min_mass_heuristic(L.min_mass_heuristic)
+    pin_max_mass_heuristic(L.max_mass_heuristic)
+
+
+def test_planted_min_mass_strict():
+    _fails(pin_min_mass_heuristic, lambda eps, m: m > eps)
+
+
+def test_planted_max_mass_factor():
+    _fails(pin_max_mass_heuristic, lambda init, m: m <= 2 * init)
