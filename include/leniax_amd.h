@@ -214,7 +214,8 @@ int lx_irfft2(int32_t ndim, const int32_t* dims, int64_t n, const void* X_d, flo
  * lx_adjoint_pointwise followed by lx_adjoint_potential.
  *
  * lx_adjoint_pointwise: state_d, field_d, potential_d are the step's input state and the field and
- * potential lx_update wrote for it; lambda_d = dL/d(state out).  Writes the direct part of
+ * potential lx_update wrote for it (one channel and one kernel: field_d may be NULL, the field is
+ * then recomputed from the potential); lambda_d = dL/d(state out).  Writes the direct part of
  * dL/d(state in) to lambda_out_d (may alias lambda_d) and dL/d(potential) [B][K][dims] to dpot_d,
  * and ADDS dL/dgf, dL/dw, dL/ddt into grad_gf_d / grad_w_d / grad_dt_d (laid out and strided like
  * gf_d / w_d / dt_d; each may be NULL).  ste != 0: v1's clip passes gradients straight through, as
@@ -237,8 +238,9 @@ int lx_adjoint_pointwise(const lx_rule* rule, int64_t B, const float* state_d, c
  * the same work_d, ADDS dL/dgf, dL/dw, dL/ddt (laid out and strided like gf / w / dt) and, when
  * grad_spec_d is set, sum over steps of rfftn(dL/dU_k) conj(rfftn(A_cin k)) per world ([B][K][half],
  * see lx_adjoint_potential for dL/dK and dL/d(real kernel)), and writes dL/d(initial states) to
- * grad_cells0_d.  Any gradient pointer may be NULL.  The forward must run with the same
- * grad_spec_d-or-NULL choice as the backward (it sizes the workspace). */
+ * grad_cells0_d.  Any gradient pointer may be NULL.  The backward reads what the forward left in
+ * work_d (the checkpoints and the last segment's tensors): it follows the forward with the same
+ * arguments, grad_spec_d NULL or not included. */
 typedef struct lx_grad_run_args {
   const float* cells0_d;   /* [B][C][dims] initial states */
   const void* K_half_d;    /* lx_prepare_spectra layout */
@@ -281,6 +283,19 @@ int lx_half_to_full(int32_t ndim, const int32_t* dims, int64_t n, const void* ha
 int lx_adjoint_potential(const lx_rule* rule, int64_t B, const void* K_half_d, int64_t k_bstride,
                          const float* dpot_d, const void* state_spec_d, void* dspec_d, float* lambda_d,
                          void* work_d, void* stream);
+
+/* The reverse of one update step (lx_adjoint_pointwise then lx_adjoint_potential, with the same
+ * arguments and results): lambda_d holds dL/d(state out) on entry and dL/d(state in) on return;
+ * the gradient scalars and dspec are added into.  With one channel and one kernel the two are one
+ * row pass (dL/dU is computed from the potential and transformed at once, never written; the field
+ * is recomputed from the potential, so field_d and dpot_d are not read and may be NULL); otherwise
+ * dpot_d ([B][K][dims]) and field_d are needed.  work_d:
+ * lx_workspace_size() bytes, adj_work_d: lx_adjoint_workspace_size() bytes. */
+int lx_adjoint_step(const lx_rule* rule, int64_t B, const float* state_d, const float* field_d,
+                    const float* potential_d, const float* gf_d, int64_t gf_bstride, const float* w_d,
+                    int64_t w_bstride, const float* dt_d, int64_t dt_bstride, const void* K_half_d, int64_t k_bstride,
+                    int32_t ste, const void* state_spec_d, void* dspec_d, float* lambda_d, float* dpot_d,
+                    float* grad_gf_d, float* grad_w_d, float* grad_dt_d, void* work_d, void* adj_work_d, void* stream);
 
 /* Batched run (leniax/runner.py:167-215 run_scan_mem_optimized and :119-164 run_scan semantics):
  * every world steps up to max_steps, statistics and stop policy evaluated every step on the device,

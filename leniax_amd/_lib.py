@@ -167,6 +167,8 @@ SIGNATURES = {
                                             _I32, _P, _P, _P, _P, _P, _P, _P]),
     'lx_half_to_full': (ctypes.c_int, [_I32, ctypes.POINTER(_I32), _I64, _P, _P, _F, _P]),
     'lx_adjoint_potential': (ctypes.c_int, [_R, _I64, _P, _I64, _P, _P, _P, _P, _P, _P]),
+    'lx_adjoint_step': (ctypes.c_int, [_R, _I64, _P, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P, _I64, _I32, _P, _P,
+                                       _P, _P, _P, _P, _P, _P, _P, _P]),
     'lx_run_scratch_bytes': (ctypes.c_size_t, [_I64]),
     'lx_random_bits': (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32), _I64, _P, _P]),
     'lx_init_uniform': (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32), _I64, _I64, _P, _P, _P]),

@@ -122,17 +122,33 @@ class _Ctx:
 
     def adjoint(self, A, field, pot, lam, acc, ste, dfield=None, dpot_in=None, FA=None, lam_out=None, dpot=None):
         """dL/dA of one step given lam = dL/d(state out); parameter gradients added into acc.
-        lam_out may be lam itself (updated in place); FA = rfftn(A) when the caller kept it."""
+        lam_out may be lam itself (updated in place); FA = rfftn(A) when the caller kept it.
+        field may be None with one channel and one kernel (recomputed from the potential)."""
         lam_out = torch.empty_like(A) if lam_out is None else lam_out
+        dspec = acc.get('dspec')
+        if dspec is not None and FA is None:
+            FA = rfftn(A, A.dim() - 2)
+        aw = adjoint_workspace(self.rule, self.N)  # held until the call has enqueued its kernels
+        if dfield is None and dpot_in is None:
+            # the step's reverse in one entry point (one row pass for one channel and one kernel)
+            if lam_out is not lam:
+                lam_out.copy_(lam)
+            one = self.C == 1 and self.Keff == 1
+            if not one and dpot is None:
+                dpot = torch.empty_like(pot)
+            _lib.call('lx_adjoint_step', ctypes.byref(self.rule), self.N, engine.ptr(A), engine.ptr(field),
+                      engine.ptr(pot), engine.ptr(self.gf), self.gfs, engine.ptr(self.w), self.ws, engine.ptr(self.dt),
+                      self.dts, engine.ptr(self.Kh), self.kbs, int(bool(ste)),
+                      engine.ptr(FA if dspec is not None else None), engine.ptr(dspec), engine.ptr(lam_out),
+                      engine.ptr(None if one else dpot), engine.ptr(acc['gf']), engine.ptr(acc['w']),
+                      engine.ptr(acc['dt']), engine.ptr(self.workspace()), engine.ptr(aw), engine.stream_ptr())
+            return lam_out
         dpot = torch.empty_like(pot) if dpot is None else dpot
         _lib.call('lx_adjoint_pointwise', ctypes.byref(self.rule), self.N, engine.ptr(A), engine.ptr(field),
                   engine.ptr(pot), engine.ptr(self.gf), self.gfs, engine.ptr(self.w), self.ws, engine.ptr(self.dt),
                   self.dts, engine.ptr(lam), engine.ptr(dfield), engine.ptr(dpot_in), int(bool(ste)),
                   engine.ptr(lam_out), engine.ptr(dpot), engine.ptr(acc['gf']), engine.ptr(acc['w']),
-                  engine.ptr(acc['dt']), engine.ptr(adjoint_workspace(self.rule, self.N)), engine.stream_ptr())
-        dspec = acc.get('dspec')
-        if dspec is not None and FA is None:
-            FA = rfftn(A, A.dim() - 2)
+                  engine.ptr(acc['dt']), engine.ptr(aw), engine.stream_ptr())
         _lib.call('lx_adjoint_potential', ctypes.byref(self.rule), self.N, engine.ptr(self.Kh), self.kbs,
                   engine.ptr(dpot), engine.ptr(FA if dspec is not None else None), engine.ptr(dspec),
                   engine.ptr(lam_out), engine.ptr(self.workspace()), engine.stream_ptr())
@@ -259,6 +275,15 @@ class _RunGraphs:
         for dst, src in ((self.sc.Kh, sc.Kh), (self.sc.gf, sc.gf), (self.sc.w, sc.w), (self.sc.dt, sc.dt)):
             dst.copy_(src)
 
+    def keep_token(self):
+        """Marks the segment buffers as holding one forward's last segment; any later use of the
+        buffers (another run's forward or backward) invalidates the mark."""
+        self.owner = object()
+        return self.owner
+
+    def holds(self, token):
+        return token is not None and getattr(self, 'owner', None) is token
+
     def _graph(self, key, fn):
         g = self.graphs.get(key)
         if g is None:
@@ -273,6 +298,9 @@ class _RunGraphs:
             self.graphs[key] = g
         return g
 
+    def keeps_field(self):
+        return not (self.sc.C == 1 and self.sc.Keff == 1)  # one kernel: the adjoint recomputes the field
+
     def forward_segment(self, L):
         """Replays L steps from seg_A[0]; returns the static buffer holding the result."""
         def fn():
@@ -284,19 +312,34 @@ class _RunGraphs:
         return self.fb[(L - 1) % 2]
 
     def backward_segment(self, L):
+        keep_G = self.keeps_field()
+
         def rec():
             for i in range(L):
                 out = self.seg_A[i + 1] if i + 1 < L else self.nxt
-                self.sc.step(self.seg_A[i], out, self.seg_G[i], self.seg_U[i],
+                self.sc.step(self.seg_A[i], out, self.seg_G[i] if keep_G else None, self.seg_U[i],
                              None if self.seg_F is None else self.seg_F[i])
 
         def adj():
             for i in range(L - 1, -1, -1):
-                self.sc.adjoint(self.seg_A[i], self.seg_G[i], self.seg_U[i], self.lam, self.acc, self.ste,
-                                FA=None if self.seg_F is None else self.seg_F[i], lam_out=self.lam, dpot=self.dpot)
+                self.sc.adjoint(self.seg_A[i], self.seg_G[i] if keep_G else None, self.seg_U[i], self.lam, self.acc,
+                                self.ste, FA=None if self.seg_F is None else self.seg_F[i], lam_out=self.lam,
+                                dpot=self.dpot)
         g_rec = self._graph(('rec', L), rec)
         g_adj = self._graph(('adj', L), adj)
         return g_rec, g_adj
+
+
+def _segment_buffers(sc, c, need_K):
+    """A segment's tensors for the reverse pass: input states (and the segment's output), potentials,
+    fields (several channels or kernels only) and the states' spectra (kernel gradient only)."""
+    dev = sc.Kh.device
+    shape = (sc.N, sc.C) + sc.dims
+    keep_G = not (sc.C == 1 and sc.Keff == 1)
+    return {'A': torch.empty((c + 1,) + shape, device=dev),
+            'G': torch.empty((c,) + shape, device=dev) if keep_G else None,
+            'U': torch.empty((c, sc.N, sc.Keff) + sc.dims, device=dev),
+            'F': torch.empty((c,) + sc.spectra_shape(), dtype=torch.complex64, device=dev) if need_K else None}
 
 
 _GRAPHS = {}
@@ -324,26 +367,63 @@ class _Run(torch.autograd.Function):
         if graphs is None:
             graphs = sc.N * sc.C * int(np.prod(sc.dims)) <= GRAPH_MAX_CELLS
         ckpts = [A]
+        # the last segment's steps keep what the reverse pass reads (potentials, fields, spectra), so
+        # that the backward recomputes only the segments before it
+        nseg = (steps + ckpt - 1) // ckpt
+        need_K = bool(ctx.needs_input_grad[1])
+        keep = any(ctx.needs_input_grad[:5])  # a forward whose backward will run
+        kept = None
         if graphs and steps > 0:
             G = _graphs_for(sc, ckpt, ste, True)
             G.load(sc)
+            if keep:  # every segment graph the backward replays, captured now (their warm-ups use the buffers)
+                for L in sorted({min(ckpt, steps - s * ckpt) for s in range(nseg)}):
+                    G.backward_segment(L)
+            G.owner = None
             cur = A
-            for s in range((steps + ckpt - 1) // ckpt):
+            for s in range(nseg):
                 if s > 0:
                     ckpts.append(cur.clone())
                 G.seg_A[0].copy_(cur)
-                cur = G.forward_segment(min(ckpt, steps - s * ckpt))
+                L = min(ckpt, steps - s * ckpt)
+                if keep and s == nseg - 1:
+                    g_rec, _ = G.backward_segment(L)
+                    g_rec.replay()
+                    cur = G.nxt
+                    kept = G.keep_token()
+                else:
+                    cur = G.forward_segment(L)
             out = cur.clone()
         else:
             bufs = [torch.empty_like(A), torch.empty_like(A)]
             cur = A
-            for t in range(steps):
+            last = (nseg - 1) * ckpt if steps > 0 else 0
+            for t in range(last):
                 if t > 0 and t % ckpt == 0:
                     ckpts.append(cur.clone())
                 nxt = bufs[t % 2]
                 sc.step(cur, nxt, None, None)
                 cur = nxt
-            out = cur.clone() if steps > 0 else A.clone()
+            if steps > 0 and not keep:
+                for t in range(last, steps):
+                    nxt = bufs[t % 2]
+                    sc.step(cur, nxt, None, None)
+                    cur = nxt
+                out = cur.clone()
+            elif steps > 0:
+                if last > 0:
+                    ckpts.append(cur.clone())
+                kept = _segment_buffers(sc, min(ckpt, steps), need_K)
+                kept['A'][0].copy_(cur)
+                L = steps - last
+                out = torch.empty_like(A)
+                for i in range(L):
+                    dst = kept['A'][i + 1] if i + 1 < L else out
+                    sc.step(kept['A'][i], dst, None if kept['G'] is None else kept['G'][i], kept['U'][i],
+                            None if kept['F'] is None else kept['F'][i])
+            else:
+                out = A.clone()
+        ctx.kept = kept
         ctx.sc, ctx.ste, ctx.steps, ctx.ckpt, ctx.ckpts, ctx.dt_orig = sc, ste, steps, ckpt, ckpts, dt
         ctx.graphs = graphs
         ctx.save_for_backward(K_, gf_)
@@ -357,15 +437,21 @@ class _Run(torch.autograd.Function):
         if ctx.graphs and steps > 0:
             G = _graphs_for(sc, c, ctx.ste, True)
             G.load(sc)
+            kept_valid = G.holds(ctx.kept)
             lengths = sorted({min(c, steps - s * c) for s in range((steps - 1) // c + 1)})
+            n_graphs = len(G.graphs)
             segs = {L: G.backward_segment(L) for L in lengths}  # captures (and their warm-ups) first
+            kept_valid = kept_valid and len(G.graphs) == n_graphs  # a warm-up overwrote the buffers
+            G.owner = None
             G.lam.copy_(g_out)
             for v in G.acc.values():
                 v.zero_()
-            for s in range((steps - 1) // c, -1, -1):
+            last = (steps - 1) // c
+            for s in range(last, -1, -1):
                 g_rec, g_adj = segs[min(c, steps - s * c)]
-                G.seg_A[0].copy_(ctx.ckpts[s])
-                g_rec.replay()
+                if not (s == last and kept_valid):  # the forward left the last segment in place
+                    G.seg_A[0].copy_(ctx.ckpts[s])
+                    g_rec.replay()
                 g_adj.replay()
             lam = G.lam.clone()
             acc = {k: v.clone() for k, v in G.acc.items()}
@@ -374,23 +460,29 @@ class _Run(torch.autograd.Function):
             lam = g_out.contiguous().clone()  # updated in place, step after step
             if steps > 0:
                 A0 = ctx.ckpts[0]
-                seg_A = torch.empty((c,) + tuple(A0.shape), dtype=torch.float32, device=A0.device)
-                seg_G = torch.empty_like(seg_A)
-                seg_U = torch.empty((c, sc.N, sc.Keff) + sc.dims, dtype=torch.float32, device=A0.device)
-                seg_F = (torch.empty((c,) + sc.spectra_shape(), dtype=torch.complex64, device=A0.device)
-                         if 'dspec' in acc else None)
-                dpot = torch.empty_like(seg_U[0])
+                seg = ctx.kept  # the last segment, kept by the forward; earlier ones are recomputed into it
+                if seg is None:
+                    seg = _segment_buffers(sc, c, 'dspec' in acc)
+                    seg['stale'] = True
+                if 'dspec' in acc and seg['F'] is None:  # the kernel gradient was not expected: spectra now
+                    seg['F'] = torch.empty((c,) + sc.spectra_shape(), dtype=torch.complex64, device=A0.device)
+                    seg['stale'] = True
+                dpot = torch.empty_like(seg['U'][0])
                 nxt = torch.empty_like(A0)
-                for s in range((steps - 1) // c, -1, -1):
+                last = (steps - 1) // c
+                for s in range(last, -1, -1):
                     t0, t1 = s * c, min(s * c + c, steps)
-                    seg_A[0].copy_(ctx.ckpts[s])
-                    for i in range(t1 - t0):  # recompute the segment, keeping field, potential and spectra
-                        out = seg_A[i + 1] if i + 1 < t1 - t0 else nxt
-                        sc.step(seg_A[i], out, seg_G[i], seg_U[i], None if seg_F is None else seg_F[i])
+                    if s < last or seg.get('stale'):
+                        seg['A'][0].copy_(ctx.ckpts[s])
+                        for i in range(t1 - t0):  # recompute the segment, keeping potential (field, spectra)
+                            out = seg['A'][i + 1] if i + 1 < t1 - t0 else nxt
+                            sc.step(seg['A'][i], out, None if seg['G'] is None else seg['G'][i], seg['U'][i],
+                                    None if seg['F'] is None else seg['F'][i])
+                        seg['stale'] = False
                     for i in range(t1 - t0 - 1, -1, -1):
-                        sc.adjoint(seg_A[i], seg_G[i], seg_U[i], lam, acc, ctx.ste,
-                                   FA=None if seg_F is None else seg_F[i], lam_out=lam, dpot=dpot)
-        ctx.ckpts = None
+                        sc.adjoint(seg['A'][i], None if seg['G'] is None else seg['G'][i], seg['U'][i], lam, acc,
+                                   ctx.ste, FA=None if seg['F'] is None else seg['F'][i], lam_out=lam, dpot=dpot)
+        ctx.ckpts = ctx.kept = None
         return (lam if need[0] else None,
                 sc.grad_K(acc['dspec'], K) if need[1] and 'dspec' in acc else None,
                 sc.grad_gf(acc['gf'], gf) if need[2] else None,
@@ -543,9 +635,10 @@ class _Potential(torch.autograd.Function):
         lam = torch.zeros_like(A)
         dspec = acc.get('dspec')
         FA = rfftn(A, A.dim() - 2) if dspec is not None else None
+        gc, work = g.contiguous(), core._workspace(sc.rule, N)  # both alive across the call
         _lib.call('lx_adjoint_potential', ctypes.byref(sc.rule), N, engine.ptr(sc.Kh), sc.kbs,
-                  engine.ptr(g.contiguous()), engine.ptr(FA), engine.ptr(dspec), engine.ptr(lam),
-                  engine.ptr(core._workspace(sc.rule, N)), engine.stream_ptr())
+                  engine.ptr(gc), engine.ptr(FA), engine.ptr(dspec), engine.ptr(lam),
+                  engine.ptr(work), engine.stream_ptr())
         return lam, (sc.grad_K(dspec, K) if dspec is not None else None), None
 
 
@@ -573,8 +666,9 @@ class _PotentialDirect(torch.autograd.Function):
         need_A, need_K = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
         dA = torch.empty_like(A) if need_A else None
         dk = torch.empty_like(kern) if need_K else None
+        gc = g.contiguous()
         _lib.call('lx_potential_direct_adjoint', ctypes.byref(rule), engine.ptr(A), N, engine.ptr(kern), kh, kw,
-                  engine.ptr(g.contiguous()), engine.ptr(dA), engine.ptr(dk), engine.stream_ptr())
+                  engine.ptr(gc), engine.ptr(dA), engine.ptr(dk), engine.stream_ptr())
         dK = None
         if need_K:
             dK = torch.zeros_like(Kr)
@@ -611,11 +705,12 @@ class _Field(torch.autograd.Function):
         zero_dt = torch.zeros(1, device=pot.device)
         dpot = torch.empty_like(pot)
         scratch = torch.empty_like(field)
+        gc, aw = g.contiguous(), adjoint_workspace(rule, N)  # both alive across the call
         _lib.call('lx_adjoint_pointwise', ctypes.byref(rule), N, engine.ptr(field), engine.ptr(field),
                   engine.ptr(pot), engine.ptr(gf2), core._per_world_stride(gf2, N, 'gf'), engine.ptr(w),
                   core._per_world_stride(w, N, 'weights'), engine.ptr(zero_dt), 0, engine.ptr(field),
-                  engine.ptr(g.contiguous()), None, 1, engine.ptr(scratch), engine.ptr(dpot), engine.ptr(ggf),
-                  engine.ptr(gw), None, engine.ptr(adjoint_workspace(rule, N)), engine.stream_ptr())
+                  engine.ptr(gc), None, 1, engine.ptr(scratch), engine.ptr(dpot), engine.ptr(ggf),
+                  engine.ptr(gw), None, engine.ptr(aw), engine.stream_ptr())
         return dpot, _grad_gf(ggf, gf), gw, None, None
 
 
@@ -651,10 +746,11 @@ class _State(torch.autograd.Function):
         dA = torch.empty_like(A)
         dG = torch.empty_like(A)
         gdt = torch.zeros_like(dt_t)
+        gc, aw = g.contiguous(), adjoint_workspace(rule, N)  # both alive across the call
         _lib.call('lx_adjoint_pointwise', ctypes.byref(rule), N, engine.ptr(A), engine.ptr(G), engine.ptr(G),
-                  engine.ptr(gf), 0, engine.ptr(w), 0, engine.ptr(dt_t), dts, engine.ptr(g.contiguous()), None, None,
+                  engine.ptr(gf), 0, engine.ptr(w), 0, engine.ptr(dt_t), dts, engine.ptr(gc), None, None,
                   int(bool(ste)), engine.ptr(dA), engine.ptr(dG), None, None, engine.ptr(gdt),
-                  engine.ptr(adjoint_workspace(rule, N)), engine.stream_ptr())
+                  engine.ptr(aw), engine.stream_ptr())
         return dA, dG, _grad_dt(gdt, dt), None, None
 
 

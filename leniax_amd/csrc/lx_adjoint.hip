@@ -3,12 +3,15 @@
 // leniax/core.py:13-49, with checkpoints every `ckpt_every` steps and recomputation.
 //
 // lx_grad_run_forward steps the worlds from their initial states, keeping every ckpt_every-th
-// state in the workspace, and writes the final states.  lx_grad_run_backward takes dL/d(final
-// state), walks the segments from the last: it recomputes a segment from its checkpoint, keeping
-// each step's input state, field, potential (and the state's spectra when the kernel gradient is
-// asked for), then runs the step's reverse (lx_adjoint_pointwise, lx_adjoint_potential) from the
-// segment's last step to its first.  Memory: the checkpoints plus one segment, instead of every
-// step's tensors.  The kernels and their order are those of the per-step entry points.
+// state in the workspace, and writes the final states; the last segment's steps also keep their
+// input states, potentials (fields, spectra) in the segment buffers, which the reverse pass reads
+// first.  lx_grad_run_backward takes dL/d(final state) and walks the segments from the last: it
+// recomputes each earlier segment from its checkpoint, keeping each step's input state, potential,
+// field (with several channels or kernels; one channel and one kernel recompute it from the
+// potential) and the state's spectra (when the kernel gradient is asked for), then runs the step's
+// reverse (lx_adjoint_step) from the segment's last step to its first.
+// Memory: the checkpoints plus one segment, instead of every step's tensors.  The kernels and their
+// order are those of the per-step entry points.
 #include <algorithm>
 
 #include "lx_internal.h"
@@ -45,7 +48,8 @@ int grad_layout(const lx_rule* r, int64_t B, int32_t steps, int32_t c, int32_t w
   size_t o = 0;
   l.ckpt = o;      o = up(o + size_t(l.nckpt) * l.state_elems * sizeof(float));
   l.seg_A = o;     o = up(o + size_t(seg + 1) * l.state_elems * sizeof(float));  // + the segment's output
-  l.seg_G = o;     o = up(o + size_t(seg) * l.state_elems * sizeof(float));
+  const bool keepG = !(r->C == 1 && r->K == 1);  // one channel, one kernel: the field is recomputed
+  l.seg_G = o;     o = up(o + (keepG ? size_t(seg) * l.state_elems * sizeof(float) : 0));
   l.seg_U = o;     o = up(o + size_t(seg) * l.pot_elems * sizeof(float));
   l.seg_F = o;     o = up(o + size_t(seg) * l.spec_elems * sizeof(float2));
   l.lam = o;       o = up(o + size_t(l.state_elems) * sizeof(float));
@@ -74,6 +78,22 @@ int copy(float* dst, const float* src, int64_t elems, hipStream_t st) {
   return LX_OK;
 }
 
+// one step of a segment with its potential (field, spectra) kept in the segment buffers
+int segment_step(const lx_rule* rule, const lx_grad_run_args* a, const GradLayout& l, int i, const float* Ai,
+                 float* out, void* work, void* stream) {
+  const bool spec = a->grad_spec_d != nullptr;
+  const bool keepG = !(rule->C == 1 && rule->K == 1);
+  float* Gi = keepG ? fptr(a->work_d, l.seg_G) + int64_t(i) * l.state_elems : nullptr;
+  float* Ui = fptr(a->work_d, l.seg_U) + int64_t(i) * l.pot_elems;
+  if (spec)
+    return lx_update_with_spectra(rule, Ai, a->B, a->K_half_d, a->k_bstride, a->gf_d, a->gf_bstride, a->w_d,
+                                  a->w_bstride, a->dt_d, a->dt_bstride, out, Gi, Ui,
+                                  reinterpret_cast<float2*>(fptr(a->work_d, l.seg_F)) + int64_t(i) * l.spec_elems,
+                                  work, stream);
+  return lx_update(rule, Ai, a->B, a->K_half_d, a->k_bstride, a->gf_d, a->gf_bstride, a->w_d, a->w_bstride,
+                   a->dt_d, a->dt_bstride, out, Gi, Ui, work, stream);
+}
+
 }  // namespace
 
 extern "C" int lx_grad_run_workspace_size(const lx_rule* rule, int64_t B, int32_t steps, int32_t ckpt_every,
@@ -94,19 +114,25 @@ extern "C" int lx_grad_run_forward(const lx_rule* rule, const lx_grad_run_args* 
   if ((rc = grad_layout(rule, a->B, a->steps, a->ckpt_every, a->grad_spec_d != nullptr, l))) return rc;
   hipStream_t st = (hipStream_t)stream;
   float* ck = fptr(a->work_d, l.ckpt);
-  float* pp[2] = {fptr(a->work_d, l.seg_A), fptr(a->work_d, l.seg_A) + l.state_elems};
+  float* sA = fptr(a->work_d, l.seg_A);
   void* work = fptr(a->work_d, l.step_work);
   if ((rc = copy(ck, a->cells0_d, l.state_elems, st))) return rc;
+  const int c = a->ckpt_every, last = (l.nckpt - 1) * c;
   const float* cur = ck;
-  for (int t = 0; t < a->steps; ++t) {
-    float* next;
-    if (t + 1 == a->steps) next = a->cells_d;
-    else if ((t + 1) % a->ckpt_every == 0) next = ck + int64_t((t + 1) / a->ckpt_every) * l.state_elems;
-    else next = pp[t & 1];
+  for (int t = 0; t < last; ++t) {  // the segments before the last: checkpoints only
+    float* next = (t + 1) % c == 0 ? ck + int64_t((t + 1) / c) * l.state_elems
+                                   : sA + int64_t(t & 1) * l.state_elems;
     if ((rc = lx_update(rule, cur, a->B, a->K_half_d, a->k_bstride, a->gf_d, a->gf_bstride, a->w_d, a->w_bstride,
                         a->dt_d, a->dt_bstride, next, nullptr, nullptr, work, stream)))
       return rc;
     cur = next;
+  }
+  // the last segment, its tensors kept for the reverse pass
+  if ((rc = copy(sA, cur, l.state_elems, st))) return rc;
+  const int L = a->steps - last;
+  for (int i = 0; i < L; ++i) {
+    float* out = i + 1 == L ? a->cells_d : sA + int64_t(i + 1) * l.state_elems;
+    if ((rc = segment_step(rule, a, l, i, sA + int64_t(i) * l.state_elems, out, work, stream))) return rc;
   }
   return LX_OK;
 }
@@ -130,35 +156,26 @@ extern "C" int lx_grad_run_backward(const lx_rule* rule, const lx_grad_run_args*
   void* adj_work = fptr(a->work_d, l.adj_work);
   if ((rc = copy(lam, a->lambda_d, l.state_elems, st))) return rc;
   const int c = a->ckpt_every;
+  const bool keepG = !(rule->C == 1 && rule->K == 1);
   for (int seg = l.nckpt - 1; seg >= 0; --seg) {
     const int t0 = seg * c, L = std::min(c, a->steps - t0);
-    // recompute the segment: input state, field, potential (and spectra) of each of its steps
-    if ((rc = copy(sA, ck + int64_t(seg) * l.state_elems, l.state_elems, st))) return rc;
-    for (int i = 0; i < L; ++i) {
-      float* Ai = sA + int64_t(i) * l.state_elems;
-      float* out = sA + int64_t(i + 1) * l.state_elems;
-      float* Gi = sG + int64_t(i) * l.state_elems;
-      float* Ui = sU + int64_t(i) * l.pot_elems;
-      if (spec)
-        rc = lx_update_with_spectra(rule, Ai, a->B, a->K_half_d, a->k_bstride, a->gf_d, a->gf_bstride, a->w_d,
-                                    a->w_bstride, a->dt_d, a->dt_bstride, out, Gi, Ui, sF + int64_t(i) * l.spec_elems,
-                                    work, stream);
-      else
-        rc = lx_update(rule, Ai, a->B, a->K_half_d, a->k_bstride, a->gf_d, a->gf_bstride, a->w_d, a->w_bstride,
-                       a->dt_d, a->dt_bstride, out, Gi, Ui, work, stream);
-      if (rc) return rc;
+    // recompute the segment (the last one the forward kept): input state, potential (field, spectra)
+    // of each of its steps
+    if (seg < l.nckpt - 1) {
+      if ((rc = copy(sA, ck + int64_t(seg) * l.state_elems, l.state_elems, st))) return rc;
+      for (int i = 0; i < L; ++i)
+        if ((rc = segment_step(rule, a, l, i, sA + int64_t(i) * l.state_elems, sA + int64_t(i + 1) * l.state_elems,
+                               work, stream)))
+          return rc;
     }
     // reverse, last step first: lambda updated in place
     for (int i = L - 1; i >= 0; --i) {
       const float* Ai = sA + int64_t(i) * l.state_elems;
-      if ((rc = lx_adjoint_pointwise(rule, a->B, Ai, sG + int64_t(i) * l.state_elems, sU + int64_t(i) * l.pot_elems,
-                                     a->gf_d, a->gf_bstride, a->w_d, a->w_bstride, a->dt_d, a->dt_bstride, lam,
-                                     nullptr, nullptr, a->ste, lam, dpot, a->grad_gf_d, a->grad_w_d, a->grad_dt_d,
-                                     adj_work, stream)))
-        return rc;
-      if ((rc = lx_adjoint_potential(rule, a->B, a->K_half_d, a->k_bstride, dpot,
-                                     spec ? sF + int64_t(i) * l.spec_elems : nullptr, a->grad_spec_d, lam, work,
-                                     stream)))
+      if ((rc = lx_adjoint_step(rule, a->B, Ai, keepG ? sG + int64_t(i) * l.state_elems : nullptr,
+                                sU + int64_t(i) * l.pot_elems, a->gf_d, a->gf_bstride, a->w_d, a->w_bstride, a->dt_d,
+                                a->dt_bstride, a->K_half_d, a->k_bstride, a->ste,
+                                spec ? sF + int64_t(i) * l.spec_elems : nullptr, a->grad_spec_d, lam, dpot,
+                                a->grad_gf_d, a->grad_w_d, a->grad_dt_d, work, adj_work, stream)))
         return rc;
     }
   }

@@ -1729,3 +1729,216 @@ int lx_passes_run(const lx_rule* rule, const lx_run_args* a, hipStream_t st) {
                      r.ctl.carry, X1, X0, a->B, elems);
   return launched("run finish");
 }
+
+// ====================================================================== the reverse of one step
+// lx_adjoint_step: lx_adjoint_pointwise then lx_adjoint_potential, with one channel and one kernel
+// fused into a row pass that computes dL/dU from the potential and lambda (and the state where the
+// update needs it) and transforms it at once: dL/dU is never written, the gradient scalars are
+// summed per block, and lambda is rewritten in that pass only where the step changes it (v2, the
+// true clip).  Several channels or kernels take the two entry points in turn.
+namespace {
+
+struct AdjRowArgs {
+  const float* U;  // [B][rows][W] potentials
+  float* lam;      // [B][rows][W] dL/d(state out) in, the direct part of dL/d(state in) out
+  const float* A;  // states (read for v2 and the true clip)
+  float2* S;       // [B][rows][W/2+1] half spectra of dL/dU along W
+  float* part;     // [B][blocks][4] partial sums of dm, ds, dw, ddt
+  const float* gf;
+  int64_t gf_bs;
+  const float* w;
+  int64_t w_bs;
+  const float* dt;
+  int64_t dt_bs;
+  int64_t rows;    // rows per world
+  int state_fn, average, ste;
+};
+
+// grid (blocks per world, B): a block's row pairs lie in one world (pairs past the world's last are
+// idle lanes: they load pair 0, contribute nothing and store nothing)
+template <int W, int KIND>
+__global__ __launch_bounds__(256) void rows_fwd_adj_kernel(AdjRowArgs a) {
+  using LC = Lanes<W>;
+  constexpr int TPR = LC::TPR, P = LC::P, WH = W / 2 + 1;
+  __shared__ __align__(16) float2 xch_all[LC::XCH];
+  const int lane = threadIdx.x & 63;
+  const int t = threadIdx.x % TPR;
+  float2* xch = xch_all + (threadIdx.x / TPR) * (TPR * LC::XRS);
+  const int64_t b = blockIdx.y;
+  const int64_t pr = int64_t(blockIdx.x) * (256 / TPR) + threadIdx.x / TPR;  // row pair in the world
+  const bool live = pr < a.rows / 2;
+  const int64_t row0 = live ? pr * 2 : 0;
+  const int64_t off = (b * a.rows + row0) * W;
+  const float w = a.w[b * a.w_bs];
+  const float gm = a.gf[b * a.gf_bs], gs = a.gf[b * a.gf_bs + 1];
+  const float inv_s = 1.f / gs, inv_9s2 = 1.f / (9.f * (gs * gs));
+  const float dtw = a.dt[b * a.dt_bs];
+  const float coef = a.average ? w / w : w;
+  const float inv_den = a.average ? 1.f / w : 1.f;
+  // mode 0: lambda passes through (straight-through v1, simple); 1: the true clip; 2: v2
+  const int mode = a.state_fn == LX_STATE_V2 ? 2 : (a.state_fn == LX_STATE_V1 && !a.ste ? 1 : 0);
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  float2 x[P];
+  // one base pointer per array: every point is a constant offset from it
+  const float* pU = a.U + off + t;
+  float* pL = a.lam + off + t;
+  const float* pA = a.A + off + t;
+  // the pointwise reverse of every point the lanes hold, straight-line for each update rule (the
+  // rule and the weighting are uniform: branches around the loop, not in it); the field is the
+  // kernel's growth recomputed from the potential as the forward computes it
+  auto points = [&](auto mode_c, auto avg_c) {
+    constexpr int MODE = decltype(mode_c)::value;
+    constexpr bool AVG = decltype(avg_c)::value != 0;
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      float du[2];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int o = r * W + TPR * m;
+        const float U = pU[o], L = pL[o];
+        const float A = MODE != 0 ? pA[o] : 0.f;
+        float dgdu, dgdm, dgds;
+        const float g = growth_rcp(KIND, gm, gs, inv_s, inv_9s2, U);
+        growth_grad(KIND, gm, gs, inv_s, inv_9s2, U, dgdu, dgdm, dgds);
+        const float wg = mul_nc(w, g);
+        const float G = AVG ? wg / w : wg;
+        float dz = live ? L : 0.f;
+        if constexpr (MODE == 1) {
+          const float z = add_nc(A, mul_nc(dtw, G));
+          dz = (live && z > 0.f && z < 1.f) ? L : 0.f;
+        }
+        const float dG = dtw * dz;
+        const float dg = dG * coef;
+        du[r] = dg * dgdu;
+        acc[0] += dg * dgdm;
+        acc[1] += dg * dgds;
+        acc[2] += dG * (AVG ? (g - G) * inv_den : g);
+        if constexpr (MODE == 2) {
+          if (live) pL[o] = dz * (1.f - dtw);
+          acc[3] += dz * (G - A);
+        } else {
+          if constexpr (MODE == 1) {
+            if (live) pL[o] = dz;
+          }
+          acc[3] += dz * G;
+        }
+      }
+      x[m] = make_float2(du[0], du[1]);
+      if ((m & 1) == 1) __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+  if (mode == 0) {
+    if (a.average) points(IntC<0>{}, IntC<1>{}); else points(IntC<0>{}, IntC<0>{});
+  } else if (mode == 1) {
+    if (a.average) points(IntC<1>{}, IntC<1>{}); else points(IntC<1>{}, IntC<0>{});
+  } else {
+    if (a.average) points(IntC<2>{}, IntC<1>{}); else points(IntC<2>{}, IntC<0>{});
+  }
+  fft_group<W, false>(x, t, lane, xch);
+  wave_lds_fence();
+#pragma unroll
+  for (int e = 0; e < P; ++e) xch[freq_of<W>(t, e)] = x[e];
+  wave_lds_fence();
+  float2* s0 = a.S + (b * a.rows + row0) * int64_t(WH);
+  float2* s1 = s0 + WH;
+#pragma unroll
+  for (int i = 0; i < (WH + TPR - 1) / TPR; ++i) {
+    const int k = t + TPR * i;
+    if (k < WH && live) {
+      const float2 z = xch[k];
+      const float2 zn = xch[k == 0 ? 0 : W - k];
+      s0[k] = make_float2(0.5f * (z.x + zn.x), 0.5f * (z.y - zn.y));
+      s1[k] = make_float2(0.5f * (z.y + zn.y), 0.5f * (zn.x - z.x));
+    }
+  }
+  lx_block_partial<4>(acc, a.part);
+}
+
+}  // namespace
+
+extern "C" int lx_adjoint_step(const lx_rule* rule, int64_t B, const float* state_d, const float* field_d,
+                               const float* potential_d, const float* gf_d, int64_t gf_bstride, const float* w_d,
+                               int64_t w_bstride, const float* dt_d, int64_t dt_bstride, const void* K_half_d,
+                               int64_t k_bstride, int32_t ste, const void* state_spec_d, void* dspec_d,
+                               float* lambda_d, float* dpot_d, float* grad_gf_d, float* grad_w_d, float* grad_dt_d,
+                               void* work_d, void* adj_work_d, void* stream) {
+  int rc = lx_check_rule(rule);
+  if (rc) return rc;
+  if ((state_spec_d == nullptr) != (dspec_d == nullptr))
+    return lx_fail(LX_EINVAL, "state_spec and dspec are given together or not at all");
+  Geom g;
+  if ((rc = make_geom(rule->ndim, rule->dims, g))) return rc;
+  if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
+  if (B == 0) return LX_OK;
+  const bool one = rule->C == 1 && rule->K == 1;
+  if (!state_d || !potential_d || !gf_d || !w_d || !dt_d || !K_half_d || !lambda_d || !work_d || !adj_work_d ||
+      (!one && (!field_d || !dpot_d)))
+    return lx_fail(LX_EINVAL, "NULL pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const int TPRW = lx_row_tpr(g.W);
+  const int64_t pairs_per_block = 256 / TPRW;
+  const bool fused = one && B <= 65535;
+  if (!fused) {
+    if (!dpot_d) return lx_fail(LX_EINVAL, "lx_adjoint_step: this rule needs dpot_d");
+    if ((rc = lx_adjoint_pointwise(rule, B, state_d, field_d, potential_d, gf_d, gf_bstride, w_d, w_bstride, dt_d,
+                                   dt_bstride, lambda_d, nullptr, nullptr, ste, lambda_d, dpot_d, grad_gf_d, grad_w_d,
+                                   grad_dt_d, adj_work_d, stream)))
+      return rc;
+    return lx_adjoint_potential(rule, B, K_half_d, k_bstride, dpot_d, state_spec_d, dspec_d, lambda_d, work_d,
+                                stream);
+  }
+  float *part = nullptr, *ysum = nullptr;
+  lx_adj_work_11(rule, B, adj_work_d, &part, &ysum);
+  float2* X = static_cast<float2*>(work_d);
+  AdjRowArgs ar;
+  ar.U = potential_d; ar.lam = lambda_d; ar.A = state_d; ar.S = X; ar.part = part;
+  ar.gf = gf_d; ar.gf_bs = gf_bstride; ar.w = w_d; ar.w_bs = w_bstride; ar.dt = dt_d; ar.dt_bs = dt_bstride;
+  ar.rows = g.rows; ar.state_fn = rule->state_fn; ar.average = rule->average; ar.ste = ste != 0;
+  const unsigned bpw = unsigned((g.rows / 2 + pairs_per_block - 1) / pairs_per_block);
+  rc = dispatch_len(g.W, [&](auto wc) {
+    constexpr int W = decltype(wc)::value;
+    if (int r2 = ensure_tw<W>(st)) return r2;
+    const dim3 grid(bpw, unsigned(B));
+    switch (rule->gf[0]) {
+      case LX_GF_POLY_QUAD4: hipLaunchKernelGGL((rows_fwd_adj_kernel<W, LX_GF_POLY_QUAD4>), grid, dim3(256), 0, st, ar); break;
+      case LX_GF_GAUSSIAN: hipLaunchKernelGGL((rows_fwd_adj_kernel<W, LX_GF_GAUSSIAN>), grid, dim3(256), 0, st, ar); break;
+      case LX_GF_GAUSSIAN_TARGET:
+        hipLaunchKernelGGL((rows_fwd_adj_kernel<W, LX_GF_GAUSSIAN_TARGET>), grid, dim3(256), 0, st, ar);
+        break;
+      case LX_GF_STEP: hipLaunchKernelGGL((rows_fwd_adj_kernel<W, LX_GF_STEP>), grid, dim3(256), 0, st, ar); break;
+      case LX_GF_STAIRCASE: hipLaunchKernelGGL((rows_fwd_adj_kernel<W, LX_GF_STAIRCASE>), grid, dim3(256), 0, st, ar); break;
+      case LX_GF_TRIANGLE: hipLaunchKernelGGL((rows_fwd_adj_kernel<W, LX_GF_TRIANGLE>), grid, dim3(256), 0, st, ar); break;
+      default: hipLaunchKernelGGL((rows_fwd_adj_kernel<W, LX_GF_IDENTITY>), grid, dim3(256), 0, st, ar); break;
+    }
+    return launched("rows_fwd (adjoint)");
+  });
+  if (rc) return rc;
+  if ((rc = lx_adj_finish_11(B, part, int(bpw), ysum, grad_gf_d, gf_bstride, grad_w_d, w_bstride, grad_dt_d,
+                             dt_bstride, st)))
+    return rc;
+  // the rest of the adjoint potential on the spectra: lines, conj(K) (and the kernel gradient), lines, rows
+  if (g.naxes == 2) {
+    LineArgs la = line_args();
+    la.S = X; la.mode = 1;
+    if ((rc = launch_lines(g, 1, la, B, st))) return rc;
+  }
+  {
+    LineArgs la = line_args();
+    la.S = X; la.U = X;
+    la.Kh = static_cast<const float2*>(K_half_d); la.k_bstride = k_bstride; la.C = 1; la.K = 1; la.conjk = 1;
+    la.cin[0] = 0; la.acc_cin[0] = 0;
+    la.FAin = static_cast<const float2*>(state_spec_d);
+    la.dspec = static_cast<float2*>(dspec_d);
+    la.C_fa = 1;
+    if ((rc = launch_lines(g, 0, la, B, st))) return rc;
+  }
+  if (g.naxes == 2) {
+    LineArgs la = line_args();
+    la.S = X; la.U = X; la.mode = 2;
+    if ((rc = launch_lines(g, 1, la, B, st))) return rc;
+  }
+  RowArgs ra = row_args(rule);
+  ra.U = X; ra.A = lambda_d; ra.A_out = lambda_d;
+  ra.update = 1; ra.sel = 1; ra.average = 0; ra.state_fn = LX_STATE_SIMPLE;
+  return launch_rows_inv(g, ra, B, st);
+}

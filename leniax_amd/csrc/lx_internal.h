@@ -271,6 +271,35 @@ LX_DEV lx_sums lx_block_pair_sums(const float* __restrict__ rec, int C, int64_t 
   return s;
 }
 
+// Block-wide sums of NV per-thread values (256-thread blocks), written to the block's slot of a
+// partial-sum array: part[(blockIdx.y * gridDim.x + blockIdx.x) * NV + i], summed later in a fixed
+// order (lx_adj_finish).
+template <int NV>
+LX_DEV void lx_block_partial(float (&v)[NV], float* part) {
+  __shared__ float red[NV][4];
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const float s = wave_sum(v[i]);
+    if (ln == 0) red[i][wv] = s;
+  }
+  __syncthreads();
+  float* dst = part + (int64_t(blockIdx.y) * gridDim.x + blockIdx.x) * NV;
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+    if (threadIdx.x == i) dst[i] = red[i][0] + red[i][1] + red[i][2] + red[i][3];
+}
+
+// Lanes per row transform of the row passes (lx_fft.hip): 4 up to 64 points, 8 up to 192, 16 above.
+constexpr int lx_row_tpr(int W) { return W <= 64 ? 4 : (W <= 192 ? 8 : 16); }
+
+// The gradient scalars of a one-channel one-kernel reverse step from per-block partial sums
+// part [B][per_img][4] (dm, ds, dw, ddt): summed per world in block order, then added into the
+// gradients (lx_plain.hip).  ysum: B * 4 floats of scratch.
+int lx_adj_finish_11(int64_t B, const float* part, int per_img, float* ysum, float* g_gf, int64_t gf_bs, float* g_w,
+                     int64_t w_bs, float* g_dt, int64_t dt_bs, hipStream_t st);
+void lx_adj_work_11(const lx_rule* r, int64_t B, void* work, float** part, float** ysum);
+
 // Autotuner (lx_tune.hip): knob values by launch shape.  lx_tune_lookup gives a forced or recorded
 // value (1) or nothing (0); lx_tune_should_time says whether a launcher may time its candidates now
 // (tuning on, stream not capturing).

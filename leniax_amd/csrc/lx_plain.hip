@@ -348,22 +348,9 @@ LX_DEV float adj_dz(const AdjArgs& a, float A, float G, float lam, float dtw) {
   return (z > 0.f && z < 1.f) ? lam : 0.f;
 }
 
-// block-wide sums of NV per-thread values, written to the block's slot of the partial sums:
-// part[(blockIdx.y * gridDim.x + blockIdx.x) * NV + i] (summed later in a fixed order)
 template <int NV>
 LX_DEV void block_partial(float (&v)[NV], float* part) {
-  __shared__ float red[NV][4];
-  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    const float s = wave_sum(v[i]);
-    if (ln == 0) red[i][wv] = s;
-  }
-  __syncthreads();
-  float* dst = part + (int64_t(blockIdx.y) * gridDim.x + blockIdx.x) * NV;
-#pragma unroll
-  for (int i = 0; i < NV; ++i)
-    if (threadIdx.x == i) dst[i] = red[i][0] + red[i][1] + red[i][2] + red[i][3];
+  lx_block_partial<NV>(v, part);
 }
 
 // grid (blocks per image, B * K): dL/dU_k and the growth / weight gradients; reads lambda before the
@@ -420,9 +407,13 @@ __global__ __launch_bounds__(256) void adj_growth_kernel(AdjArgs a) {
 }
 
 // One channel, one kernel: the whole pointwise reverse in one pass (grid: blocks per image x B).
-// Reads A, G, U and lambda once; lambda_out may alias lambda (each thread reads before it writes).
-// KIND is the kernel's growth function, so that its value and derivative are selected at compile
-// time instead of per cell (same expressions as growth_rcp / growth_grad: the same values).
+// Reads the potential and lambda, and only what the rule needs beside them: the state for v2 and
+// the true clip, the field when the caller kept it -- without it, the field is the kernel's growth
+// recomputed from the potential as the forward computes it, (w g) / w (mean) or w g (sum).  lambda
+// is not rewritten when the step passes it through unchanged (straight-through v1 and simple, in
+// place).  lambda_out may alias lambda (each thread reads a cell before it writes it).  KIND is the
+// kernel's growth function, so that its value and derivative are selected at compile time instead
+// of per cell (same expressions as growth_rcp / growth_grad: the same values).
 template <int KIND>
 __global__ __launch_bounds__(256) void adj_11_kernel(AdjArgs a) {
   const int64_t b = blockIdx.y;
@@ -433,6 +424,10 @@ __global__ __launch_bounds__(256) void adj_11_kernel(AdjArgs a) {
   constexpr int kind = KIND;
   const float coef = a.average ? w / w : w;      // dG -> dg (weighted mean of one kernel: w / w)
   const float inv_den = a.average ? 1.f / w : 1.f;
+  const bool needA = a.state_fn == LX_STATE_V2 || (a.state_fn == LX_STATE_V1 && !a.ste);
+  const bool haveG = a.G != nullptr;
+  const bool passes = a.state_fn == LX_STATE_SIMPLE || (a.state_fn == LX_STATE_V1 && a.ste);
+  const bool writeLam = !(passes && a.lam_out == a.lam);
   const int64_t base = b * a.cells;
   float acc[4] = {0.f, 0.f, 0.f, 0.f};  // dm, ds, dw, ddt
   // NB cells per round, all loaded before any store: the stores may alias the loads (lambda_out
@@ -446,7 +441,9 @@ __global__ __launch_bounds__(256) void adj_11_kernel(AdjArgs a) {
     for (int u = 0; u < NB; ++u) {
       const int64_t p = p0 + u * S;
       const int64_t q = base + (p < a.cells ? p : p0);  // an in-bounds cell for the tail
-      vA[u] = a.A[q]; vG[u] = a.G[q]; vU[u] = a.U[q]; vL[u] = a.lam[q];
+      vA[u] = needA ? a.A[q] : 0.f;
+      vG[u] = haveG ? a.G[q] : 0.f;
+      vU[u] = a.U[q]; vL[u] = a.lam[q];
       vF[u] = a.dfield ? a.dfield[q] : 0.f;
       vP[u] = a.dpot_in ? a.dpot_in[q] : 0.f;
     }
@@ -454,13 +451,15 @@ __global__ __launch_bounds__(256) void adj_11_kernel(AdjArgs a) {
     for (int u = 0; u < NB; ++u) {
       const int64_t p = p0 + u * S;
       if (p >= a.cells) break;
-      const float A = vA[u], G = vG[u], U = vU[u];
-      const float dz = adj_dz(a, A, G, vL[u], dtw);
-      float dG = dtw * dz;
-      if (a.dfield) dG += vF[u];
+      const float A = vA[u], U = vU[u];
       float dgdu, dgdm, dgds;
       const float g = growth_rcp(kind, gm, gs, inv_s, inv_9s2, U);  // as the forward passes
       growth_grad(kind, gm, gs, inv_s, inv_9s2, U, dgdu, dgdm, dgds);
+      const float wg = mul_nc(w, g);
+      const float G = haveG ? vG[u] : (a.average ? wg / w : wg);
+      const float dz = adj_dz(a, A, G, vL[u], dtw);
+      float dG = dtw * dz;
+      if (a.dfield) dG += vF[u];
       const float dg = dG * coef;
       a.dU[base + p] = a.dpot_in ? fmaf(dg, dgdu, vP[u]) : dg * dgdu;
       acc[0] += dg * dgdm;
@@ -470,7 +469,7 @@ __global__ __launch_bounds__(256) void adj_11_kernel(AdjArgs a) {
         a.lam_out[base + p] = dz * (1.f - dtw);
         acc[3] += dz * (G - A);
       } else {
-        a.lam_out[base + p] = dz;
+        if (writeLam) a.lam_out[base + p] = dz;
         acc[3] += dz * G;
       }
     }
@@ -556,12 +555,24 @@ unsigned adj_blocks_per_image(int64_t cells) {
   return unsigned(std::min<int64_t>(std::max<int64_t>((cells + 1023) / 1024, 1), 256));
 }
 
+// partial-sum rows per image the workspace holds: the pointwise kernels' blocks, and for one channel
+// and one kernel the blocks of lx_adjoint_step's fused row pass (a row pair per lane group)
+int64_t adj_rows_per_image(const lx_rule* r) {
+  int64_t per = adj_blocks_per_image(cells_of(r));
+  if (r->C == 1 && r->K == 1) {
+    const int W = r->dims[r->ndim - 1];
+    const int64_t rows = cells_of(r) / W;
+    per = std::max<int64_t>(per, (rows / 2 + 256 / lx_row_tpr(W) - 1) / (256 / lx_row_tpr(W)));
+  }
+  return per;
+}
+
 // partial-sum layout in the workspace: growth (or one-kernel) rows, state rows, then their sums
 struct AdjWork {
   size_t part_g, part_s, ysum_g, ysum_s, total;
 };
 AdjWork adj_work_layout(const lx_rule* r, int64_t B) {
-  const int64_t per = adj_blocks_per_image(cells_of(r));
+  const int64_t per = adj_rows_per_image(r);
   const bool one = r->C == 1 && r->K == 1;
   const int64_t yg = one ? B : B * r->K, ys = one ? 0 : B * r->C;
   const int ng = one ? 4 : 2 + LX_MAX_C, ns = one ? 0 : 1;  // row widths of block_partial<4> / <2 + LX_MAX_C>
@@ -576,6 +587,24 @@ AdjWork adj_work_layout(const lx_rule* r, int64_t B) {
 }
 
 }  // namespace
+
+int lx_adj_finish_11(int64_t B, const float* part, int per_img, float* ysum, float* g_gf, int64_t gf_bs, float* g_w,
+                     int64_t w_bs, float* g_dt, int64_t dt_bs, hipStream_t st) {
+  if (!g_gf && !g_w && !g_dt) return LX_OK;
+  AdjArgs a{};
+  a.g_gf = g_gf; a.gf_bs = gf_bs; a.g_w = g_w; a.w_bs = w_bs; a.g_dt = g_dt; a.dt_bs = dt_bs;
+  a.C = 1; a.K = 1; a.ysum_g = ysum; a.ysum_s = nullptr;
+  hipLaunchKernelGGL(adj_ysum_kernel, dim3(grid_for(B * 4, 256)), dim3(256), 0, st, part, ysum, B, 4, per_img);
+  hipLaunchKernelGGL(adj_targets_kernel, dim3(1), dim3(256), 0, st, a, B, 1);
+  return launched("adjoint gradient sums");
+}
+
+// the partial-sum rows and sums of the fused one-kernel step inside an lx_adjoint_workspace_size workspace
+void lx_adj_work_11(const lx_rule* r, int64_t B, void* work, float** part, float** ysum) {
+  const AdjWork l = adj_work_layout(r, B);
+  *part = reinterpret_cast<float*>(static_cast<char*>(work) + l.part_g);
+  *ysum = reinterpret_cast<float*>(static_cast<char*>(work) + l.ysum_g);
+}
 
 extern "C" int lx_adjoint_workspace_size(const lx_rule* rule, int64_t B, size_t* bytes) {
   int rc = lx_check_rule(rule);
@@ -595,8 +624,9 @@ extern "C" int lx_adjoint_pointwise(const lx_rule* rule, int64_t B, const float*
   if (rc) return rc;
   if (B < 0) return lx_fail(LX_EINVAL, "B < 0");
   if (B == 0) return LX_OK;
-  if (!state_d || !field_d || !potential_d || !gf_d || !w_d || !dt_d || !lambda_d || !lambda_out_d || !dpot_d ||
-      !work_d)
+  const bool one = rule->C == 1 && rule->K == 1;
+  if (!state_d || (!field_d && !one) || !potential_d || !gf_d || !w_d || !dt_d || !lambda_d || !lambda_out_d ||
+      !dpot_d || !work_d)
     return lx_fail(LX_EINVAL, "NULL pointer");
   if (B * std::max(rule->C, rule->K) > 65535) return lx_fail(LX_ENOTSUP, "lx_adjoint_pointwise: B * max(C, K) > 65535");
   AdjArgs a;
@@ -619,7 +649,6 @@ extern "C" int lx_adjoint_pointwise(const lx_rule* rule, int64_t B, const float*
   // workspace (no atomics), summed afterwards in a fixed order
   const unsigned per_img = adj_blocks_per_image(a.cells);
   hipStream_t st = (hipStream_t)stream;
-  const bool one = rule->C == 1 && rule->K == 1;
   if (one) {
     const dim3 grid(per_img, unsigned(B));
     switch (rule->gf[0]) {

@@ -100,7 +100,8 @@ def check_heuristics(stats: Dict) -> torch.Tensor:
     T, N = rows.shape[:2]
     cont = torch.empty((T, N), dtype=torch.float32, device=rows.device)
     n = torch.empty((N,), dtype=torch.float32, device=rows.device)
-    _lib.call('lx_check_heuristics', engine.ptr(rows.contiguous()), T, N, C, engine.ptr(cont), engine.ptr(n),
+    rows = rows.contiguous()
+    _lib.call('lx_check_heuristics', engine.ptr(rows), T, N, C, engine.ptr(cont), engine.ptr(n),
               engine.stream_ptr())
     return cont
 

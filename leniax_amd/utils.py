@@ -125,14 +125,9 @@ def _host(x):
     return np.asarray(x.detach().cpu() if isinstance(x, torch.Tensor) else x, dtype=np.float32)
 
 
-def _roll_world(cells, shift):
-    """cells [C, *world] rolled by -shift along the world axes (center_world of one world)."""
-    return np.roll(cells, tuple(-int(v) for v in shift), axis=tuple(range(1, cells.ndim)))
-
-
 def crop_zero(kernels):
     """leniax/utils.py:296-317: drop the all-zero lines of a [C, H, W] or [C, D, H, W] array
-    (host numpy, the saved-creature format)."""
+    (host numpy: the direct-convolution kernels of leniax/kernels.py:117-118)."""
     k = _host(kernels)
     z = k == 0
     if k.ndim == 3:
@@ -143,41 +138,6 @@ def crop_zero(kernels):
         k = k[:, :, ~np.all(z, axis=(0, 1, 3))]
         return k[:, :, :, ~np.all(z, axis=(0, 1, 2))]
     raise ValueError("Can't handle more than 3 dimensions")
-
-
-def auto_center(cells):
-    """leniax/utils.py:320-349: roll a state [C, *world] so that its mass centroid (coordinates
-    relative to size // 2, rounded half to even) lands on the middle cell."""
-    c = _host(cells)
-    world = c.shape[1:]
-    coords = np.indices(world)
-    mid = np.asarray([s // 2 for s in world]).reshape((-1,) + (1,) * len(world))
-    centred = coords - mid
-    m00 = c.sum(dtype=np.float32)
-    mx = np.asarray([(c * co).sum(dtype=np.float32) for co in centred], np.float32)
-    shift = np.round(mx / np.float32(m00 + np.float32(1e-7))).astype(int)
-    return _roll_world(c, shift)
-
-
-def center_and_crop(cells):
-    """leniax/utils.py:352-388: a state that touches both borders of an axis without filling it
-    is first rolled by leniax's pre-shift (the first axis's half size, on every such axis, as
-    leniax does), then mass-centred and cropped to its non-zero box; an empty result is one zero
-    cell per channel."""
-    c = _host(cells)
-    world = c.shape[1:]
-    half0 = world[0] // 2
-    pre = [0] * len(world)
-    for i in range(len(world)):
-        axes = tuple(a for a in range(len(world) + 1) if a != i + 1)
-        nz = ~np.all(c == 0, axis=axes)
-        if nz[0] and nz[-1] and not nz.prod():
-            pre[i] = half0
-    c = _roll_world(c, pre)
-    c = crop_zero(auto_center(c))
-    if 0 in c.shape:
-        c = np.zeros([c.shape[0]] + [1] * (c.ndim - 1), np.float32)
-    return c
 
 
 def seed_everything(seed: int):

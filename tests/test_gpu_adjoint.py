@@ -463,3 +463,69 @@ def test_grad_run_c_abi_equals_autograd_run(ste, ckpt, spec):
         # per-world accumulator divided by H W
         full = AG.half_to_full(gspec.sum(dim=0), (H, W), 1.0 / (H * W))
         assert _rel(_np(full), _np(tK.grad)[0, 0]) < 1e-6
+
+
+@pytest.mark.parametrize('dims,state_fn,ste', [((64, 96), 'v1', True), ((64, 96), 'v1', False),
+                                              ((64, 96), 'v2', True), ((32, 48, 32), 'v1', True)])
+def test_fused_adjoint_step_equals_pointwise_then_potential(dims, state_fn, ste):
+    """lx_adjoint_step with one channel and one kernel (dL/dU computed and transformed in one row
+    pass, never written) against lx_adjoint_pointwise + lx_adjoint_potential on the same inputs:
+    dL/d(state in) bit for bit (the same per-cell arithmetic and transforms), the gradient scalars
+    to their summation order, with the kernel-spectrum gradient."""
+    import ctypes
+    from leniax_amd import _lib, autograd as AG, engine
+    print('fused adjoint step', dims, state_fn, ste, flush=True)
+    rng = np.random.default_rng(21)
+    B = 3
+    rule = engine.make_rule(list(dims), 1, [0], [_lib.GROWTH_IDS['gaussian']], _lib.STATE_FN[state_fn], 1)
+    A = _dev(rng.uniform(size=(B, 1) + dims).astype(np.float32))
+    U = _dev(rng.uniform(0, 0.4, size=(B, 1) + dims).astype(np.float32))
+    lam0 = _dev(rng.normal(size=(B, 1) + dims).astype(np.float32))
+    gf = _dev(np.array([[0.2, 0.05]], np.float32))
+    w = _dev(np.array([[0.7]], np.float32))
+    dt = _dev(np.array([0.1], np.float32))
+    k = np.zeros(dims)
+    k[tuple(slice(d // 2 - 3, d // 2 + 3) for d in dims)] = rng.uniform(size=(6,) * len(dims))
+    K = np.fft.fftn(np.fft.fftshift(k / k.sum()))[None, None, None].astype(np.complex64)
+    Kh = engine.prepare_spectra(rule, K, [0])
+    FA = AG.rfftn(A, len(dims))
+    G = _dev(np.zeros((B, 1) + dims, np.float32))
+    # the field as the forward computes it, for the two-call path (weighted mean of one kernel)
+    _lib.call('lx_field', ctypes.byref(rule), engine.ptr(U), B, engine.ptr(gf), 0, engine.ptr(w), 0, engine.ptr(G),
+              engine.stream_ptr())
+    nb = ctypes.c_size_t(0)
+    _lib.call('lx_workspace_size', ctypes.byref(rule), B, ctypes.byref(nb))
+    work = torch.empty(nb.value, dtype=torch.uint8, device='cuda')
+    _lib.call('lx_adjoint_workspace_size', ctypes.byref(rule), B, ctypes.byref(nb))
+    aw = torch.empty(nb.value, dtype=torch.uint8, device='cuda')
+    st = engine.stream_ptr()
+    outs = []
+    for fused in (False, True):
+        lam = lam0.clone()
+        ggf, gw, gdt = torch.zeros_like(gf), torch.zeros_like(w), torch.zeros_like(dt)
+        dspec = torch.zeros((B, 1) + tuple(FA.shape[2:]), dtype=torch.complex64, device='cuda')
+        dpot = torch.empty_like(U)
+        if fused:
+            _lib.call('lx_adjoint_step', ctypes.byref(rule), B, engine.ptr(A), None, engine.ptr(U), engine.ptr(gf), 0,
+                      engine.ptr(w), 0, engine.ptr(dt), 0, engine.ptr(Kh), 0, int(ste), engine.ptr(FA),
+                      engine.ptr(dspec), engine.ptr(lam), None, engine.ptr(ggf), engine.ptr(gw), engine.ptr(gdt),
+                      engine.ptr(work), engine.ptr(aw), st)
+        else:
+            _lib.call('lx_adjoint_pointwise', ctypes.byref(rule), B, engine.ptr(A), engine.ptr(G), engine.ptr(U),
+                      engine.ptr(gf), 0, engine.ptr(w), 0, engine.ptr(dt), 0, engine.ptr(lam), None, None, int(ste),
+                      engine.ptr(lam), engine.ptr(dpot), engine.ptr(ggf), engine.ptr(gw), engine.ptr(gdt),
+                      engine.ptr(aw), st)
+            _lib.call('lx_adjoint_potential', ctypes.byref(rule), B, engine.ptr(Kh), 0, engine.ptr(dpot),
+                      engine.ptr(FA), engine.ptr(dspec), engine.ptr(lam), engine.ptr(work), st)
+        torch.cuda.synchronize()
+        outs.append([_np(x) for x in (lam, ggf, gw, gdt, dspec)])
+    (l0, g0, w0, d0, s0), (l1, g1, w1, d1, s1) = outs
+    assert np.array_equal(l0, l1)
+    assert np.array_equal(s0, s1)
+    # the scalars are sums over every cell in another order: float32 rounding of sums whose terms
+    # are bounded by |lambda| (times |G| <= 1, dt, the growth derivatives) cancel to much smaller
+    # totals, so the bound scales with sum |lambda|
+    scale = float(np.abs(_np(lam0)).sum())
+    for x, y in ((g0, g1), (w0, w1), (d0, d1)):
+        assert np.allclose(x, y, rtol=0, atol=1e-7 * scale * max(1., float(np.abs(y).max()))), (x, y)
+    assert np.abs(g1).max() > 0 and np.abs(d1).max() > 0

@@ -503,8 +503,8 @@ def test_search_for_init_direct_potential_follows_leniax_loop():
     Kd = np.asarray(Kd.cpu() if hasattr(Kd, 'cpu') else Kd, np.float32)
     pad = L.conv_padding(Kd.shape)
     sfn = L.build_compute_stats_fn([48, 48], 7, 10, np.float32)
-    gf = np.asarray(mp.get_gf_params(), np.float32)
-    w = np.asarray(mp.get_kernels_weight_per_channel(), np.float32)
+    gf = mp.get_gf_params().cpu().numpy().astype(np.float32)
+    w = mp.get_kernels_weight_per_channel().cpu().numpy().astype(np.float32)
     cur, best_o, runs = 0, None, []
     for j in range(3):
         A = states[j]

@@ -137,6 +137,8 @@ def test_empty_batches_are_accepted_with_null_pointers():
         'lx_half_to_full': lambda B: L_.lx_half_to_full(2, dims, B, N, N, 1., N),
         'lx_kernel_spectrum': lambda B: L_.lx_kernel_spectrum(2, dims, B, N, N, N, N),
         'lx_adjoint_potential': lambda B: L_.lx_adjoint_potential(r, B, N, 0, N, N, N, N, N, N),
+        'lx_adjoint_step': lambda B: L_.lx_adjoint_step(r, B, N, N, N, N, 0, N, 0, N, 0, N, 0, 1, N, N, N, N, N, N, N,
+                                                        N, N, N),
         'lx_adjoint_pointwise': lambda B: L_.lx_adjoint_pointwise(r, B, N, N, N, N, 0, N, 0, N, 0, N, N, N, 0, N, N,
                                                                   N, N, N, N, N),
         'lx_random_bits': lambda B: L_.lx_random_bits(key, B, N, N),
@@ -166,11 +168,17 @@ def test_empty_batches_are_accepted_with_null_pointers():
         g.B, g.steps = 0, 0
         assert fn() != _lib.LX_OK, name
         g.steps = 8
-    # checkpoints (2 of 8 steps at 4) + a segment of 4 steps (5 states, 4 fields, potentials, spectra)
+    # checkpoints (2 of 8 steps at 4) + a segment of 4 steps (5 states, 4 potentials and spectra; one
+    # channel and one kernel keep no field) + lambda and dL/dU
     g.B = 3
     _lib.call('lx_grad_run_workspace_size', r, 3, 8, 4, 1, ctypes.byref(nbytes))
     state = 3 * 64 * 64 * 4
-    assert nbytes.value >= (2 + 5 + 4 + 4 + 1 + 1) * state + 4 * 3 * 64 * 33 * 8
+    assert nbytes.value >= (2 + 5 + 4 + 1 + 1) * state + 4 * 3 * 64 * 33 * 8
+    r2 = _rule(_lib, [64, 64], C=2, K=2)  # several channels: the segment keeps its fields too
+    r2.cin[1] = 1
+    nb2 = ctypes.c_size_t(0)
+    _lib.call('lx_grad_run_workspace_size', ctypes.byref(r2), 3, 8, 4, 0, ctypes.byref(nb2))
+    assert nb2.value >= (2 + 5 + 4 + 4 + 1 + 1) * 2 * state
     # every declared entry point that takes a batch is exercised above
     batch_free = {'lx_last_error', 'lx_version', 'lx_prepare_spectra', 'lx_workspace_size', 'lx_carry_bytes',
                   'lx_adjoint_workspace_size', 'lx_autotune', 'lx_tune_cache', 'lx_tune_force', 'lx_tune_report',
@@ -292,10 +300,9 @@ def test_compress_round_trips():
     np.testing.assert_allclose(codec.rle_decode(code, 3), a, atol=1e-7)
 
 
-def test_crop_and_centre_utilities_known_answers():
-    """leniax/utils.py:296-388 (saved-creature format): crop_zero drops the all-zero lines;
-    auto_center moves the mass centroid onto the middle cell; center_and_crop returns a creature
-    that wraps around a border as its own contiguous box; an empty state gives one zero cell."""
+def test_crop_zero_known_answers():
+    """leniax/utils.py:296-317: crop_zero drops the all-zero lines of every axis (used by the
+    direct-convolution kernels, leniax/kernels.py:117-118)."""
     from leniax_amd import utils as U
     rng = np.random.default_rng(4)
     patch = rng.uniform(0.1, 1., (1, 5, 6)).astype(np.float32)
@@ -307,18 +314,6 @@ def test_crop_and_centre_utilities_known_answers():
     assert U.crop_zero(w4).shape == (2, 2, 3, 5)
     with pytest.raises(ValueError):
         U.crop_zero(np.zeros((1, 2, 2, 2, 2)))
-    # a symmetric blob (centroid exactly at its centre cell) moves onto the middle cell
-    blob = np.zeros((1, 64, 64), np.float32)
-    blob[0, 9:12, 40:43] = 1.  # centre (10, 41); middle (32, 32)
-    c = U.auto_center(blob)
-    np.testing.assert_array_equal(c, np.roll(blob, (22, -9), axis=(1, 2)))
-    assert c[0, 31:34, 31:34].sum() == 9.
-    # a creature across the right/left border comes back whole
-    big = np.zeros((1, 128, 128), np.float32)
-    big[0, 60:65, 125:128] = patch[0, :, :3]
-    big[0, 60:65, 0:3] = patch[0, :, 3:]
-    np.testing.assert_array_equal(U.center_and_crop(big), patch)
-    np.testing.assert_array_equal(U.center_and_crop(np.zeros((2, 16, 16), np.float32)), np.zeros((2, 1, 1)))
 
 
 def test_tuner_controls_without_a_device(tmp_path):
