@@ -350,6 +350,16 @@ int lx_run_scratch_size(const lx_rule* rule, int64_t B, int32_t slots, size_t* b
 size_t lx_run_scratch_bytes(int64_t B);
 int lx_run(const lx_rule* rule, const lx_run_args* args, void* stream);
 
+/* lx_run on the fused path with no device allocation and no workspace: args->scratch_d is ignored
+ * and a fixed device array of the library (one per device, kept until the process ends) holds the
+ * run's queues and parked carries.  No counterpart in leniax: it is for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, hosts that forbid hipMalloc).  At most
+ * LX_RUN_STATIC_MAX_WORLDS worlds (LX_EINVAL beyond); rules without a fused kernel are LX_ENOTSUP.
+ * The autotuner times nothing inside it (forced, recorded or default schedules only).
+ * Every call on a device shares that device's array: calls that overlap on one device race. */
+#define LX_RUN_STATIC_MAX_WORLDS 65536
+int lx_run_static(const lx_rule* rule, const lx_run_args* args, void* stream);
+
 /* Autotuner.  The passes have variants that compute the same thing in different schedules:
  *   "conv_staged"  the convolution pass at 256 points staged through LDS (1) or read in the lane
  *                  groups' layout (0);

@@ -103,6 +103,7 @@ class lx_run_args(ctypes.Structure):
 
 LX_RUN_FUSED = 1
 LX_RUN_PASSES = 2
+LX_RUN_STATIC_MAX_WORLDS = 65536
 TUNE = {'off': 0, 'on': 1, 'cached': 2}
 KNOBS = ('conv_staged', 'line_xcd', 'lines96_tpr', 'fused_probe', 'graph_steps')
 
@@ -179,6 +180,7 @@ SIGNATURES = {
     'lx_run_supported': (ctypes.c_int, [_R]),
     'lx_run_scratch_size': (ctypes.c_int, [_R, _I64, _I32, ctypes.POINTER(ctypes.c_size_t)]),
     'lx_run': (ctypes.c_int, [_R, ctypes.POINTER(lx_run_args), _P]),
+    'lx_run_static': (ctypes.c_int, [_R, ctypes.POINTER(lx_run_args), _P]),
     'lx_autotune': (ctypes.c_int, [_I32]),
     'lx_tune_cache': (ctypes.c_int, [ctypes.c_char_p]),
     'lx_tune_force': (ctypes.c_int, [ctypes.c_char_p, _I32]),

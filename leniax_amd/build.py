@@ -11,7 +11,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 OUT = os.path.join(HERE, 'libleniax_amd.so')
-SOURCES = ['lx_api.hip', 'lx_fused2d.hip', 'lx_fused2d_n32.hip', 'lx_fused2d_n64.hip', 'lx_fused2d_n128.hip', 'lx_plain.hip', 'lx_fft.hip', 'lx_direct.hip', 'lx_kernels.hip', 'lx_init.hip', 'lx_adjoint.hip', 'lx_tune.hip']
+SOURCES = ['lx_api.hip', 'lx_fused2d.hip', 'lx_fused2d_n32.hip', 'lx_fused2d_n64.hip', 'lx_fused2d_n128.hip', 'lx_plain.hip', 'lx_fft.hip', 'lx_direct.hip', 'lx_kernels.hip', 'lx_init.hip', 'lx_static.hip', 'lx_adjoint.hip', 'lx_tune.hip']
 ARCH = os.environ.get('LENIAX_AMD_ARCH', 'gfx950')
 
 FLAGS = [

@@ -307,6 +307,12 @@ int lx_tune_mode();
 int lx_tune_lookup(const char* knob, const char* key, int* value);
 void lx_tune_record(const char* knob, const char* key, int value);
 bool lx_tune_should_time(hipStream_t st);
+// While one of these lives on a thread, lx_tune_should_time is false there: the launches it covers
+// use forced, recorded or default schedules and create no events.
+struct LxTuneQuiet {
+  LxTuneQuiet();
+  ~LxTuneQuiet();
+};
 
 // Milliseconds per call of enqueue() on stream st: one untimed call, then `reps` timed ones between
 // two events (the caller's stream waits for them; < 0 on a HIP error).

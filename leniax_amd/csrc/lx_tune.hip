@@ -92,8 +92,15 @@ void lx_tune_record(const char* knob, const char* key, int value) {
   }
 }
 
+namespace {
+thread_local int quiet = 0;
+}
+
+LxTuneQuiet::LxTuneQuiet() { ++quiet; }
+LxTuneQuiet::~LxTuneQuiet() { --quiet; }
+
 bool lx_tune_should_time(hipStream_t st) {
-  if (lx_tune_mode() != LX_TUNE_ON) return false;
+  if (quiet > 0 || lx_tune_mode() != LX_TUNE_ON) return false;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
   return true;

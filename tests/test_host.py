@@ -182,7 +182,7 @@ def test_empty_batches_are_accepted_with_null_pointers():
     # every declared entry point that takes a batch is exercised above
     batch_free = {'lx_last_error', 'lx_version', 'lx_prepare_spectra', 'lx_workspace_size', 'lx_carry_bytes',
                   'lx_adjoint_workspace_size', 'lx_autotune', 'lx_tune_cache', 'lx_tune_force', 'lx_tune_report',
-                  'lx_run_scratch_bytes', 'lx_run_scratch_size', 'lx_run_supported', 'lx_run'}
+                  'lx_run_scratch_bytes', 'lx_run_scratch_size', 'lx_run_supported', 'lx_run', 'lx_run_static'}
     assert sorted(set(_declared()) - batch_free - set(calls) - set(run_calls)) == []
     assert L_.lx_step_stats(r, N, 0, N, 0, N, 0, N, 0, N, 0, N, N, N, 0., 10., N, N, N, N) != _lib.LX_OK  # R <= 0
     # the workspace holds the two spectra sets and the row-pair records of lx_step_stats
