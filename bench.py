@@ -88,7 +88,7 @@ def cpu_baseline(sample_worlds=64, steps=384, single_worlds=16, single_steps=192
                               'sample': f'{single_worlds} worlds x {single_steps} steps, workers=1, {dt1:.1f} s'}}
 
 
-def adjoint_rate(n=256, steps=64, ckpt=16, reps=5):
+def adjoint_rate(n=256, steps=64, ckpt=16, reps=10):
     """Gradient, single world n x n (rule above, ring weights b=[1]): loss = MSE(final,
     roll(A0, (8, 8))) after `steps` updates; gradients w.r.t. A0, (m, s), b and dt through
     the checkpointed adjoint.  Returns forward and forward+backward times."""
@@ -125,7 +125,8 @@ def adjoint_rate(n=256, steps=64, ckpt=16, reps=5):
 
     out = {}
     for name, fn in (('fwd_ms', fwd), ('fwd_bwd_ms', fwd_bwd)):
-        fn()
+        for _ in range(3):  # warm-up: first-call allocations and schedule decisions
+            fn()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
@@ -381,6 +382,8 @@ def main():
                     help='time the 3C6K 256^2 step-and-statistics workload only (profiling runs)')
     ap.add_argument('--only-c4', action='store_true',
                     help='time the 16 x 96^3 checkpointed adjoint workload only (profiling runs)')
+    ap.add_argument('--only-c3', action='store_true',
+                    help='time the 256^2 single-world gradient workload only (profiling runs)')
     ap.add_argument('--only-c2-run', action='store_true',
                     help='time the 3C6K search as one early-stopped lx_run only (profiling runs)')
     args = ap.parse_args()
@@ -399,6 +402,9 @@ def main():
         engine.autotune('on', cache=os.environ.get('LENIAX_AMD_TUNE_CACHE') or None)
     if args.only_c4:
         print(json.dumps(c4_adjoint_rate(reps=1)), flush=True)
+        return
+    if args.only_c3:
+        print(json.dumps(adjoint_rate()), flush=True)
         return
     if args.only_c2_run:
         print(json.dumps({'leniax_stop': c2_run(stop='leniax'), 'every_step': c2_run(stop='none')}), flush=True)

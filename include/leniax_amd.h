@@ -240,7 +240,8 @@ int lx_adjoint_pointwise(const lx_rule* rule, int64_t B, const float* state_d, c
  * see lx_adjoint_potential for dL/dK and dL/d(real kernel)), and writes dL/d(initial states) to
  * grad_cells0_d.  Any gradient pointer may be NULL.  The backward reads what the forward left in
  * work_d (the checkpoints and the last segment's tensors): it follows the forward with the same
- * arguments, grad_spec_d NULL or not included. */
+ * arguments, grad_spec_d NULL or not included, and does not read cells_d.  A forward with
+ * forward_only = 1 only steps the worlds (no checkpoints, no segment kept). */
 typedef struct lx_grad_run_args {
   const float* cells0_d;   /* [B][C][dims] initial states */
   const void* K_half_d;    /* lx_prepare_spectra layout */
@@ -263,6 +264,7 @@ typedef struct lx_grad_run_args {
   float* grad_dt_d;
   void* grad_spec_d;       /* [B][K][half] complex64, added into, or NULL */
   void* work_d;            /* lx_grad_run_workspace_size() bytes: checkpoints + one segment */
+  int32_t forward_only;    /* 1: no backward follows; the forward keeps nothing but writes cells_d */
 } lx_grad_run_args;
 int lx_grad_run_workspace_size(const lx_rule* rule, int64_t B, int32_t steps, int32_t ckpt_every, int32_t want_spec,
                                size_t* bytes);

@@ -131,6 +131,7 @@ class lx_grad_run_args(ctypes.Structure):
         ('grad_dt_d', ctypes.c_void_p),
         ('grad_spec_d', ctypes.c_void_p),
         ('work_d', ctypes.c_void_p),
+        ('forward_only', ctypes.c_int32),
     ]
 
 

@@ -4,15 +4,16 @@ leniax differentiates its step with ``jax.grad`` through ``core.update``
 (``leniax/core.py:13-49``), inside ``runner.make_pipeline_fn`` /
 ``make_gradient_fn`` (``leniax/runner.py:363-444``); v1's clip is a
 straight-through estimator (``leniax/core.py:265-270``).  Here the reverse
-mode is hand-written HIP (``lx_adjoint_pointwise`` + ``lx_adjoint_potential``)
-wrapped as ``torch.autograd.Function``s:
+mode is hand-written HIP (``lx_adjoint_step``, or ``lx_adjoint_pointwise`` +
+``lx_adjoint_potential``) wrapped as ``torch.autograd.Function``s:
 
 * ``update``: one differentiable step with the arguments and outputs of
   ``core.update`` (gradients for state, K, gf_params, weights and dt);
 * ``run``: ``steps`` updates as ONE autograd node, checkpointed every
-  ``ckpt_every`` steps: the backward recomputes each segment from its
-  checkpoint (field and potential kept), then walks it in reverse, so memory is
-  ``steps / ckpt_every + ckpt_every`` states instead of ``steps``;
+  ``ckpt_every`` steps (``lx_grad_run_forward`` / ``lx_grad_run_backward``): the
+  backward recomputes each segment but the last from its checkpoint (potential
+  kept), then walks it in reverse, so memory is ``steps / ckpt_every +
+  ckpt_every`` states instead of ``steps``;
 * ``kernel_spectrum`` / ``ring_kernel``: differentiable ``K = fftn(kernel)`` and
   the circle_2d kernel as a function of its ring weights ``b``
   (``leniax/kernels.py:176-212``), so gradients reach ``b``.
@@ -21,7 +22,6 @@ wrapped as ``torch.autograd.Function``s:
 (what central finite differences of the forward measure).  Complex gradients
 use torch's convention (``dL/dRe + i dL/dIm``), the conjugate of JAX's.
 """
-import copy
 import ctypes
 import math
 
@@ -94,10 +94,22 @@ class _Ctx:
         self.w = w.contiguous()
         self.ws = core._per_world_stride(self.w, N, 'weights')
         self.dt, self.dts = core._dt_tensor(dt, N)
-        self.work = None  # a workspace of its own (graph-captured segments), else the shared one
 
     def workspace(self):
-        return self.work if self.work is not None else core._workspace(self.rule, self.N)
+        return core._workspace(self.rule, self.N)
+
+    def grad_run_args(self, A0, steps, ckpt, ste, work, acc):
+        """lx_grad_run_args of a run from A0 (gradient pointers from acc; cells_d, lambda_d and
+        grad_cells0_d left to the caller)."""
+        a = _lib.lx_grad_run_args()
+        a.cells0_d, a.K_half_d, a.k_bstride = A0.data_ptr(), self.Kh.data_ptr(), self.kbs
+        a.gf_d, a.gf_bstride, a.w_d, a.w_bstride = self.gf.data_ptr(), self.gfs, self.w.data_ptr(), self.ws
+        a.dt_d, a.dt_bstride = self.dt.data_ptr(), self.dts
+        a.B, a.steps, a.ckpt_every, a.ste = self.N, steps, ckpt, int(bool(ste))
+        a.grad_gf_d, a.grad_w_d, a.grad_dt_d = (acc[k].data_ptr() if k in acc else None for k in ('gf', 'w', 'dt'))
+        a.grad_spec_d = acc['dspec'].data_ptr() if 'dspec' in acc else None
+        a.work_d = work.data_ptr()
+        return a
 
     def step(self, A, out, field, pot, FA=None):
         """One plain step A -> out; field / potential / the state's spectra FA kept when given."""
@@ -160,7 +172,7 @@ class _Ctx:
             dspec = dspec.sum(dim=0, keepdim=True)
         full = half_to_full(dspec, self.dims, 1.0 / float(np.prod(self.dims)))  # [S, Keff, *dims]
         g = torch.zeros_like(K)
-        g.view((self.S, -1) + self.dims)[:, self.slots] = full
+        g.view((self.S, -1) + self.dims)[:, _slot_index(self.slots, g.device)] = full
         return g
 
     def grad_gf(self, g_pad, gf_orig):
@@ -168,6 +180,21 @@ class _Ctx:
 
     def grad_dt(self, g, dt_orig):
         return _grad_dt(g, dt_orig)
+
+
+_SLOT_INDEX = {}
+
+
+def _slot_index(slots, device):
+    """The kernel slots as an index: a slice when they are consecutive, else a device tensor made
+    once (an index built from a host list each call is a pageable copy, which waits for the stream)."""
+    slots = tuple(int(s) for s in slots)
+    if slots == tuple(range(slots[0], slots[0] + len(slots))):
+        return slice(slots[0], slots[0] + len(slots))
+    key = (device, slots)
+    if key not in _SLOT_INDEX:
+        _SLOT_INDEX[key] = torch.tensor(slots, dtype=torch.int64, device=device)
+    return _SLOT_INDEX[key]
 
 
 def adjoint_workspace(rule, N):
@@ -244,245 +271,54 @@ def update(state, K, gf_params, kernels_weight_per_channel, dt, tc_indices=None,
 
 
 # ------------------------------------------------------------------ checkpointed run
-class _RunGraphs:
-    """HIP graphs of autograd.run's segments for one rule and shape: the forward's c steps,
-    the backward's recompute of a segment and its reverse walk.  Single small worlds are
-    launch-bound (3 launches per forward step, 10 per backward step); a replay issues a
-    segment at once.  The graphs read their own copies of the spectra and parameters
-    (``load``) and work in their own buffers; checkpoints stay with the caller."""
-
-    def __init__(self, sc, c, ste, need_K):
-        self.sc = copy.copy(sc)
-        self.sc.Kh, self.sc.gf, self.sc.w, self.sc.dt = (sc.Kh.clone(), sc.gf.clone(), sc.w.clone(), sc.dt.clone())
-        nbytes = ctypes.c_size_t(0)
-        _lib.call('lx_workspace_size', ctypes.byref(sc.rule), sc.N, ctypes.byref(nbytes))
-        self.sc.work = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device=sc.Kh.device)
-        self.c, self.ste = c, ste
-        shape = (sc.N, sc.C) + sc.dims
-        dev = sc.Kh.device
-        self.seg_A = torch.empty((c,) + shape, device=dev)
-        self.seg_G = torch.empty_like(self.seg_A)
-        self.seg_U = torch.empty((c, sc.N, sc.Keff) + sc.dims, device=dev)
-        self.seg_F = torch.empty((c,) + sc.spectra_shape(), dtype=torch.complex64, device=dev) if need_K else None
-        self.nxt = torch.empty(shape, device=dev)
-        self.fb = [torch.empty(shape, device=dev), torch.empty(shape, device=dev)]
-        self.lam = torch.empty(shape, device=dev)
-        self.dpot = torch.empty_like(self.seg_U[0])
-        self.acc = self.sc.accumulators(need_K)
-        self.graphs = {}
-
-    def load(self, sc):
-        for dst, src in ((self.sc.Kh, sc.Kh), (self.sc.gf, sc.gf), (self.sc.w, sc.w), (self.sc.dt, sc.dt)):
-            dst.copy_(src)
-
-    def keep_token(self):
-        """Marks the segment buffers as holding one forward's last segment; any later use of the
-        buffers (another run's forward or backward) invalidates the mark."""
-        self.owner = object()
-        return self.owner
-
-    def holds(self, token):
-        return token is not None and getattr(self, 'owner', None) is token
-
-    def _graph(self, key, fn):
-        g = self.graphs.get(key)
-        if g is None:
-            side = torch.cuda.Stream()
-            side.wait_stream(torch.cuda.current_stream())
-            with torch.cuda.stream(side):
-                fn()  # warm-up outside the capture
-            torch.cuda.current_stream().wait_stream(side)
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                fn()
-            self.graphs[key] = g
-        return g
-
-    def keeps_field(self):
-        return not (self.sc.C == 1 and self.sc.Keff == 1)  # one kernel: the adjoint recomputes the field
-
-    def forward_segment(self, L):
-        """Replays L steps from seg_A[0]; returns the static buffer holding the result."""
-        def fn():
-            cur = self.seg_A[0]
-            for i in range(L):
-                self.sc.step(cur, self.fb[i % 2], None, None)
-                cur = self.fb[i % 2]
-        self._graph(('fwd', L), fn).replay()
-        return self.fb[(L - 1) % 2]
-
-    def backward_segment(self, L):
-        keep_G = self.keeps_field()
-
-        def rec():
-            for i in range(L):
-                out = self.seg_A[i + 1] if i + 1 < L else self.nxt
-                self.sc.step(self.seg_A[i], out, self.seg_G[i] if keep_G else None, self.seg_U[i],
-                             None if self.seg_F is None else self.seg_F[i])
-
-        def adj():
-            for i in range(L - 1, -1, -1):
-                self.sc.adjoint(self.seg_A[i], self.seg_G[i] if keep_G else None, self.seg_U[i], self.lam, self.acc,
-                                self.ste, FA=None if self.seg_F is None else self.seg_F[i], lam_out=self.lam,
-                                dpot=self.dpot)
-        g_rec = self._graph(('rec', L), rec)
-        g_adj = self._graph(('adj', L), adj)
-        return g_rec, g_adj
-
-
-def _segment_buffers(sc, c, need_K):
-    """A segment's tensors for the reverse pass: input states (and the segment's output), potentials,
-    fields (several channels or kernels only) and the states' spectra (kernel gradient only)."""
-    dev = sc.Kh.device
-    shape = (sc.N, sc.C) + sc.dims
-    keep_G = not (sc.C == 1 and sc.Keff == 1)
-    return {'A': torch.empty((c + 1,) + shape, device=dev),
-            'G': torch.empty((c,) + shape, device=dev) if keep_G else None,
-            'U': torch.empty((c, sc.N, sc.Keff) + sc.dims, device=dev),
-            'F': torch.empty((c,) + sc.spectra_shape(), dtype=torch.complex64, device=dev) if need_K else None}
-
-
-_GRAPHS = {}
-GRAPH_MAX_CELLS = 1 << 22  # graphs pay off where launches, not HBM, bound a step
-
-
-def _graphs_for(sc, c, ste, need_K):
-    r = sc.rule
-    key = (torch.cuda.current_device(), sc.N, sc.C, sc.Keff, sc.dims, tuple(r.cin[:sc.Keff]), tuple(r.gf[:sc.Keff]),
-           r.state_fn, r.average, tuple(sc.Kh.shape), sc.kbs, tuple(sc.gf.shape), sc.gfs, tuple(sc.w.shape), sc.ws,
-           sc.dt.numel(), sc.dts, c, bool(ste), bool(need_K))
-    g = _GRAPHS.get(key)
-    if g is None:
-        g = _GRAPHS[key] = _RunGraphs(sc, c, ste, need_K)
-    return g
-
-
 class _Run(torch.autograd.Function):
+    """``steps`` updates as one node through the C ABI's run-level gradient: lx_grad_run_forward
+    keeps a checkpoint every ``ckpt`` steps and the last segment's tensors in a workspace of the
+    node's own, lx_grad_run_backward recomputes the earlier segments and walks every step in
+    reverse.  The step loops run in the library: one host call per pass."""
 
     @staticmethod
     def forward(ctx, state0, K, gf, w, dt, meta):
-        tc_indices, growth_fn_t, weighted_fn, slug, ste, steps, ckpt, graphs = meta
+        tc_indices, growth_fn_t, weighted_fn, slug, ste, steps, ckpt = meta
         A, K_, gf_, w_ = _as_tensors(state0, K, gf, w)
+        A = A.contiguous()
         sc = _Ctx(A, K_, gf_, w_, dt, tc_indices, growth_fn_t, weighted_fn, slug)
-        if graphs is None:
-            graphs = sc.N * sc.C * int(np.prod(sc.dims)) <= GRAPH_MAX_CELLS
-        ckpts = [A]
-        # the last segment's steps keep what the reverse pass reads (potentials, fields, spectra), so
-        # that the backward recomputes only the segments before it
-        nseg = (steps + ckpt - 1) // ckpt
-        need_K = bool(ctx.needs_input_grad[1])
         keep = any(ctx.needs_input_grad[:5])  # a forward whose backward will run
-        kept = None
-        if graphs and steps > 0:
-            G = _graphs_for(sc, ckpt, ste, True)
-            G.load(sc)
-            if keep:  # every segment graph the backward replays, captured now (their warm-ups use the buffers)
-                for L in sorted({min(ckpt, steps - s * ckpt) for s in range(nseg)}):
-                    G.backward_segment(L)
-            G.owner = None
-            cur = A
-            for s in range(nseg):
-                if s > 0:
-                    ckpts.append(cur.clone())
-                G.seg_A[0].copy_(cur)
-                L = min(ckpt, steps - s * ckpt)
-                if keep and s == nseg - 1:
-                    g_rec, _ = G.backward_segment(L)
-                    g_rec.replay()
-                    cur = G.nxt
-                    kept = G.keep_token()
-                else:
-                    cur = G.forward_segment(L)
-            out = cur.clone()
-        else:
-            bufs = [torch.empty_like(A), torch.empty_like(A)]
-            cur = A
-            last = (nseg - 1) * ckpt if steps > 0 else 0
-            for t in range(last):
-                if t > 0 and t % ckpt == 0:
-                    ckpts.append(cur.clone())
-                nxt = bufs[t % 2]
-                sc.step(cur, nxt, None, None)
-                cur = nxt
-            if steps > 0 and not keep:
-                for t in range(last, steps):
-                    nxt = bufs[t % 2]
-                    sc.step(cur, nxt, None, None)
-                    cur = nxt
-                out = cur.clone()
-            elif steps > 0:
-                if last > 0:
-                    ckpts.append(cur.clone())
-                kept = _segment_buffers(sc, min(ckpt, steps), need_K)
-                kept['A'][0].copy_(cur)
-                L = steps - last
-                out = torch.empty_like(A)
-                for i in range(L):
-                    dst = kept['A'][i + 1] if i + 1 < L else out
-                    sc.step(kept['A'][i], dst, None if kept['G'] is None else kept['G'][i], kept['U'][i],
-                            None if kept['F'] is None else kept['F'][i])
-            else:
-                out = A.clone()
-        ctx.kept = kept
-        ctx.sc, ctx.ste, ctx.steps, ctx.ckpt, ctx.ckpts, ctx.dt_orig = sc, ste, steps, ckpt, ckpts, dt
-        ctx.graphs = graphs
+        ctx.sc, ctx.ste, ctx.steps, ctx.ckpt, ctx.dt_orig = sc, ste, steps, ckpt, dt
+        ctx.work = ctx.A0 = ctx.acc = None
         ctx.save_for_backward(K_, gf_)
+        if steps == 0:
+            return A.clone()
+        out = torch.empty_like(A)
+        acc = sc.accumulators(bool(ctx.needs_input_grad[1])) if keep else {}
+        nb = ctypes.c_size_t(0)
+        _lib.call('lx_grad_run_workspace_size', ctypes.byref(sc.rule), sc.N, steps, ckpt, int('dspec' in acc),
+                  ctypes.byref(nb))
+        work = torch.empty(max(int(nb.value), 256), dtype=torch.uint8, device=A.device)
+        a = sc.grad_run_args(A, steps, ckpt, ste, work, acc)
+        a.cells_d = out.data_ptr()
+        a.forward_only = 0 if keep else 1
+        _lib.call('lx_grad_run_forward', ctypes.byref(sc.rule), ctypes.byref(a), engine.stream_ptr())
+        if keep:
+            ctx.work, ctx.A0, ctx.acc = work, A, acc
         return out
 
     @staticmethod
     def backward(ctx, g_out):
         K, gf = ctx.saved_tensors
-        sc, steps, c = ctx.sc, ctx.steps, ctx.ckpt
+        sc, steps = ctx.sc, ctx.steps
         need = ctx.needs_input_grad
-        if ctx.graphs and steps > 0:
-            G = _graphs_for(sc, c, ctx.ste, True)
-            G.load(sc)
-            kept_valid = G.holds(ctx.kept)
-            lengths = sorted({min(c, steps - s * c) for s in range((steps - 1) // c + 1)})
-            n_graphs = len(G.graphs)
-            segs = {L: G.backward_segment(L) for L in lengths}  # captures (and their warm-ups) first
-            kept_valid = kept_valid and len(G.graphs) == n_graphs  # a warm-up overwrote the buffers
-            G.owner = None
-            G.lam.copy_(g_out)
-            for v in G.acc.values():
-                v.zero_()
-            last = (steps - 1) // c
-            for s in range(last, -1, -1):
-                g_rec, g_adj = segs[min(c, steps - s * c)]
-                if not (s == last and kept_valid):  # the forward left the last segment in place
-                    G.seg_A[0].copy_(ctx.ckpts[s])
-                    g_rec.replay()
-                g_adj.replay()
-            lam = G.lam.clone()
-            acc = {k: v.clone() for k, v in G.acc.items()}
-        else:
+        if steps == 0:
             acc = sc.accumulators(need[1])
-            lam = g_out.contiguous().clone()  # updated in place, step after step
-            if steps > 0:
-                A0 = ctx.ckpts[0]
-                seg = ctx.kept  # the last segment, kept by the forward; earlier ones are recomputed into it
-                if seg is None:
-                    seg = _segment_buffers(sc, c, 'dspec' in acc)
-                    seg['stale'] = True
-                if 'dspec' in acc and seg['F'] is None:  # the kernel gradient was not expected: spectra now
-                    seg['F'] = torch.empty((c,) + sc.spectra_shape(), dtype=torch.complex64, device=A0.device)
-                    seg['stale'] = True
-                dpot = torch.empty_like(seg['U'][0])
-                nxt = torch.empty_like(A0)
-                last = (steps - 1) // c
-                for s in range(last, -1, -1):
-                    t0, t1 = s * c, min(s * c + c, steps)
-                    if s < last or seg.get('stale'):
-                        seg['A'][0].copy_(ctx.ckpts[s])
-                        for i in range(t1 - t0):  # recompute the segment, keeping potential (field, spectra)
-                            out = seg['A'][i + 1] if i + 1 < t1 - t0 else nxt
-                            sc.step(seg['A'][i], out, None if seg['G'] is None else seg['G'][i], seg['U'][i],
-                                    None if seg['F'] is None else seg['F'][i])
-                        seg['stale'] = False
-                    for i in range(t1 - t0 - 1, -1, -1):
-                        sc.adjoint(seg['A'][i], None if seg['G'] is None else seg['G'][i], seg['U'][i], lam, acc,
-                                   ctx.ste, FA=None if seg['F'] is None else seg['F'][i], lam_out=lam, dpot=dpot)
-        ctx.ckpts = ctx.kept = None
+            lam = g_out.contiguous().clone()
+        else:
+            acc = ctx.acc
+            lam_in = g_out.contiguous()
+            lam = torch.empty_like(lam_in)
+            a = sc.grad_run_args(ctx.A0, steps, ctx.ckpt, ctx.ste, ctx.work, acc)
+            a.lambda_d, a.grad_cells0_d = lam_in.data_ptr(), lam.data_ptr()
+            _lib.call('lx_grad_run_backward', ctypes.byref(sc.rule), ctypes.byref(a), engine.stream_ptr())
+        ctx.work = ctx.A0 = ctx.acc = None
         return (lam if need[0] else None,
                 sc.grad_K(acc['dspec'], K) if need[1] and 'dspec' in acc else None,
                 sc.grad_gf(acc['gf'], gf) if need[2] else None,
@@ -492,17 +328,16 @@ class _Run(torch.autograd.Function):
 
 
 def run(state0, K, gf_params, kernels_weight_per_channel, dt, steps, tc_indices=None, growth_fn_t=('gaussian',),
-        weighted_fn=None, state_slug='v1', ste=True, ckpt_every=16, graphs=None):
-    """Final state after ``steps`` updates, differentiable with checkpointed recompute.
-    ``graphs``: replay the segments as HIP graphs (default: for batches of at most
-    GRAPH_MAX_CELLS cells, where launches bound a step)."""
+        weighted_fn=None, state_slug='v1', ste=True, ckpt_every=16):
+    """Final state after ``steps`` updates, differentiable with checkpointed recompute (a checkpoint
+    every ``ckpt_every`` steps; the backward recomputes every segment but the last)."""
     if weighted_fn is None:
         weighted_fn = core.weighted_mean
     if ckpt_every < 1:
         raise ValueError('ckpt_every must be >= 1')
     return _Run.apply(state0, K, gf_params, kernels_weight_per_channel, dt,
                       (tc_indices, tuple(growth_fn_t), weighted_fn, state_slug, ste, int(steps),
-                       int(min(ckpt_every, max(steps, 1))), graphs))
+                       int(min(ckpt_every, max(steps, 1)))))
 
 
 # ------------------------------------------------------------------ kernels

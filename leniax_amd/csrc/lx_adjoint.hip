@@ -5,7 +5,7 @@
 // lx_grad_run_forward steps the worlds from their initial states, keeping every ckpt_every-th
 // state in the workspace, and writes the final states; the last segment's steps also keep their
 // input states, potentials (fields, spectra) in the segment buffers, which the reverse pass reads
-// first.  lx_grad_run_backward takes dL/d(final state) and walks the segments from the last: it
+// first.  With forward_only it keeps nothing (no reverse pass follows).  lx_grad_run_backward takes dL/d(final state) and walks the segments from the last: it
 // recomputes each earlier segment from its checkpoint, keeping each step's input state, potential,
 // field (with several channels or kernels; one channel and one kernel recompute it from the
 // potential) and the state's spectra (when the kernel gradient is asked for), then runs the step's
@@ -60,12 +60,13 @@ int grad_layout(const lx_rule* r, int64_t B, int32_t steps, int32_t c, int32_t w
   return LX_OK;
 }
 
-int check_args(const lx_rule* rule, const lx_grad_run_args* a) {
+int check_args(const lx_rule* rule, const lx_grad_run_args* a, bool forward) {
   int rc = lx_check_rule(rule);
   if (rc) return rc;
   if (!a) return lx_fail(LX_EINVAL, "args is NULL");
   if (a->B < 0 || a->steps < 1 || a->ckpt_every < 1) return lx_fail(LX_EINVAL, "B >= 0, steps and ckpt_every >= 1");
-  if (a->B > 0 && (!a->cells0_d || !a->K_half_d || !a->gf_d || !a->w_d || !a->dt_d || !a->cells_d || !a->work_d))
+  if (a->B > 0 && (!a->cells0_d || !a->K_half_d || !a->gf_d || !a->w_d || !a->dt_d || (forward && !a->cells_d) ||
+                   !a->work_d))
     return lx_fail(LX_EINVAL, "NULL device pointer in lx_grad_run_args");
   return LX_OK;
 }
@@ -108,7 +109,7 @@ extern "C" int lx_grad_run_workspace_size(const lx_rule* rule, int64_t B, int32_
 }
 
 extern "C" int lx_grad_run_forward(const lx_rule* rule, const lx_grad_run_args* a, void* stream) {
-  int rc = check_args(rule, a);
+  int rc = check_args(rule, a, true);
   if (rc || a->B == 0) return rc;
   GradLayout l;
   if ((rc = grad_layout(rule, a->B, a->steps, a->ckpt_every, a->grad_spec_d != nullptr, l))) return rc;
@@ -116,17 +117,22 @@ extern "C" int lx_grad_run_forward(const lx_rule* rule, const lx_grad_run_args* 
   float* ck = fptr(a->work_d, l.ckpt);
   float* sA = fptr(a->work_d, l.seg_A);
   void* work = fptr(a->work_d, l.step_work);
-  if ((rc = copy(ck, a->cells0_d, l.state_elems, st))) return rc;
-  const int c = a->ckpt_every, last = (l.nckpt - 1) * c;
-  const float* cur = ck;
-  for (int t = 0; t < last; ++t) {  // the segments before the last: checkpoints only
-    float* next = (t + 1) % c == 0 ? ck + int64_t((t + 1) / c) * l.state_elems
-                                   : sA + int64_t(t & 1) * l.state_elems;
+  const bool only = a->forward_only != 0;
+  if (!only && (rc = copy(ck, a->cells0_d, l.state_elems, st))) return rc;
+  // forward only: every step here, the last into cells_d; else the segments before the last, their
+  // states at multiples of ckpt_every going to the checkpoints
+  const int c = a->ckpt_every, last = only ? a->steps : (l.nckpt - 1) * c;
+  const float* cur = only ? a->cells0_d : ck;
+  for (int t = 0; t < last; ++t) {
+    float* next = t + 1 == a->steps ? a->cells_d
+                  : (t + 1) % c == 0 ? ck + int64_t((t + 1) / c) * l.state_elems
+                                     : sA + int64_t(t & 1) * l.state_elems;
     if ((rc = lx_update(rule, cur, a->B, a->K_half_d, a->k_bstride, a->gf_d, a->gf_bstride, a->w_d, a->w_bstride,
                         a->dt_d, a->dt_bstride, next, nullptr, nullptr, work, stream)))
       return rc;
     cur = next;
   }
+  if (only) return LX_OK;
   // the last segment, its tensors kept for the reverse pass
   if ((rc = copy(sA, cur, l.state_elems, st))) return rc;
   const int L = a->steps - last;
@@ -138,7 +144,7 @@ extern "C" int lx_grad_run_forward(const lx_rule* rule, const lx_grad_run_args* 
 }
 
 extern "C" int lx_grad_run_backward(const lx_rule* rule, const lx_grad_run_args* a, void* stream) {
-  int rc = check_args(rule, a);
+  int rc = check_args(rule, a, false);
   if (rc || a->B == 0) return rc;
   if (!a->lambda_d) return lx_fail(LX_EINVAL, "lx_grad_run_backward: lambda_d is NULL");
   GradLayout l;
@@ -157,6 +163,15 @@ extern "C" int lx_grad_run_backward(const lx_rule* rule, const lx_grad_run_args*
   if ((rc = copy(lam, a->lambda_d, l.state_elems, st))) return rc;
   const int c = a->ckpt_every;
   const bool keepG = !(rule->C == 1 && rule->K == 1);
+  // one channel and one kernel: the steps' reverses chained through fused row passes (each step's
+  // last row pass shares a kernel with the first of the step before it, inside a segment), the
+  // gradient scalars accumulated per block over the steps and summed once at the end
+  const bool chain = !keepG && a->B <= 65535;
+  LxAdj11 ac;
+  if (chain && (rc = lx_adj11_init(rule, a->B, a->gf_d, a->gf_bstride, a->w_d, a->w_bstride, a->dt_d,
+                                   a->dt_bstride, a->K_half_d, a->k_bstride, a->ste, lam, a->grad_gf_d,
+                                   a->grad_w_d, a->grad_dt_d, work, adj_work, stream, &ac)))
+    return rc;
   for (int seg = l.nckpt - 1; seg >= 0; --seg) {
     const int t0 = seg * c, L = std::min(c, a->steps - t0);
     // recompute the segment (the last one the forward kept): input state, potential (field, spectra)
@@ -167,6 +182,23 @@ extern "C" int lx_grad_run_backward(const lx_rule* rule, const lx_grad_run_args*
         if ((rc = segment_step(rule, a, l, i, sA + int64_t(i) * l.state_elems, sA + int64_t(i + 1) * l.state_elems,
                                work, stream)))
           return rc;
+    }
+    if (chain) {
+      for (int i = L - 1; i >= 0; --i) {
+        const float* Ui = sU + int64_t(i) * l.pot_elems;
+        if (i == L - 1 &&
+            (rc = lx_adj11_rows(ac, Ui, sA + int64_t(i) * l.state_elems, false, seg < l.nckpt - 1)))
+          return rc;
+        if ((rc = lx_adj11_lines(ac, spec ? sF + int64_t(i) * l.spec_elems : nullptr, a->grad_spec_d))) return rc;
+        if (i > 0) {
+          if ((rc = lx_adj11_rows(ac, sU + int64_t(i - 1) * l.pot_elems, sA + int64_t(i - 1) * l.state_elems, true,
+                                  true)))
+            return rc;
+        } else if ((rc = lx_adj11_last(ac, seg == 0))) {
+          return rc;
+        }
+      }
+      continue;
     }
     // reverse, last step first: lambda updated in place
     for (int i = L - 1; i >= 0; --i) {

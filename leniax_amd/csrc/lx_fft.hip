@@ -566,6 +566,9 @@ struct RowArgs {
   RunView rv;
   float* cells_hist;
   int64_t hist_B;
+  // the reverse step's gradient scalars, summed by fin.blocks workgroups after the row pairs' (not in
+  // the statistics or LIST kernels)
+  LxAdjFinish fin;
 };
 
 // UNI: every wave's row pairs lie in one image (the host checks that the pairs per image are a
@@ -581,6 +584,13 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
   const int lane = threadIdx.x & 63;
   const int t = threadIdx.x % TPR;
   float2* xch = xch_all + (threadIdx.x / TPR) * (TPR * LC::XRS);
+  if constexpr (!STATS && !LIST) {
+    if (a.fin.blocks > 0 && int(blockIdx.x) >= int(gridDim.x) - a.fin.blocks) {
+      const int64_t i = int64_t(int(blockIdx.x) - (int(gridDim.x) - a.fin.blocks)) * 4 + (threadIdx.x >> 6);
+      if (i < lx_adj_finish_outputs(a.fin)) lx_adj_finish_wave(a.fin, i);
+      return;
+    }
+  }
   const int64_t pr = (int64_t(blockIdx.x) * 256 + threadIdx.x) / TPR;
   int64_t npairs = a.npairs;
   if constexpr (LIST) {
@@ -981,8 +991,9 @@ int launch_rows_inv(const Geom& g, RowArgs ra, int64_t nimg, hipStream_t st) {
   return dispatch_len(g.W, [&](auto w) {
     constexpr int W = decltype(w)::value;
     if (int rc = ensure_tw<W>(st)) return rc;
-    const dim3 grid(blocks_for(ra.npairs * Lanes<W>::TPR));
+    const dim3 grid(blocks_for(ra.npairs * Lanes<W>::TPR) + unsigned(ra.fin.blocks));
     const bool uni = (ra.rows / 2) % (64 / Lanes<W>::TPR) == 0;  // waves never straddle images
+    if (ra.part && ra.fin.blocks) return lx_fail(LX_EINVAL, "rows_inv: statistics and gradient sums together");
     if (ra.part) {
       if (uni) hipLaunchKernelGGL((rows_inv_kernel<W, true, true>), grid, dim3(256), 0, st, ra);
       else hipLaunchKernelGGL((rows_inv_kernel<W, true, false>), grid, dim3(256), 0, st, ra);
@@ -1007,6 +1018,7 @@ RowArgs row_args(const lx_rule* rule) {
   ra.part = nullptr; ra.carry = nullptr;
   ra.rv = RunView{nullptr, nullptr, nullptr, nullptr, nullptr, 0};
   ra.cells_hist = nullptr; ra.hist_B = 0;
+  ra.fin = LxAdjFinish{};
   ra.ndim = rule->ndim;
   ra.d0 = rule->ndim == 3 ? rule->dims[0] : 1;
   ra.d1 = rule->dims[rule->ndim - 2];
@@ -1752,11 +1764,20 @@ struct AdjRowArgs {
   int64_t dt_bs;
   int64_t rows;    // rows per world
   int state_fn, average, ste;
+  int accumulate;  // 1: add the block sums into part (a run's steps), 0: overwrite
+  // BRIDGE: the later step's adjoint potential, half spectra transformed along every axis but W
+  // (may alias S); its inverse row transform is added to lam before the pointwise reverse
+  const float2* Sin;
+  float inv_n;     // 1 / cells per world
 };
 
 // grid (blocks per world, B): a block's row pairs lie in one world (pairs past the world's last are
-// idle lanes: they load pair 0, contribute nothing and store nothing)
-template <int W, int KIND>
+// idle lanes: they load pair 0, contribute nothing and store nothing).
+// BRIDGE: the last row pass of one reverse step (lambda += inverse transform of its adjoint
+// potential, as rows_inv_kernel in selection mode) and the first of the step before it in one
+// kernel: lambda of the later step never leaves the registers between the two.  Every row pair is
+// read and written by its own lane group only, so Sin may alias S.
+template <int W, int KIND, bool BRIDGE = false>
 __global__ __launch_bounds__(256) void rows_fwd_adj_kernel(AdjRowArgs a) {
   using LC = Lanes<W>;
   constexpr int TPR = LC::TPR, P = LC::P, WH = W / 2 + 1;
@@ -1779,6 +1800,41 @@ __global__ __launch_bounds__(256) void rows_fwd_adj_kernel(AdjRowArgs a) {
   const int mode = a.state_fn == LX_STATE_V2 ? 2 : (a.state_fn == LX_STATE_V1 && !a.ste ? 1 : 0);
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
   float2 x[P];
+  if constexpr (BRIDGE) {
+    // the later step's adjoint potential along W: stage the pair's half spectra, repack them as one
+    // complex row Z[k] = A[k] + i B[k] (A[k] = conj(A[W-k]) above W/2), inverse transform
+    const float2* u0 = a.Sin + (b * a.rows + row0) * int64_t(WH);
+    const float2* u1 = u0 + WH;
+#pragma unroll
+    for (int i = 0; i < (WH + TPR - 1) / TPR; ++i) {
+      const int col = t + TPR * i;
+      if (col < WH) {
+        xch[col] = u0[col];
+        xch[WH + col] = u1[col];
+      }
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int e = 0; e < P; ++e) {
+      const int kk = freq_of<W>(t, e);
+      const bool mirror = kk >= W / 2;
+      const float sg = mirror ? -1.f : 1.f;
+      const int col = mirror ? W - kk : kk;
+      const float2 ya = xch[col], yb = xch[WH + col];
+      float2 z = make_float2(fmaf(-sg, yb.y, ya.x), fmaf(sg, ya.y, yb.x));
+      if (kk == 0 || kk == W / 2) z = make_float2(ya.x, yb.x);
+      x[e] = z;
+    }
+    wave_lds_fence();
+    fft_group<W, true>(x, t, lane, xch);
+    // parked in the group's exchange rows (point TPR m + t of both rows at TPR m + t) until the
+    // pointwise loop adds them to lambda: the registers are free for that loop
+    wave_lds_fence();
+#pragma unroll
+    for (int m = 0; m < P; ++m) xch[TPR * m + t] = x[m];
+    wave_lds_fence();
+    __builtin_amdgcn_sched_barrier(0);
+  }
   // one base pointer per array: every point is a constant offset from it
   const float* pU = a.U + off + t;
   float* pL = a.lam + off + t;
@@ -1795,7 +1851,15 @@ __global__ __launch_bounds__(256) void rows_fwd_adj_kernel(AdjRowArgs a) {
 #pragma unroll
       for (int r = 0; r < 2; ++r) {
         const int o = r * W + TPR * m;
-        const float U = pU[o], L = pL[o];
+        const float U = pU[o];
+        float L = pL[o];
+        if constexpr (BRIDGE) {
+          // dL/d(state in) of the later step: its direct part plus the correlation term, as
+          // rows_inv_kernel's selection mode adds them
+          const float2 inc = xch[TPR * m + t];
+          L = add_nc(L, mul_nc(r == 0 ? inc.x : inc.y, a.inv_n));
+          if (MODE == 0 && live) pL[o] = L;
+        }
         const float A = MODE != 0 ? pA[o] : 0.f;
         float dgdu, dgdm, dgds;
         const float g = growth_rcp(KIND, gm, gs, inv_s, inv_9s2, U);
@@ -1823,6 +1887,9 @@ __global__ __launch_bounds__(256) void rows_fwd_adj_kernel(AdjRowArgs a) {
           acc[3] += dz * G;
         }
       }
+      // the point's arithmetic is done here, next to its loads: left free, the compiler sinks it
+      // to the transform below and keeps every point's loaded values live until then
+      __asm__ volatile("" : "+v"(du[0]), "+v"(du[1]), "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]));
       x[m] = make_float2(du[0], du[1]);
       if ((m & 1) == 1) __builtin_amdgcn_sched_barrier(0);
     }
@@ -1851,10 +1918,114 @@ __global__ __launch_bounds__(256) void rows_fwd_adj_kernel(AdjRowArgs a) {
       s1[k] = make_float2(0.5f * (z.y + zn.y), 0.5f * (zn.x - z.x));
     }
   }
-  lx_block_partial<4>(acc, a.part);
+  lx_block_partial<4>(acc, a.part, a.accumulate != 0);
 }
 
 }  // namespace
+
+int lx_adj11_init(const lx_rule* rule, int64_t B, const float* gf_d, int64_t gf_bstride, const float* w_d,
+                  int64_t w_bstride, const float* dt_d, int64_t dt_bstride, const void* K_half_d, int64_t k_bstride,
+                  int32_t ste, float* lambda_d, float* grad_gf_d, float* grad_w_d, float* grad_dt_d, void* work_d,
+                  void* adj_work_d, void* stream, LxAdj11* c) {
+  Geom g;
+  if (int rc = make_geom(rule->ndim, rule->dims, g)) return rc;
+  if (!(rule->C == 1 && rule->K == 1) || B < 1 || B > 65535)
+    return lx_fail(LX_EINVAL, "fused reverse step: one channel, one kernel, 1 to 65535 worlds");
+  c->rule = rule; c->B = B; c->stream = stream;
+  c->gf = gf_d; c->gf_bs = gf_bstride; c->w = w_d; c->w_bs = w_bstride; c->dt = dt_d; c->dt_bs = dt_bstride;
+  c->K_half = K_half_d; c->k_bs = k_bstride; c->ste = ste;
+  c->lam = lambda_d; c->g_gf = grad_gf_d; c->g_w = grad_w_d; c->g_dt = grad_dt_d;
+  c->X = work_d;
+  c->part = lx_adj_work_11(rule, B, adj_work_d);
+  const int64_t pairs_per_block = 256 / lx_row_tpr(g.W);
+  c->bpw = int((g.rows / 2 + pairs_per_block - 1) / pairs_per_block);
+  return LX_OK;
+}
+
+// rows_fwd_adj_kernel over the step whose potential and input state are U and A (bridge: after
+// the inverse row transform of the later step's adjoint potential in X)
+int lx_adj11_rows(const LxAdj11& c, const float* U, const float* A, bool bridge, bool accumulate) {
+  Geom g;
+  if (int rc = make_geom(c.rule->ndim, c.rule->dims, g)) return rc;
+  hipStream_t st = (hipStream_t)c.stream;
+  float2* X = static_cast<float2*>(c.X);
+  AdjRowArgs ar;
+  ar.U = U; ar.lam = c.lam; ar.A = A; ar.S = X; ar.part = c.part;
+  ar.gf = c.gf; ar.gf_bs = c.gf_bs; ar.w = c.w; ar.w_bs = c.w_bs; ar.dt = c.dt; ar.dt_bs = c.dt_bs;
+  ar.rows = g.rows; ar.state_fn = c.rule->state_fn; ar.average = c.rule->average; ar.ste = c.ste != 0;
+  ar.accumulate = accumulate ? 1 : 0;
+  ar.Sin = X;
+  ar.inv_n = 1.f / float(g.rows * g.W);
+  return dispatch_len(g.W, [&](auto wc) {
+    constexpr int W = decltype(wc)::value;
+    if (int r2 = ensure_tw<W>(st)) return r2;
+    const dim3 grid(unsigned(c.bpw), unsigned(c.B));
+    auto go = [&](auto kind_c) {
+      constexpr int KIND = decltype(kind_c)::value;
+      if (bridge) hipLaunchKernelGGL((rows_fwd_adj_kernel<W, KIND, true>), grid, dim3(256), 0, st, ar);
+      else hipLaunchKernelGGL((rows_fwd_adj_kernel<W, KIND, false>), grid, dim3(256), 0, st, ar);
+    };
+    switch (c.rule->gf[0]) {
+      case LX_GF_POLY_QUAD4: go(IntC<LX_GF_POLY_QUAD4>{}); break;
+      case LX_GF_GAUSSIAN: go(IntC<LX_GF_GAUSSIAN>{}); break;
+      case LX_GF_GAUSSIAN_TARGET: go(IntC<LX_GF_GAUSSIAN_TARGET>{}); break;
+      case LX_GF_STEP: go(IntC<LX_GF_STEP>{}); break;
+      case LX_GF_STAIRCASE: go(IntC<LX_GF_STAIRCASE>{}); break;
+      case LX_GF_TRIANGLE: go(IntC<LX_GF_TRIANGLE>{}); break;
+      default: go(IntC<LX_GF_IDENTITY>{}); break;
+    }
+    return launched(bridge ? "rows (adjoint bridge)" : "rows_fwd (adjoint)");
+  });
+}
+
+// the adjoint potential's passes between the row passes: lines, conj(K) (and the kernel gradient), lines
+int lx_adj11_lines(const LxAdj11& c, const void* state_spec_d, void* dspec_d) {
+  Geom g;
+  int rc = make_geom(c.rule->ndim, c.rule->dims, g);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)c.stream;
+  float2* X = static_cast<float2*>(c.X);
+  if (g.naxes == 2) {
+    LineArgs la = line_args();
+    la.S = X; la.mode = 1;
+    if ((rc = launch_lines(g, 1, la, c.B, st))) return rc;
+  }
+  {
+    LineArgs la = line_args();
+    la.S = X; la.U = X;
+    la.Kh = static_cast<const float2*>(c.K_half); la.k_bstride = c.k_bs; la.C = 1; la.K = 1; la.conjk = 1;
+    la.cin[0] = 0; la.acc_cin[0] = 0;
+    la.FAin = static_cast<const float2*>(state_spec_d);
+    la.dspec = static_cast<float2*>(dspec_d);
+    la.C_fa = 1;
+    if ((rc = launch_lines(g, 0, la, c.B, st))) return rc;
+  }
+  if (g.naxes == 2) {
+    LineArgs la = line_args();
+    la.S = X; la.U = X; la.mode = 2;
+    if ((rc = launch_lines(g, 1, la, c.B, st))) return rc;
+  }
+  return LX_OK;
+}
+
+// the last row pass of the earliest step: lambda += the inverse row transform of X; finish: the
+// gradient scalars summed from the partial sums (by extra workgroups of the same launch)
+int lx_adj11_last(const LxAdj11& c, bool finish) {
+  Geom g;
+  if (int rc = make_geom(c.rule->ndim, c.rule->dims, g)) return rc;
+  RowArgs ra = row_args(c.rule);
+  ra.U = static_cast<float2*>(c.X); ra.A = c.lam; ra.A_out = c.lam;
+  ra.update = 1; ra.sel = 1; ra.average = 0; ra.state_fn = LX_STATE_SIMPLE;
+  if (finish) {
+    LxAdjFinish& f = ra.fin;
+    f.part = c.part; f.B = c.B; f.per = c.bpw;
+    f.g_gf = c.g_gf; f.gf_bs = c.gf_bs; f.g_w = c.g_w; f.w_bs = c.w_bs; f.g_dt = c.g_dt; f.dt_bs = c.dt_bs;
+    const int64_t nout = (c.g_gf ? (c.gf_bs ? c.B : 1) * 2 : 0) + (c.g_w ? (c.w_bs ? c.B : 1) : 0) +
+                         (c.g_dt ? (c.dt_bs ? c.B : 1) : 0);
+    f.blocks = int((nout + 3) / 4);
+  }
+  return launch_rows_inv(g, ra, c.B, (hipStream_t)c.stream);
+}
 
 extern "C" int lx_adjoint_step(const lx_rule* rule, int64_t B, const float* state_d, const float* field_d,
                                const float* potential_d, const float* gf_d, int64_t gf_bstride, const float* w_d,
@@ -1874,9 +2045,6 @@ extern "C" int lx_adjoint_step(const lx_rule* rule, int64_t B, const float* stat
   if (!state_d || !potential_d || !gf_d || !w_d || !dt_d || !K_half_d || !lambda_d || !work_d || !adj_work_d ||
       (!one && (!field_d || !dpot_d)))
     return lx_fail(LX_EINVAL, "NULL pointer");
-  hipStream_t st = (hipStream_t)stream;
-  const int TPRW = lx_row_tpr(g.W);
-  const int64_t pairs_per_block = 256 / TPRW;
   const bool fused = one && B <= 65535;
   if (!fused) {
     if (!dpot_d) return lx_fail(LX_EINVAL, "lx_adjoint_step: this rule needs dpot_d");
@@ -1887,58 +2055,11 @@ extern "C" int lx_adjoint_step(const lx_rule* rule, int64_t B, const float* stat
     return lx_adjoint_potential(rule, B, K_half_d, k_bstride, dpot_d, state_spec_d, dspec_d, lambda_d, work_d,
                                 stream);
   }
-  float *part = nullptr, *ysum = nullptr;
-  lx_adj_work_11(rule, B, adj_work_d, &part, &ysum);
-  float2* X = static_cast<float2*>(work_d);
-  AdjRowArgs ar;
-  ar.U = potential_d; ar.lam = lambda_d; ar.A = state_d; ar.S = X; ar.part = part;
-  ar.gf = gf_d; ar.gf_bs = gf_bstride; ar.w = w_d; ar.w_bs = w_bstride; ar.dt = dt_d; ar.dt_bs = dt_bstride;
-  ar.rows = g.rows; ar.state_fn = rule->state_fn; ar.average = rule->average; ar.ste = ste != 0;
-  const unsigned bpw = unsigned((g.rows / 2 + pairs_per_block - 1) / pairs_per_block);
-  rc = dispatch_len(g.W, [&](auto wc) {
-    constexpr int W = decltype(wc)::value;
-    if (int r2 = ensure_tw<W>(st)) return r2;
-    const dim3 grid(bpw, unsigned(B));
-    switch (rule->gf[0]) {
-      case LX_GF_POLY_QUAD4: hipLaunchKernelGGL((rows_fwd_adj_kernel<W, LX_GF_POLY_QUAD4>), grid, dim3(256), 0, st, ar); break;
-      case LX_GF_GAUSSIAN: hipLaunchKernelGGL((rows_fwd_adj_kernel<W, LX_GF_GAUSSIAN>), grid, dim3(256), 0, st, ar); break;
-      case LX_GF_GAUSSIAN_TARGET:
-        hipLaunchKernelGGL((rows_fwd_adj_kernel<W, LX_GF_GAUSSIAN_TARGET>), grid, dim3(256), 0, st, ar);
-        break;
-      case LX_GF_STEP: hipLaunchKernelGGL((rows_fwd_adj_kernel<W, LX_GF_STEP>), grid, dim3(256), 0, st, ar); break;
-      case LX_GF_STAIRCASE: hipLaunchKernelGGL((rows_fwd_adj_kernel<W, LX_GF_STAIRCASE>), grid, dim3(256), 0, st, ar); break;
-      case LX_GF_TRIANGLE: hipLaunchKernelGGL((rows_fwd_adj_kernel<W, LX_GF_TRIANGLE>), grid, dim3(256), 0, st, ar); break;
-      default: hipLaunchKernelGGL((rows_fwd_adj_kernel<W, LX_GF_IDENTITY>), grid, dim3(256), 0, st, ar); break;
-    }
-    return launched("rows_fwd (adjoint)");
-  });
-  if (rc) return rc;
-  if ((rc = lx_adj_finish_11(B, part, int(bpw), ysum, grad_gf_d, gf_bstride, grad_w_d, w_bstride, grad_dt_d,
-                             dt_bstride, st)))
+  LxAdj11 c;
+  if ((rc = lx_adj11_init(rule, B, gf_d, gf_bstride, w_d, w_bstride, dt_d, dt_bstride, K_half_d, k_bstride, ste,
+                          lambda_d, grad_gf_d, grad_w_d, grad_dt_d, work_d, adj_work_d, stream, &c)))
     return rc;
-  // the rest of the adjoint potential on the spectra: lines, conj(K) (and the kernel gradient), lines, rows
-  if (g.naxes == 2) {
-    LineArgs la = line_args();
-    la.S = X; la.mode = 1;
-    if ((rc = launch_lines(g, 1, la, B, st))) return rc;
-  }
-  {
-    LineArgs la = line_args();
-    la.S = X; la.U = X;
-    la.Kh = static_cast<const float2*>(K_half_d); la.k_bstride = k_bstride; la.C = 1; la.K = 1; la.conjk = 1;
-    la.cin[0] = 0; la.acc_cin[0] = 0;
-    la.FAin = static_cast<const float2*>(state_spec_d);
-    la.dspec = static_cast<float2*>(dspec_d);
-    la.C_fa = 1;
-    if ((rc = launch_lines(g, 0, la, B, st))) return rc;
-  }
-  if (g.naxes == 2) {
-    LineArgs la = line_args();
-    la.S = X; la.U = X; la.mode = 2;
-    if ((rc = launch_lines(g, 1, la, B, st))) return rc;
-  }
-  RowArgs ra = row_args(rule);
-  ra.U = X; ra.A = lambda_d; ra.A_out = lambda_d;
-  ra.update = 1; ra.sel = 1; ra.average = 0; ra.state_fn = LX_STATE_SIMPLE;
-  return launch_rows_inv(g, ra, B, st);
+  if ((rc = lx_adj11_rows(c, potential_d, state_d, false, false))) return rc;
+  if ((rc = lx_adj11_lines(c, state_spec_d, dspec_d))) return rc;
+  return lx_adj11_last(c, true);
 }

@@ -273,9 +273,9 @@ LX_DEV lx_sums lx_block_pair_sums(const float* __restrict__ rec, int C, int64_t 
 
 // Block-wide sums of NV per-thread values (256-thread blocks), written to the block's slot of a
 // partial-sum array: part[(blockIdx.y * gridDim.x + blockIdx.x) * NV + i], summed later in a fixed
-// order (lx_adj_finish).
+// order (lx_adj_finish_wave, adj_ysum_kernel).
 template <int NV>
-LX_DEV void lx_block_partial(float (&v)[NV], float* part) {
+LX_DEV void lx_block_partial(float (&v)[NV], float* part, bool accumulate = false) {
   __shared__ float red[NV][4];
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
 #pragma unroll
@@ -287,18 +287,95 @@ LX_DEV void lx_block_partial(float (&v)[NV], float* part) {
   float* dst = part + (int64_t(blockIdx.y) * gridDim.x + blockIdx.x) * NV;
 #pragma unroll
   for (int i = 0; i < NV; ++i)
-    if (threadIdx.x == i) dst[i] = red[i][0] + red[i][1] + red[i][2] + red[i][3];
+    if (threadIdx.x == i) {
+      const float sum = red[i][0] + red[i][1] + red[i][2] + red[i][3];
+      dst[i] = accumulate ? dst[i] + sum : sum;
+    }
 }
 
 // Lanes per row transform of the row passes (lx_fft.hip): 4 up to 64 points, 8 up to 192, 16 above.
 constexpr int lx_row_tpr(int W) { return W <= 64 ? 4 : (W <= 192 ? 8 : 16); }
 
-// The gradient scalars of a one-channel one-kernel reverse step from per-block partial sums
-// part [B][per_img][4] (dm, ds, dw, ddt): summed per world in block order, then added into the
-// gradients (lx_plain.hip).  ysum: B * 4 floats of scratch.
-int lx_adj_finish_11(int64_t B, const float* part, int per_img, float* ysum, float* g_gf, int64_t gf_bs, float* g_w,
-                     int64_t w_bs, float* g_dt, int64_t dt_bs, hipStream_t st);
-void lx_adj_work_11(const lx_rule* r, int64_t B, void* work, float** part, float** ysum);
+// The gradient scalars of a one-channel one-kernel reverse step, summed by `blocks` extra workgroups
+// at the end of a later kernel's grid (the reverse step's last row pass) instead of a launch of
+// their own: one wave per gradient element (dm, ds per growth row; dw; ddt), whose lanes add the
+// partial sums part[b][blk][4] of the worlds that feed the element, block blk by lane blk mod 64 in
+// order, then a fixed-order wave sum.  Deterministic; a world's own elements do not depend on the
+// batch it is in.
+struct LxAdjFinish {
+  const float* part;
+  int64_t B;
+  int per;     // partial-sum rows per world
+  int blocks;  // extra workgroups (0: none)
+  float* g_gf;
+  int64_t gf_bs;
+  float* g_w;
+  int64_t w_bs;
+  float* g_dt;
+  int64_t dt_bs;
+};
+
+LX_DEV int64_t lx_adj_finish_outputs(const LxAdjFinish& f) {
+  const int64_t ngf = f.g_gf ? (f.gf_bs ? f.B : 1) * 2 : 0;
+  const int64_t nw = f.g_w ? (f.w_bs ? f.B : 1) : 0;
+  const int64_t ndt = f.g_dt ? (f.dt_bs ? f.B : 1) : 0;
+  return ngf + nw + ndt;
+}
+
+// element i of the gradient scalars, by the whole wave
+LX_DEV void lx_adj_finish_wave(const LxAdjFinish& f, int64_t i) {
+  const int ln = threadIdx.x & 63;
+  const int64_t ngf = f.g_gf ? (f.gf_bs ? f.B : 1) * 2 : 0;
+  const int64_t nw = f.g_w ? (f.w_bs ? f.B : 1) : 0;
+  int slot;
+  int64_t bt, bs;
+  float* dst;
+  if (i < ngf) {
+    bt = i / 2; slot = int(i - bt * 2); bs = f.gf_bs; dst = f.g_gf + bt * f.gf_bs + slot;
+  } else if (i < ngf + nw) {
+    bt = i - ngf; slot = 2; bs = f.w_bs; dst = f.g_w + bt * f.w_bs;
+  } else {
+    bt = i - ngf - nw; slot = 3; bs = f.dt_bs; dst = f.g_dt + bt * f.dt_bs;
+  }
+  const int64_t b0 = bs ? bt : 0, b1 = bs ? bt + 1 : f.B;
+  float acc = 0.f;
+  for (int64_t b = b0; b < b1; ++b)
+    for (int blk = ln; blk < f.per; blk += 64) acc += f.part[(b * f.per + blk) * 4 + slot];
+  acc = wave_sum(acc);
+  if (ln == 0) *dst += acc;
+}
+
+// One-channel one-kernel reverse steps in pieces (lx_fft.hip), for lx_adjoint_step and a run's reverse
+// pass: rows (the first row pass of a step: pointwise reverse + row transform into X; bridge: after
+// the later step's last row pass, in the same kernel), lines (the passes on X between), last (the
+// earliest step's last row pass; finish: the gradient scalars from the partial sums, which a run
+// accumulates over its steps).  lambda is updated in place; X is the step workspace.
+struct LxAdj11 {
+  const lx_rule* rule;
+  int64_t B;
+  void* stream;
+  const float *gf, *w, *dt;
+  int64_t gf_bs, w_bs, dt_bs;
+  const void* K_half;
+  int64_t k_bs;
+  int32_t ste;
+  float* lam;
+  float *g_gf, *g_w, *g_dt;
+  void* X;
+  float* part;
+  int bpw;  // row-pass blocks per world (partial-sum rows per world)
+};
+int lx_adj11_init(const lx_rule* rule, int64_t B, const float* gf_d, int64_t gf_bstride, const float* w_d,
+                  int64_t w_bstride, const float* dt_d, int64_t dt_bstride, const void* K_half_d, int64_t k_bstride,
+                  int32_t ste, float* lambda_d, float* grad_gf_d, float* grad_w_d, float* grad_dt_d, void* work_d,
+                  void* adj_work_d, void* stream, LxAdj11* c);
+int lx_adj11_rows(const LxAdj11& c, const float* U, const float* A, bool bridge, bool accumulate);
+int lx_adj11_lines(const LxAdj11& c, const void* state_spec_d, void* dspec_d);
+int lx_adj11_last(const LxAdj11& c, bool finish);
+
+// The partial-sum rows [B][per][4] (dm, ds, dw, ddt) of a one-channel one-kernel reverse step inside
+// an lx_adjoint_workspace_size workspace (lx_plain.hip).
+float* lx_adj_work_11(const lx_rule* r, int64_t B, void* work);
 
 // Autotuner (lx_tune.hip): knob values by launch shape.  lx_tune_lookup gives a forced or recorded
 // value (1) or nothing (0); lx_tune_should_time says whether a launcher may time its candidates now

@@ -588,22 +588,9 @@ AdjWork adj_work_layout(const lx_rule* r, int64_t B) {
 
 }  // namespace
 
-int lx_adj_finish_11(int64_t B, const float* part, int per_img, float* ysum, float* g_gf, int64_t gf_bs, float* g_w,
-                     int64_t w_bs, float* g_dt, int64_t dt_bs, hipStream_t st) {
-  if (!g_gf && !g_w && !g_dt) return LX_OK;
-  AdjArgs a{};
-  a.g_gf = g_gf; a.gf_bs = gf_bs; a.g_w = g_w; a.w_bs = w_bs; a.g_dt = g_dt; a.dt_bs = dt_bs;
-  a.C = 1; a.K = 1; a.ysum_g = ysum; a.ysum_s = nullptr;
-  hipLaunchKernelGGL(adj_ysum_kernel, dim3(grid_for(B * 4, 256)), dim3(256), 0, st, part, ysum, B, 4, per_img);
-  hipLaunchKernelGGL(adj_targets_kernel, dim3(1), dim3(256), 0, st, a, B, 1);
-  return launched("adjoint gradient sums");
-}
-
-// the partial-sum rows and sums of the fused one-kernel step inside an lx_adjoint_workspace_size workspace
-void lx_adj_work_11(const lx_rule* r, int64_t B, void* work, float** part, float** ysum) {
-  const AdjWork l = adj_work_layout(r, B);
-  *part = reinterpret_cast<float*>(static_cast<char*>(work) + l.part_g);
-  *ysum = reinterpret_cast<float*>(static_cast<char*>(work) + l.ysum_g);
+// the partial-sum rows of the fused one-kernel step inside an lx_adjoint_workspace_size workspace
+float* lx_adj_work_11(const lx_rule* r, int64_t B, void* work) {
+  return reinterpret_cast<float*>(static_cast<char*>(work) + adj_work_layout(r, B).part_g);
 }
 
 extern "C" int lx_adjoint_workspace_size(const lx_rule* rule, int64_t B, size_t* bytes) {

@@ -336,35 +336,91 @@ def test_field_and_state_gradients_like_reference():
         assert np.allclose(_np(tA.grad), dA_ref, atol=1e-7) and np.allclose(_np(tG.grad), 0.1, atol=1e-7)
 
 
-def test_graph_replays_match_eager():
-    """autograd.run with its segments replayed as HIP graphs equals the eager run on three
-    calls in a row (new inputs, same shapes, a tail segment), with eager work in between."""
+def test_interleaved_runs_match_runs_alone():
+    """Two autograd.run nodes alive at once (forward 1, forward 2, a forward without gradients,
+    backward 2, backward 1), each with its own checkpoints and kept segment, equal the same runs
+    done one at a time, bit for bit; a second pass repeats them exactly (deterministic sums)."""
     from leniax_amd import autograd as AG
-    from leniax_amd import core
     K = _dev(_asym_K(30, 1, 1, 64, 64))
     w = _dev(np.array([[1.0]], np.float32))
-    for rep in range(3):
-        print('replay', rep, flush=True)
+    cases = []
+    for rep in range(2):
         A = _world(31 + rep, 2, 1, 64, 64, 24)
         gf = np.array([[0.25 + 0.01 * rep, 0.05]], np.float32)
         Rw = _dev(np.random.default_rng(40 + rep).normal(size=A.shape).astype(np.float32))
-        res = []
-        for graphs in (False, True):
-            tA, tgf = _dev(A).requires_grad_(), _dev(gf).requires_grad_()
-            tdt = torch.tensor(0.1, device='cuda', requires_grad=True)
-            fin = AG.run(tA, K, tgf, w, tdt, 20, ckpt_every=8, graphs=graphs)
-            (fin * Rw).sum().backward()
-            torch.cuda.synchronize()
-            res.append((_np(fin), _np(tA.grad), _np(tgf.grad), float(tdt.grad)))
-            # unrelated eager work on the same stream between graph replays
-            core.fused_update(_dev(A), K, _dev(gf), w, 0.1, None, ('gaussian',), core.weighted_mean, 'v1')
-            torch.cuda.synchronize()
-        (f0, a0, g0, d0), (f1, a1, g1, d1) = res
-        assert np.array_equal(f0, f1)
-        assert np.array_equal(a0, a1)
-        # the gradient scalars are reduced in a fixed order (no atomics): replays repeat them exactly
-        assert np.array_equal(g0, g1)
-        assert d0 == d1
+        cases.append((A, gf, Rw))
+
+    def leaves(A, gf):
+        return (_dev(A).requires_grad_(), _dev(gf).requires_grad_(),
+                torch.tensor(0.1, device='cuda', requires_grad=True))
+
+    def result(fin, tA, tgf, tdt):
+        return _np(fin), _np(tA.grad), _np(tgf.grad), float(tdt.grad)
+
+    alone = []
+    for A, gf, Rw in cases:
+        tA, tgf, tdt = leaves(A, gf)
+        fin = AG.run(tA, K, tgf, w, tdt, 20, ckpt_every=8)
+        (fin * Rw).sum().backward()
+        torch.cuda.synchronize()
+        alone.append(result(fin, tA, tgf, tdt))
+    for rep in range(2):
+        print('interleaved pass', rep, flush=True)
+        ls = [leaves(A, gf) for A, gf, _ in cases]
+        fins = [AG.run(l[0], K, l[1], w, l[2], 20, ckpt_every=8) for l in ls]
+        with torch.no_grad():
+            AG.run(_dev(cases[0][0]), K, _dev(cases[1][1]), w, 0.1, 20, ckpt_every=8)
+        for i in (1, 0):
+            (fins[i] * cases[i][2]).sum().backward()
+        torch.cuda.synchronize()
+        for i in range(2):
+            got = result(fins[i], *ls[i])
+            assert np.array_equal(got[0], alone[i][0])
+            assert np.array_equal(got[1], alone[i][1])
+            assert np.array_equal(got[2], alone[i][2])
+            assert got[3] == alone[i][3]
+
+
+@pytest.mark.parametrize('state_fn,ste,dims', [('v1', True, (64, 64)), ('v1', False, (64, 96)),
+                                               ('v2', True, (64, 64)), ('v1', True, (16, 32, 48))])
+def test_chained_run_reverse_equals_step_by_step(state_fn, ste, dims):
+    """autograd.run (one channel, one kernel: each reverse step's last row pass fused with the first
+    of the step before it, gradient scalars accumulated over the steps) against the same steps as
+    separate autograd.update nodes (one lx_adjoint_step each): dL/dA0 bit for bit (the same
+    per-cell arithmetic), the parameter and kernel gradients to their summation order."""
+    from leniax_amd import autograd as AG
+    print('chained reverse', state_fn, ste, dims, flush=True)
+    N, steps, ckpt = 2, 11, 4
+    rng = np.random.default_rng(17)
+    A = np.zeros((N, 1) + dims, np.float32)
+    sl = tuple(slice(d // 2 - 6, d // 2 + 6) for d in dims)
+    A[(slice(None), slice(None)) + sl] = rng.uniform(0, 1, (N, 1) + (12,) * len(dims))
+    k = np.zeros(dims)
+    k[tuple(slice(d // 2 - 3, d // 2 + 4) for d in dims)] = rng.uniform(size=(7,) * len(dims))
+    K = np.fft.fftn(np.fft.fftshift(k / k.sum()))[None, None, None].astype(np.complex64)
+    gf = np.array([[0.22, 0.06]], np.float32)
+    w = np.array([[1.0]], np.float32)
+    Rw = _dev(rng.normal(size=A.shape).astype(np.float32))
+    res = []
+    for chained in (True, False):
+        tA, tgf = _dev(A).requires_grad_(), _dev(gf).requires_grad_()
+        tK = _dev(K).requires_grad_()
+        tdt = torch.tensor(0.1, device='cuda', requires_grad=True)
+        if chained:
+            fin = AG.run(tA, tK, tgf, _dev(w), tdt, steps, state_slug=state_fn, ste=ste, ckpt_every=ckpt)
+        else:
+            fin = tA
+            for _ in range(steps):
+                fin = AG.update(fin, tK, tgf, _dev(w), tdt, state_slug=state_fn, ste=ste)[0]
+        (fin * Rw).sum().backward()
+        torch.cuda.synchronize()
+        res.append((_np(fin), _np(tA.grad), _np(tgf.grad), float(tdt.grad), _np(tK.grad)))
+    (f0, a0, g0, d0, k0), (f1, a1, g1, d1, k1) = res
+    assert np.array_equal(f0, f1)
+    assert np.array_equal(a0, a1)
+    assert _rel(g0, g1) < 1e-5
+    assert abs(d0 - d1) <= 1e-5 * max(abs(d1), 1e-6)
+    assert _rel(k0, k1) < 1e-5
 
 
 @pytest.mark.parametrize('C,K', [(1, 1), (2, 3)])
@@ -414,10 +470,11 @@ def test_pointwise_adjoint_gradients_do_not_depend_on_the_batch(C, K):
 
 @pytest.mark.parametrize('ste,ckpt,spec', [(True, 8, False), (False, 7, True)])
 def test_grad_run_c_abi_equals_autograd_run(ste, ckpt, spec):
-    """lx_grad_run_forward / lx_grad_run_backward (the checkpointed gradient of a run through the C
-    ABI) against autograd.run (eager), which test_checkpointed_run_matches_oracle pins to the
-    float64 oracle: the same kernels in the same order, so the final state, dL/dA0 and the growth
-    and dt gradients agree bit for bit, and the kernel-spectrum accumulator gives autograd's dL/dK."""
+    """lx_grad_run_forward / lx_grad_run_backward called directly through the C ABI (a caller-sized
+    workspace, the kernel spectra prepared by hand) against autograd.run, which
+    test_checkpointed_run_matches_oracle pins to the float64 oracle: the final state, dL/dA0 and the
+    growth and dt gradients agree bit for bit, and the kernel-spectrum accumulator gives autograd's
+    dL/dK."""
     import ctypes
     from leniax_amd import autograd as AG
     from leniax_amd import _lib, engine
@@ -429,7 +486,7 @@ def test_grad_run_c_abi_equals_autograd_run(ste, ckpt, spec):
     print(f'grad run ste={ste} ckpt={ckpt} spec={spec}', flush=True)
     tA, tgf, tK = _dev(A).requires_grad_(), _dev(gf).requires_grad_(), _dev(K).requires_grad_(spec)
     tdt = torch.tensor(0.1, device='cuda', requires_grad=True)
-    fin = AG.run(tA, tK, tgf, _dev(w), tdt, steps, ckpt_every=ckpt, ste=ste, graphs=False)
+    fin = AG.run(tA, tK, tgf, _dev(w), tdt, steps, ckpt_every=ckpt, ste=ste)
     (fin * _dev(Rw)).sum().backward()
     torch.cuda.synchronize()
 
