@@ -1800,6 +1800,21 @@ __global__ __launch_bounds__(256) void rows_fwd_adj_kernel(AdjRowArgs a) {
   const int mode = a.state_fn == LX_STATE_V2 ? 2 : (a.state_fn == LX_STATE_V1 && !a.ste ? 1 : 0);
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
   float2 x[P];
+  // one base pointer per array: every point is a constant offset from it
+  const float* pU = a.U + off + t;
+  float* pL = a.lam + off + t;
+  const float* pA = a.A + off + t;
+  // every point's potential and lambda loaded at once (their latency overlaps the bridge's inverse
+  // transform, and the loop below waits once instead of once per point)
+  float uv[2][P], lv[2][P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      uv[r][m] = pU[r * W + TPR * m];
+      lv[r][m] = pL[r * W + TPR * m];
+    }
+  }
   if constexpr (BRIDGE) {
     // the later step's adjoint potential along W: stage the pair's half spectra, repack them as one
     // complex row Z[k] = A[k] + i B[k] (A[k] = conj(A[W-k]) above W/2), inverse transform
@@ -1835,10 +1850,6 @@ __global__ __launch_bounds__(256) void rows_fwd_adj_kernel(AdjRowArgs a) {
     wave_lds_fence();
     __builtin_amdgcn_sched_barrier(0);
   }
-  // one base pointer per array: every point is a constant offset from it
-  const float* pU = a.U + off + t;
-  float* pL = a.lam + off + t;
-  const float* pA = a.A + off + t;
   // the pointwise reverse of every point the lanes hold, straight-line for each update rule (the
   // rule and the weighting are uniform: branches around the loop, not in it); the field is the
   // kernel's growth recomputed from the potential as the forward computes it
@@ -1851,8 +1862,8 @@ __global__ __launch_bounds__(256) void rows_fwd_adj_kernel(AdjRowArgs a) {
 #pragma unroll
       for (int r = 0; r < 2; ++r) {
         const int o = r * W + TPR * m;
-        const float U = pU[o];
-        float L = pL[o];
+        const float U = uv[r][m];
+        float L = lv[r][m];
         if constexpr (BRIDGE) {
           // dL/d(state in) of the later step: its direct part plus the correlation term, as
           // rows_inv_kernel's selection mode adds them
