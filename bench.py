@@ -462,7 +462,10 @@ def main():
     full_ms = e0.elapsed_time(e1)
     full_ws = B * MAX_STEPS / (full_ms / 1e3)
     log(f'fused full run: {full_ms:.1f} ms for {B * MAX_STEPS} world-steps -> {full_ws / 1e6:.2f} M world-steps/s')
-    # the same search under leniax's own stop rule (its heuristics, leniax/statistics.py:134-205)
+    # the same search under leniax's own stop rule (its heuristics, leniax/statistics.py:134-205);
+    # one untimed call first (the autotuner decides this shape's probe length on its first call)
+    fused(stop='leniax')
+    torch.cuda.synchronize()
     e0.record(stream)
     lres = fused(stop='leniax')
     e1.record(stream)
