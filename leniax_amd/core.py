@@ -49,13 +49,18 @@ def _spectra(rule, K, slots):
 
 
 def _workspace(rule, B):
+    """The step workspace of the current device and stream: calls on one stream run in order and
+    share it; another stream gets its own (calls on two streams at once would otherwise race on the
+    spectra).  A buffer that grows is replaced on its own stream, where the caching allocator
+    reuses the old memory only after the stream's earlier work."""
     nbytes = ctypes.c_size_t(0)
     _lib.call('lx_workspace_size', ctypes.byref(rule), B, ctypes.byref(nbytes))
     dev = engine.device()
-    buf = _WORK.get(dev.index)
+    key = (dev.index, torch.cuda.current_stream(dev).cuda_stream)
+    buf = _WORK.get(key)
     if buf is None or buf.numel() < nbytes.value:
         buf = torch.empty(max(nbytes.value, 1), dtype=torch.uint8, device=dev)
-        _WORK[dev.index] = buf
+        _WORK[key] = buf
     return buf
 
 

@@ -122,3 +122,34 @@ def test_update_extreme_sizes_match_oracle(shape):
     assert eu < 2e-6
     assert eg < 2e-4  # the Gaussian's slope 1/s^2 ~ 1e3 times the potential's rounding
     assert float(np.abs(_np(new) - L.get_state(A, _np(field), dt)).max()) < 1e-6
+
+
+def test_side_streams_match_the_default_stream():
+    """The step called on two side streams at once (each with its own workspace: they run
+    concurrently) and first called there for its world size, against the same step on the default
+    stream: the same kernels, so the same results bit for bit."""
+    from leniax_amd import core
+    from leniax_amd.growth_functions import register as gfr
+    H, W = 192, 192
+    print('side streams', (H, W), flush=True)
+    R = 20.
+    K, mp = L.get_kernels_and_mapping(_bump(R, 0.2, 0.03), [H, W], 1, R, np.float32)
+    rng = np.random.default_rng(5)
+    As = [(rng.random((3, 1, H, W), dtype=np.float32) * (rng.random((3, 1, H, W)) > 0.4)).astype(np.float32)
+          for _ in range(2)]
+    args = (K, mp.get_gf_params(), mp.get_kernels_weight_per_channel(), np.float32(0.1), mp.tc_indices(),
+            (gfr['gaussian'],), core.weighted_mean, 'v1')
+    ins = [torch.as_tensor(a).cuda() for a in As]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    side = [None, None]
+    for i, s in enumerate(streams):
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            side[i] = [core.fused_update(ins[i], *args) for _ in range(2)][-1]
+    torch.cuda.synchronize()
+    for i in range(2):
+        ref = core.fused_update(ins[i], *args)
+        torch.cuda.synchronize()
+        for x, y in zip(side[i], ref):
+            assert torch.equal(x, y)
