@@ -419,6 +419,57 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
   }
 }
 
+// The convolution pass of the adjoint potential with the kernel-spectrum gradient, one channel and
+// one kernel (the reverse steps of lx_adjoint_step and of a run): the line's kernel spectrum, state
+// spectrum and gradient accumulator are loaded before the forward transform, so that their latency
+// overlaps it instead of following it (the single-world gradient is bound by these chains).  The
+// same arithmetic as lines_kernel's mode 0 with dspec.
+template <int L, int T>
+__global__ __launch_bounds__(256) void lines_gradk_kernel(LineArgs a) {
+  using LC = Lanes<L, T>;
+  constexpr int TPR = LC::TPR, P = LC::P;
+  __shared__ __align__(16) float2 xch_all[LC::XL ? LC::XCH : 2];
+  const int lane = threadIdx.x & 63;
+  const int t = threadIdx.x % TPR;
+  float2* xch = xch_all + (LC::XL ? (threadIdx.x / TPR) * (TPR * LC::XRS) : 0);
+  const int64_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : int64_t(blockIdx.x);
+  const int64_t lid = (blk * 256 + threadIdx.x) / TPR;
+  const bool live = lid < a.nlines;
+  const int64_t per_img = int64_t(a.A) * a.WH;
+  const int64_t img = live ? lid / per_img : 0;  // one channel: image = world
+  const int rem = live ? int(lid - img * per_img) : 0;
+  const int aa = rem / a.WH;
+  const int off = aa * int(a.a_stride) + (rem - aa * a.WH);
+  const int st = int(a.st);
+  const float2* src = a.S + img * a.img_elems + off;
+  const float2* kk = a.Kh + img * a.k_bstride + off;
+  const float2* fa = a.FAin + img * a.img_elems + off;
+  float2* ds = a.dspec + img * a.img_elems + off;
+  float2 x[P], kv[P], fv[P], dv[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) x[m] = src[(TPR * m + t) * st];
+#pragma unroll
+  for (int e = 0; e < P; ++e) {
+    const int o = freq_of<L, T>(t, e) * st;
+    kv[e] = kk[o];
+    fv[e] = fa[o];
+    dv[e] = ds[o];
+  }
+  fft_group<L, false, T>(x, t, lane, xch);
+  const float csign = a.conjk ? -1.f : 1.f;
+#pragma unroll
+  for (int e = 0; e < P; ++e) {
+    if (live) ds[freq_of<L, T>(t, e) * st] = cadd(dv[e], cmulc(x[e], fv[e]));
+    x[e] = cmul(x[e], make_float2(kv[e].x, csign * kv[e].y));
+  }
+  fft_group<L, true, T>(x, t, lane, xch);
+  float2* dst = a.U + img * a.img_elems + off;
+  if (live) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) dst[(TPR * m + t) * st] = x[m];
+  }
+}
+
 // The convolution pass (mode 0 without FA or dspec) with its global reads and writes staged
 // through LDS.  A block holds LB = 256 / TPR lines; neighbouring lines are neighbouring half-
 // spectrum columns, so a load or store instruction of the staging loop covers LB columns of
@@ -865,6 +916,12 @@ int launch_lines_v(LineArgs la, int L, int64_t lines, LineVariant v, hipStream_t
             hipLaunchKernelGGL((conv_staged_kernel<LL, LIST>), grid, dim3(256), 0, st, la);
             return launched("conv (staged)");
           }
+        }
+        // latency-bound grids (less than two blocks per CU) only: its registers cost occupancy
+        if (!LIST && la.dspec && !la.FA && la.C == 1 && la.K == 1 && la.C_fa == 1 && la.acc_cin[0] == 0 &&
+            la.cin[0] == 0 && grid.x < 512) {
+          hipLaunchKernelGGL((lines_gradk_kernel<LL, T>), grid, dim3(256), 0, st, la);
+          return launched("lines (kernel gradient)");
         }
         hipLaunchKernelGGL((lines_kernel<LL, 0, LIST, T>), grid, dim3(256), 0, st, la);
       } else if (la.mode == 1) {
