@@ -419,6 +419,50 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
   }
 }
 
+// The convolution pass of one channel and one kernel on a latency-bound grid (a few worlds): the
+// line's kernel spectrum is loaded before the forward transform, so that its latency overlaps the
+// transform instead of following it.  The same arithmetic as lines_kernel's mode 0.
+template <int L, int T>
+__global__ __launch_bounds__(256) void lines_conv1_kernel(LineArgs a) {
+  using LC = Lanes<L, T>;
+  constexpr int TPR = LC::TPR, P = LC::P;
+  __shared__ __align__(16) float2 xch_all[LC::XL ? LC::XCH : 2];
+  const int lane = threadIdx.x & 63;
+  const int t = threadIdx.x % TPR;
+  float2* xch = xch_all + (LC::XL ? (threadIdx.x / TPR) * (TPR * LC::XRS) : 0);
+  const int64_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : int64_t(blockIdx.x);
+  const int64_t lid = (blk * 256 + threadIdx.x) / TPR;
+  const bool live = lid < a.nlines;
+  const int64_t per_img = int64_t(a.A) * a.WH;
+  const int64_t img = live ? lid / per_img : 0;  // one channel: image = world
+  const int rem = live ? int(lid - img * per_img) : 0;
+  const int aa = rem / a.WH;
+  const int off = aa * int(a.a_stride) + (rem - aa * a.WH);
+  const int st = int(a.st);
+  const float2* src = a.S + img * a.img_elems + off;
+  const float2* kk = a.Kh + img * a.k_bstride + off;
+  float2 x[P], kv[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) x[m] = src[(TPR * m + t) * st];
+#pragma unroll
+  for (int e = 0; e < P; ++e) kv[e] = kk[freq_of<L, T>(t, e) * st];
+  fft_group<L, false, T>(x, t, lane, xch);
+  if (a.FA && live) {
+    float2* fa = a.FA + img * a.img_elems + off;
+#pragma unroll
+    for (int e = 0; e < P; ++e) fa[freq_of<L, T>(t, e) * st] = x[e];
+  }
+  const float csign = a.conjk ? -1.f : 1.f;
+#pragma unroll
+  for (int e = 0; e < P; ++e) x[e] = cmul(x[e], make_float2(kv[e].x, csign * kv[e].y));
+  fft_group<L, true, T>(x, t, lane, xch);
+  float2* dst = a.U + img * a.img_elems + off;
+  if (live) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) dst[(TPR * m + t) * st] = x[m];
+  }
+}
+
 // The convolution pass of the adjoint potential with the kernel-spectrum gradient, one channel and
 // one kernel (the reverse steps of lx_adjoint_step and of a run): the line's kernel spectrum, state
 // spectrum and gradient accumulator are loaded before the forward transform, so that their latency
@@ -922,6 +966,10 @@ int launch_lines_v(LineArgs la, int L, int64_t lines, LineVariant v, hipStream_t
             la.cin[0] == 0 && grid.x < 512) {
           hipLaunchKernelGGL((lines_gradk_kernel<LL, T>), grid, dim3(256), 0, st, la);
           return launched("lines (kernel gradient)");
+        }
+        if (!LIST && !la.dspec && la.C == 1 && la.K == 1 && la.cin[0] == 0 && grid.x < 512) {
+          hipLaunchKernelGGL((lines_conv1_kernel<LL, T>), grid, dim3(256), 0, st, la);
+          return launched("lines (convolution, one kernel)");
         }
         hipLaunchKernelGGL((lines_kernel<LL, 0, LIST, T>), grid, dim3(256), 0, st, la);
       } else if (la.mode == 1) {
