@@ -1572,36 +1572,38 @@ __global__ __launch_bounds__(kPairThreads) void run_stats_kernel(RunStatArgs a) 
 }
 
 // One block: the next step's live list, the slots whose world goes on (in order) and then worlds
-// from the queue while slots are free.
+// from the queue while slots are free.  NT threads: 1024, or one wave for the buckets of at most 64
+// slots (the scan's rounds are then barriers of a single wave).
 constexpr int kCtlThreads = 1024;
-__global__ __launch_bounds__(kCtlThreads) void run_control_kernel(RunCtl c, int par, int slots, int64_t B) {
-  __shared__ int scan[kCtlThreads];
+template <int NT>
+__global__ __launch_bounds__(NT) void run_control_kernel(RunCtl c, int par, int slots, int64_t B) {
+  __shared__ int scan[NT];
   __shared__ int64_t head_s;
   const int tid = threadIdx.x;
   const int32_t* in = par ? c.list1 : c.list0;
   int32_t* out = par ? c.list0 : c.list1;
   const int n = c.nlive[par];
-  const int per = (n + kCtlThreads - 1) / kCtlThreads;
+  const int per = (n + NT - 1) / NT;
   const int s0 = min(tid * per, n), s1 = min(s0 + per, n);
   int cnt = 0;
   for (int s = s0; s < s1; ++s) cnt += c.done[s] ? 0 : 1;
   scan[tid] = cnt;
   if (tid == 0) head_s = *c.head;
   __syncthreads();
-  for (int off = 1; off < kCtlThreads; off <<= 1) {  // inclusive scan of the survivor counts
+  for (int off = 1; off < NT; off <<= 1) {  // inclusive scan of the survivor counts
     const int v = tid >= off ? scan[tid - off] : 0;
     __syncthreads();
     scan[tid] += v;
     __syncthreads();
   }
   int o = scan[tid] - cnt;
-  const int kept = scan[kCtlThreads - 1];
+  const int kept = scan[NT - 1];
   for (int s = s0; s < s1; ++s)
     if (!c.done[s]) out[o++] = in[s];
   const int64_t h = head_s;
   const int64_t room = slots - kept, left = B - h;
   const int take = int(room < left ? room : left);
-  for (int i = tid; i < take; i += kCtlThreads) out[kept + i] = int32_t(h + i);
+  for (int i = tid; i < take; i += NT) out[kept + i] = int32_t(h + i);
   if (tid == 0) {
     *c.head = int32_t(h + take);
     c.nlive[par ^ 1] = kept + take;
@@ -1742,7 +1744,10 @@ int enqueue_step(const PassesRun& r, int par, int bucket, hipStream_t st) {
   sa.cells_total = float(cells_per_world(rule) * C);
   sa.stats = a->stats_d; sa.n_out = a->n_d; sa.steps_out = a->steps_d;
   hipLaunchKernelGGL(run_stats_kernel, dim3(unsigned(bucket)), dim3(kPairThreads), 0, st, sa);
-  hipLaunchKernelGGL(run_control_kernel, dim3(1), dim3(kCtlThreads), 0, st, r.ctl, par, r.slots, a->B);
+  if (bucket <= 64)
+    hipLaunchKernelGGL(run_control_kernel<64>, dim3(1), dim3(64), 0, st, r.ctl, par, r.slots, a->B);
+  else
+    hipLaunchKernelGGL(run_control_kernel<kCtlThreads>, dim3(1), dim3(kCtlThreads), 0, st, r.ctl, par, r.slots, a->B);
   return launched("run statistics / control");
 }
 
