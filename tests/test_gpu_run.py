@@ -60,6 +60,29 @@ def test_passes_run_equals_step_loop(case, slots):
         assert torch.equal(got, ref)
 
 
+def test_passes_run_large_batch_equals_step_loop():
+    """384 worlds of the 3C6K 64^2 rule (per-world spectra): the run's grids are past the
+    latency-bound threshold (512 blocks), so its passes take the throughput variants; the same
+    results as the per-step path bit for bit."""
+    from leniax_amd import engine
+    print('passes run, 384 worlds', flush=True)
+    rule, Kh, gf, w, a0 = _rule_case('3c6k_64')
+    reps = 128
+    Kh = Kh.repeat((reps,) + (1,) * (Kh.dim() - 1)).contiguous()
+    gf = gf.repeat((reps,) + (1,) * (gf.dim() - 1)).contiguous()
+    w = w.repeat((reps,) + (1,) * (w.dim() - 1)).contiguous()
+    a0 = a0.repeat((reps,) + (1,) * (a0.dim() - 1)).contiguous()
+    assert engine.run_path(rule) == 'passes' and a0.shape[0] == 384
+    dt = torch.tensor([0.1], device=a0.device)
+    steps = 6
+    rows, n, final, _ = _step_loop(rule, Kh, gf, w, a0, steps, dt)
+    res = _run(rule, Kh, gf, w, a0, steps, dt)
+    torch.cuda.synchronize()
+    assert torch.equal(res['cells'], final)
+    assert torch.equal(res['stats'], rows.transpose(0, 1))
+    assert torch.equal(res['n'].to(torch.float32), n)
+
+
 def _uneven_case(B=7):
     """3 channels, 6 kernels, 64^2 (per-world spectra and parameters): live soups, an empty world
     (dies at once) and a nearly full one (over 90% of the cells alive)."""
