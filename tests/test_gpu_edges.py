@@ -153,3 +153,22 @@ def test_side_streams_match_the_default_stream():
         torch.cuda.synchronize()
         for x, y in zip(side[i], ref):
             assert torch.equal(x, y)
+
+
+def test_convolution_variants_agree_on_large_and_small_grids():
+    """The potential of 1024 worlds of 128^2 (a throughput grid: the general convolution pass) and of
+    the first two worlds alone (a latency-bound grid: the one-kernel variant that loads the kernel
+    spectrum before the forward transform): the same arithmetic, so the same potentials bit for bit;
+    both against the oracle on the first two worlds."""
+    from leniax_amd import core
+    H = W = 128
+    R = 13.
+    K, mp = L.get_kernels_and_mapping(_bump(R), [H, W], 1, R, np.float32)
+    rng = np.random.default_rng(9)
+    A = (rng.random((1024, 1, H, W), dtype=np.float32) * (rng.random((1024, 1, H, W)) > 0.5)).astype(np.float32)
+    big = core.get_potential_fft(torch.as_tensor(A).cuda(), K, mp.tc_indices())
+    small = core.get_potential_fft(torch.as_tensor(A[:2]).cuda(), K, mp.tc_indices())
+    torch.cuda.synchronize()
+    assert torch.equal(big[:2], small)
+    ref = L.get_potential_fft(A[:2], K, mp.tc_indices())
+    assert float(np.abs(_np(small) - ref).max()) < 2e-6
