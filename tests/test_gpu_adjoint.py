@@ -586,3 +586,47 @@ def test_fused_adjoint_step_equals_pointwise_then_potential(dims, state_fn, ste)
     for x, y in ((g0, g1), (w0, w1), (d0, d1)):
         assert np.allclose(x, y, rtol=0, atol=1e-7 * scale * max(1., float(np.abs(y).max()))), (x, y)
     assert np.abs(g1).max() > 0 and np.abs(d1).max() > 0
+
+
+def test_adjoint_step_large_and_small_grids_agree():
+    """lx_adjoint_step on 1024 worlds of 64^2 (a throughput grid: the general convolution pass with
+    the kernel gradient) and on its first two worlds alone (a latency-bound grid: the variant that
+    loads the spectra before the forward transform), per-world parameters: dL/d(state in), the
+    kernel-spectrum gradient and every world's gradient scalars of the first two worlds bit for bit."""
+    import ctypes
+    from leniax_amd import _lib, autograd as AG, engine
+    rng = np.random.default_rng(23)
+    dims = (64, 64)
+    rule = engine.make_rule(list(dims), 1, [0], [_lib.GROWTH_IDS['gaussian']], _lib.STATE_FN['v1'], 1)
+    Bb = 1024
+    A = _dev(rng.uniform(size=(Bb, 1) + dims).astype(np.float32))
+    U = _dev(rng.uniform(0, 0.4, size=(Bb, 1) + dims).astype(np.float32))
+    lam0 = _dev(rng.normal(size=(Bb, 1) + dims).astype(np.float32))
+    gf = _dev(np.tile(np.array([[0.2, 0.05]], np.float32), (Bb, 1)))
+    w = _dev(np.tile(np.array([0.7], np.float32), Bb))
+    dt = _dev(np.full(Bb, 0.1, np.float32))
+    k = np.zeros(dims)
+    k[tuple(slice(d // 2 - 3, d // 2 + 3) for d in dims)] = rng.uniform(size=(6,) * len(dims))
+    K = np.fft.fftn(np.fft.fftshift(k / k.sum()))[None, None, None].astype(np.complex64)
+    Kh = engine.prepare_spectra(rule, K, [0])
+    FA = AG.rfftn(A, len(dims))
+    outs = []
+    for B in (Bb, 2):
+        print('adjoint step, worlds', B, flush=True)
+        nb = ctypes.c_size_t(0)
+        _lib.call('lx_workspace_size', ctypes.byref(rule), B, ctypes.byref(nb))
+        work = torch.empty(nb.value, dtype=torch.uint8, device='cuda')
+        _lib.call('lx_adjoint_workspace_size', ctypes.byref(rule), B, ctypes.byref(nb))
+        aw = torch.empty(nb.value, dtype=torch.uint8, device='cuda')
+        lam = lam0[:B].clone()
+        ggf, gw, gdt = torch.zeros_like(gf[:B]), torch.zeros_like(w[:B]), torch.zeros_like(dt[:B])
+        dspec = torch.zeros((B, 1) + tuple(FA.shape[2:]), dtype=torch.complex64, device='cuda')
+        _lib.call('lx_adjoint_step', ctypes.byref(rule), B, engine.ptr(A), None, engine.ptr(U), engine.ptr(gf), 2,
+                  engine.ptr(w), 1, engine.ptr(dt), 1, engine.ptr(Kh), 0, 1, engine.ptr(FA), engine.ptr(dspec),
+                  engine.ptr(lam), None, engine.ptr(ggf), engine.ptr(gw), engine.ptr(gdt), engine.ptr(work),
+                  engine.ptr(aw), engine.stream_ptr())
+        torch.cuda.synchronize()
+        outs.append([_np(x[:2]) for x in (lam, dspec, ggf, gw, gdt)])
+    for x, y in zip(*outs):
+        assert np.array_equal(x, y)
+    assert np.abs(outs[1][2]).max() > 0
