@@ -745,7 +745,19 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
     }
     __builtin_amdgcn_sched_barrier(0);  // the loads are issued here, before the transform that follows
   };
+  // PF: the pair's input state, loaded before the transforms as well
+  float av0[PF ? P : 1], av1[PF ? P : 1];
   if constexpr (PF) {
+    if (a.update && live) {
+      const float* Ap = LIST ? run_state(a.rv, bw, false) + (int64_t(c) * a.rows + r2) * W
+                             : a.A + (img * a.rows + r2) * int64_t(W);
+      const int tl = threadIdx.x % TPR;
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        av0[m] = Ap[TPR * m + tl];
+        av1[m] = Ap[W + TPR * m + tl];
+      }
+    }
     for (int k = 0; k < a.K; ++k) {
       const float wk = a.sel ? (a.cin[k] == c ? 1.f : 0.f) : pw[c * a.K + k];
       int owner = 0;
@@ -897,7 +909,14 @@ __global__ __launch_bounds__(256) void rows_inv_kernel(RowArgs a) {
     float g0 = G0[m], g1 = G1[m];
     if (a.average) { g0 = (g0 / den); g1 = (g1 / den); }
     if (fld) { fld[col] = g0; fld[W + col] = g1; }
-    const float v0 = Ain[col], v1 = Ain[W + col];
+    float v0, v1;
+    if constexpr (PF) {
+      v0 = av0[m];
+      v1 = av1[m];
+    } else {
+      v0 = Ain[col];
+      v1 = Ain[W + col];
+    }
     if (LIST && chist) { chist[col] = v0; chist[W + col] = v1; }
     if (STATS) {
       int zc = col - sh[2];
@@ -1147,6 +1166,11 @@ int launch_rows_inv(const Geom& g, RowArgs ra, int64_t nimg, hipStream_t st) {
     const dim3 grid(blocks_for(ra.npairs * Lanes<W>::TPR) + unsigned(ra.fin.blocks));
     const bool uni = (ra.rows / 2) % (64 / Lanes<W>::TPR) == 0;  // waves never straddle images
     if (ra.part && ra.fin.blocks) return lx_fail(LX_EINVAL, "rows_inv: statistics and gradient sums together");
+    if (!ra.part && grid.x < 512) {  // latency-bound: loads issued ahead of the transforms
+      if (uni) hipLaunchKernelGGL((rows_inv_kernel<W, false, true, false, true>), grid, dim3(256), 0, st, ra);
+      else hipLaunchKernelGGL((rows_inv_kernel<W, false, false, false, true>), grid, dim3(256), 0, st, ra);
+      return launched("rows_inv");
+    }
     if (ra.part) {
       if (uni) hipLaunchKernelGGL((rows_inv_kernel<W, true, true>), grid, dim3(256), 0, st, ra);
       else hipLaunchKernelGGL((rows_inv_kernel<W, true, false>), grid, dim3(256), 0, st, ra);
