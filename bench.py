@@ -139,12 +139,14 @@ def adjoint_rate(n=256, steps=64, ckpt=16, reps=10):
     return out
 
 
-def c2_search(worlds=256, n=256, seed=0):
+def c2_search(worlds=256, n=256, seed=0, quadrant=False):
     """The batched 3-channel 6-kernel parameter search: `worlds` worlds of n x n, per-world kernels
     (kernel k reads channel k // 2, writes a random channel; r_k = R_k / 15 with R_k ~ U[10, 15];
     1-3 rings, b ~ U[0, 1]; gaussian growth mu ~ U[0.1, 0.4], sigma ~ U[0.01, 0.08]; weight
     h ~ U[0.1, 1]; weighted sum), per-channel U[0, 1) 64x64 centred patches.  Returns the rule,
-    half spectra [worlds, 6, ...], gf, w and the initial states, resident on the device."""
+    half spectra [worlds, 6, ...] (quadrant: the real quadrants [worlds, 6, n/2 + 1, n/2 + 1] of
+    these radially symmetric kernels, and the rule that reads them), gf, w and the initial states,
+    resident on the device."""
     import torch
     from leniax_amd import _lib, engine, kernels
     dev = torch.device('cuda', 0)
@@ -161,6 +163,8 @@ def c2_search(worlds=256, n=256, seed=0):
     del K, Ks
     gf = torch.stack(gfs).contiguous()
     w = torch.stack(ws).contiguous()
+    if quadrant:
+        Kh, rule = engine.quadrant_spectra(rule, Kh)
     return rule, Kh, gf, w, torch.as_tensor(cells).to(dev)
 
 
@@ -190,20 +194,23 @@ def c2_world_params(worlds=256, n=256, seed=0):
     return kps, cells
 
 
-def c2_flops_bytes(n=256):
+def c2_flops_bytes(n=256, quadrant=False):
     F = 2.5 * n * n * math.log2(n * n) * 9 + 6 * 6 * n * (n // 2 + 1) + 10 * 6 * n * n + 9 * n * n
-    # algorithmic HBM bytes per world-step: state read + write, the world's 6 half spectra
-    Bw = 8 * 3 * n * n + 8 * 6 * n * (n // 2 + 1)
+    # algorithmic HBM bytes per world-step: state read + write, the world's 6 half spectra (complex,
+    # as leniax holds them), or their real quadrants
+    Bw = 8 * 3 * n * n + (4 * 6 * (n // 2 + 1) ** 2 if quadrant else 8 * 6 * n * (n // 2 + 1))
     return F, Bw
 
 
-def c2_run(worlds=256, n=256, max_steps=512, stop='leniax', reps=1):
+def c2_run(worlds=256, n=256, max_steps=512, stop='leniax', reps=1, quadrant=True):
     """The 3C6K search as one lx_run (multi-pass path): every world up to max_steps steps with the
     per-step statistics, retired when the stop policy says so (uneven times), graph-replayed
-    steps over the live list.  Executed world-steps are counted."""
+    steps over the live list, the kernels' spectra as real quadrants (quadrant=False: complex half
+    spectra).  Executed world-steps are counted; the roofline fraction is against the bytes of
+    complex spectra (leniax's layout)."""
     import torch
     from leniax_amd import engine
-    rule, Kh, gf, w, a0 = c2_search(worlds, n)
+    rule, Kh, gf, w, a0 = c2_search(worlds, n, quadrant=quadrant)
     dt = torch.tensor([np.float32(1) / np.float32(C2_T)], device=a0.device)
     out = torch.empty_like(a0)
 
@@ -223,7 +230,8 @@ def c2_run(worlds=256, n=256, max_steps=512, stop='leniax', reps=1):
     cells = executed * 3 * n * n / sec
     bound = min(157.3e12 / (F / (3 * n * n)), 8.0e12 / (Bw / (3 * n * n)))
     return {'config': f'{worlds} worlds of {n}x{n}, 3 channels, 6 kernels, per-world spectra, <= {max_steps} steps, '
-                      f'stop policy {stop}, per-step statistics (lx_run, multi-pass path with retirement)',
+                      f'stop policy {stop}, per-step statistics (lx_run, multi-pass path with retirement), '
+                      f'{"real quadrant" if quadrant else "complex half"} spectra',
             'ms': sec * 1e3, 'executed_world_steps': executed, 'step_budget': worlds * max_steps,
             'steps_per_world': {'min': int(st.min()), 'median': int(np.median(st)), 'max': int(st.max())},
             'world_steps_per_s': executed / sec, 'cell_updates_per_s': cells, 'roofline_frac': cells / bound}
@@ -231,11 +239,15 @@ def c2_run(worlds=256, n=256, max_steps=512, stop='leniax', reps=1):
 
 def c2_rate(worlds=256, n=256, steps=8, seed=0):
     """One step of the 3C6K search over all its worlds: the step and its statistics (lx_step_stats,
-    no field or potential output, as run_scan_mem_optimized), timed beside lx_update + lx_stats."""
+    no field or potential output, as run_scan_mem_optimized) reading the kernels' real quadrant
+    spectra, timed beside the same step reading complex half spectra and beside lx_update +
+    lx_stats.  The roofline fraction is against the bytes of complex spectra (leniax's layout);
+    roofline_frac_quadrant_bytes against the quadrants' bytes."""
     import torch
     from leniax_amd import _lib, engine
     dev = torch.device('cuda', 0)
-    rule, Kh, gf, w, a = c2_search(worlds, n, seed)
+    rule_c, Kh_c, gf, w, a = c2_search(worlds, n, seed)
+    Kh, rule = engine.quadrant_spectra(rule_c, Kh_c)
     bbuf = torch.empty_like(a)
     field = torch.empty_like(a)
     pot = torch.empty((worlds, 6, n, n), device=dev)
@@ -245,10 +257,10 @@ def c2_rate(worlds=256, n=256, steps=8, seed=0):
     work = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
     carry = engine.carry_init(worlds)
     stats = torch.empty((worlds, 11 + 3), device=dev)
-    kbs = int(Kh[0].numel())
     st = engine.stream_ptr()
 
-    def step(x, y, one_pass):
+    def step(x, y, one_pass, rule=rule, Kh=Kh):
+        kbs = int(Kh[0].numel())
         if one_pass:  # what run_scan_mem_optimized runs per step: no field / potential output
             _lib.call('lx_step_stats', ctypes.byref(rule), engine.ptr(x), worlds, engine.ptr(Kh), kbs, engine.ptr(gf),
                       12, engine.ptr(w), 18, engine.ptr(dt), 0, engine.ptr(y), None, None, ctypes.c_float(15.),
@@ -260,29 +272,34 @@ def c2_rate(worlds=256, n=256, steps=8, seed=0):
         _lib.call('lx_stats', ctypes.byref(rule), engine.ptr(x), engine.ptr(field), engine.ptr(pot), worlds,
                   ctypes.c_float(15.), ctypes.c_float(10.), engine.ptr(carry), engine.ptr(stats), st)
 
-    def timed(one_pass):
+    def timed(one_pass, *spectra):
         nonlocal a, bbuf
-        step(a, bbuf, one_pass)
+        step(a, bbuf, one_pass, *spectra)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for i in range(steps):
-            step(a, bbuf, one_pass)
+            step(a, bbuf, one_pass, *spectra)
             a, bbuf = bbuf, a
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps
 
     sec = timed(True)
+    sec_c = timed(True, rule_c, Kh_c)
     sec2 = timed(False)
     F, Bw = c2_flops_bytes(n)
+    Bq = c2_flops_bytes(n, quadrant=True)[1]
     cells = worlds * 3 * n * n / sec
     bound = min(157.3e12 / (F / (3 * n * n)), 8.0e12 / (Bw / (3 * n * n)))  # cell-updates/s
-    return {'config': f'{worlds} worlds of {n}x{n}, 3 channels, 6 kernels, per-world spectra, step + statistics '
-                      '(lx_step_stats, as run_scan_mem_optimized)',
+    bound_q = min(157.3e12 / (F / (3 * n * n)), 8.0e12 / (Bq / (3 * n * n)))
+    return {'config': f'{worlds} worlds of {n}x{n}, 3 channels, 6 kernels, per-world spectra (real quadrants), '
+                      'step + statistics (lx_step_stats, as run_scan_mem_optimized)',
             'ms_per_step': sec * 1e3, 'world_steps_per_s': worlds / sec,
+            'complex_spectra_ms_per_step': sec_c * 1e3,
             'cell_updates_per_s': cells,
             'flops_per_world_step': F, 'achieved_tflops': worlds * F / sec / 1e12,
             'bytes_per_world_step': Bw, 'achieved_gbps': worlds * Bw / sec / 1e9,
             'roofline_frac': cells / bound,
+            'bytes_per_world_step_quadrant': Bq, 'roofline_frac_quadrant_bytes': cells / bound_q,
             'two_calls_ms_per_step': sec2 * 1e3,
             'two_calls_note': 'lx_update with field and potential outputs, then lx_stats'}
 
@@ -523,10 +540,12 @@ def main():
     c2 = c2_rate()
     log(f"3C6K 256 x 256^2 step + statistics in one pass: {c2['ms_per_step']:.3f} ms/step, "
         f"{c2['world_steps_per_s']:.0f} world-steps/s, {c2['achieved_gbps']:.0f} GB/s algorithmic, "
-        f"{c2['roofline_frac']:.3f} of its bound (update + stats as two calls: {c2['two_calls_ms_per_step']:.3f} ms)")
+        f"{c2['roofline_frac']:.3f} of its bound (complex half spectra: {c2['complex_spectra_ms_per_step']:.3f} ms; "
+        f"update + stats as two calls: {c2['two_calls_ms_per_step']:.3f} ms)")
     c2_stop = c2_run(stop='leniax')
     c2_full = c2_run(stop='none')
-    for name, r in (('leniax stop', c2_stop), ('every step', c2_full)):
+    c2_stop_c = c2_run(stop='leniax', quadrant=False)
+    for name, r in (('leniax stop', c2_stop), ('every step', c2_full), ('leniax stop, complex spectra', c2_stop_c)):
         log(f"3C6K search, 256 x 256^2, <= 512 steps, {name}: {r['ms']:.1f} ms, {r['executed_world_steps']} "
             f"world-steps (per world {r['steps_per_world']}), {r['world_steps_per_s']:.0f} world-steps/s, "
             f"{r['roofline_frac']:.3f} of its bound")
@@ -561,6 +580,7 @@ def main():
         'plain_3c6k_256_worlds_256x256': c2,
         'search_3c6k_256_worlds_leniax_stop': c2_stop,
         'search_3c6k_256_worlds_every_step': c2_full,
+        'search_3c6k_256_worlds_leniax_stop_complex_spectra': c2_stop_c,
         'direct_potential': dr,
         'plain_single_world_steps_per_s': plain_single,
         'plain_batched_world_steps_per_s': plain_batched,

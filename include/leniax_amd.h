@@ -74,6 +74,13 @@ typedef struct lx_rule {
   int32_t gf[LX_MAX_K];     /* enum lx_growth of kernel k */
   int32_t state_fn;         /* enum lx_state_fn */
   int32_t average;          /* 1: weighted_mean (leniax/core.py:228-242), 0: weighted_sum (:202-225) */
+  /* 0: K_half_d arguments hold Hermitian half spectra (lx_prepare_spectra layout, complex64).
+   * 1 (2-D rules only): they hold real quadrants (lx_quadrant_spectra layout, float32
+   * [S][K][dims[0]/2 + 1][dims[1]/2 + 1], k_bstride in float32 elements), the spectra of kernels
+   * that are even along each axis (leniax's radially symmetric kernels).  Accepted by lx_update,
+   * lx_update_with_spectra, lx_step_stats, lx_potential and lx_run (its multi-pass path); the
+   * adjoint entry points and lx_run_static return LX_ENOTSUP. */
+  int32_t k_quadrant;
 } lx_rule;
 
 /* Library / device information. */
@@ -89,6 +96,18 @@ int lx_version(void);
  * so that the real-to-complex path reproduces real(ifftn(fftn(A) * K)) for any K. */
 int lx_prepare_spectra(const lx_rule* rule, const void* K_full_d, int64_t S, int32_t maxk,
                        const int32_t* slot, void* K_half_d, void* stream);
+
+/* Real quadrants of half spectra for rules with k_quadrant = 1: K_quad[i][u][v] = Re K_half[i][u][v]
+ * for u <= dims[0]/2, i < n (no counterpart in leniax, which keeps full complex spectra: leniax's
+ * kernels are radially symmetric, so their spectra are real and even, and the quadrant holds a
+ * quarter of the half spectrum's bytes).  rule: 2-D, dims[0] even.  K_half_d complex64
+ * [n][dims[0]][dims[1]/2 + 1]; K_quad_d float32 [n][dims[0]/2 + 1][dims[1]/2 + 1]; dev_d float32[2]
+ * receives the largest departure from a real even spectrum (max over i, u, v of |Im K[u][v]| +
+ * |Im K[dims[0] - u][v]| + |Re K[u][v] - Re K[dims[0] - u][v]|, infinite for NaN) and the largest
+ * |Re K|: the caller decides from them whether the kernels are even enough for the quadrant layout.
+ * n == 0 writes nothing. */
+int lx_quadrant_spectra(const lx_rule* rule, const void* K_half_d, int64_t n, void* K_quad_d, float* dev_d,
+                        void* stream);
 
 /* leniax/kernels.py:145-149: K = fftn(fftshift(k)) of real kernels on the device.
  * k_d float32 [n][dims] holds kernels already zero-padded to the world and fftshift-ed

@@ -48,6 +48,7 @@ class lx_rule(ctypes.Structure):
         ('gf', ctypes.c_int32 * LX_MAX_K),
         ('state_fn', ctypes.c_int32),
         ('average', ctypes.c_int32),
+        ('k_quadrant', ctypes.c_int32),
     ]
 
 
@@ -146,6 +147,7 @@ SIGNATURES = {
     'lx_last_error': (ctypes.c_char_p, []),
     'lx_version': (ctypes.c_int, []),
     'lx_prepare_spectra': (ctypes.c_int, [_R, _P, _I64, _I32, ctypes.POINTER(_I32), _P, _P]),
+    'lx_quadrant_spectra': (ctypes.c_int, [_R, _P, _I64, _P, _P, _P]),
     'lx_kernel_spectrum': (ctypes.c_int, [_I32, ctypes.POINTER(_I32), _I64, _P, _P, _P, _P]),
     'lx_workspace_size': (ctypes.c_int, [_R, _I64, ctypes.POINTER(ctypes.c_size_t)]),
     'lx_potential': (ctypes.c_int, [_R, _P, _I64, _P, _I64, _P, _P, _P]),

@@ -62,7 +62,7 @@ int grad_layout(const lx_rule* r, int64_t B, int32_t steps, int32_t c, int32_t w
 
 int check_args(const lx_rule* rule, const lx_grad_run_args* a, bool forward) {
   int rc = lx_check_rule(rule);
-  if (rc) return rc;
+  if (rc || (rc = lx_no_quadrant(rule, "lx_grad_run"))) return rc;
   if (!a) return lx_fail(LX_EINVAL, "args is NULL");
   if (a->B < 0 || a->steps < 1 || a->ckpt_every < 1) return lx_fail(LX_EINVAL, "B >= 0, steps and ckpt_every >= 1");
   if (a->B > 0 && (!a->cells0_d || !a->K_half_d || !a->gf_d || !a->w_d || !a->dt_d || (forward && !a->cells_d) ||

@@ -4,6 +4,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include "lx_internal.h"
 
 static thread_local char g_err[512] = "";
@@ -35,6 +37,10 @@ int lx_check_rule(const lx_rule* rule) {
   }
   if (rule->state_fn < 0 || rule->state_fn > LX_STATE_SIMPLE)
     return lx_fail(LX_EINVAL, "state_fn %d unknown", rule->state_fn);
+  if (rule->k_quadrant != 0 && rule->k_quadrant != 1)
+    return lx_fail(LX_EINVAL, "k_quadrant must be 0 or 1 (got %d)", rule->k_quadrant);
+  if (rule->k_quadrant && (rule->ndim != 2 || rule->dims[0] % 2))
+    return lx_fail(LX_EINVAL, "quadrant spectra need a 2-D rule with an even first dimension");
   return LX_OK;
 }
 
@@ -87,6 +93,64 @@ extern "C" int lx_prepare_spectra(const lx_rule* rule, const void* K_full_d, int
                      rule->C * maxk, d0, d1, d2, sl);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return lx_fail(LX_EDEVICE, "prepare_spectra launch: %s", hipGetErrorString(e));
+  return LX_OK;
+}
+
+// Real quadrants of half spectra (K_quad[i][u][v] = Re K[i][u][v], u <= H / 2) and how far the
+// spectra are from real and even along the first axis: dev[0] = max over (i, u, v) of
+// |Im K[u][v]| + |Im K[H - u][v]| + |Re K[u][v] - Re K[H - u][v]| (NaN counts as infinite),
+// dev[1] = max |Re K|.  Non-negative floats order as their bit patterns: one unsigned max per wave.
+__global__ void quadrant_clear_kernel(unsigned* dev) {
+  if (threadIdx.x < 2) dev[threadIdx.x] = 0u;
+}
+
+__global__ void quadrant_kernel(const float2* __restrict__ kh, float* __restrict__ kq, int64_t n, int H, int WH,
+                                unsigned* dev) {
+  const int HQ = H / 2 + 1;
+  const int64_t total = n * HQ * int64_t(WH);
+  float d = 0.f, mx = 0.f;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x) {
+    const int v = int(i % WH);
+    const int u = int((i / WH) % HQ);
+    const int64_t img = i / (int64_t(HQ) * WH);
+    const float2* kb = kh + img * H * WH;
+    const float2 a = kb[int64_t(u) * WH + v];
+    const float2 b = kb[int64_t((H - u) % H) * WH + v];
+    kq[i] = a.x;
+    const float e = fabsf(a.y) + fabsf(b.y) + fabsf(a.x - b.x);
+    d = (e == e) ? fmaxf(d, e) : __builtin_inff();
+    const float m = fabsf(a.x);
+    mx = (m == m) ? fmaxf(mx, m) : __builtin_inff();
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    d = fmaxf(d, __shfl_xor(d, o));
+    mx = fmaxf(mx, __shfl_xor(mx, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(dev, __float_as_uint(d));
+    atomicMax(dev + 1, __float_as_uint(mx));
+  }
+}
+
+extern "C" int lx_quadrant_spectra(const lx_rule* rule, const void* K_half_d, int64_t n, void* K_quad_d, float* dev_d,
+                                   void* stream) {
+  int rc = lx_check_rule(rule);
+  if (rc) return rc;
+  if (rule->ndim != 2 || rule->dims[0] % 2)
+    return lx_fail(LX_EINVAL, "lx_quadrant_spectra: a 2-D rule with an even first dimension");
+  if (n < 0) return lx_fail(LX_EINVAL, "n < 0");
+  if (n == 0) return LX_OK;
+  if (!dev_d || !K_half_d || !K_quad_d) return lx_fail(LX_EINVAL, "bad arguments: NULL pointer");
+  hipStream_t st = (hipStream_t)stream;
+  unsigned* dev = reinterpret_cast<unsigned*>(dev_d);
+  hipLaunchKernelGGL(quadrant_clear_kernel, dim3(1), dim3(64), 0, st, dev);
+  const int H = rule->dims[0], WH = rule->dims[1] / 2 + 1;
+  const int64_t total = n * (H / 2 + 1) * int64_t(WH);
+  const unsigned blocks = unsigned(std::min<int64_t>((total + 255) / 256, 8192));
+  hipLaunchKernelGGL(quadrant_kernel, dim3(blocks), dim3(256), 0, st, static_cast<const float2*>(K_half_d),
+                     static_cast<float*>(K_quad_d), n, H, WH, dev);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return lx_fail(LX_EDEVICE, "quadrant_spectra launch: %s", hipGetErrorString(e));
   return LX_OK;
 }
 

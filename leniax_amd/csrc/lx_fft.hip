@@ -299,6 +299,11 @@ struct LineArgs {
   int cin[LX_MAX_K];   // mode 0: kernel k reads channel cin[k]
   int mode;            // 0: conv, 1: forward in place, 2: inverse S -> U
   int conjk;           // mode 0: multiply by conj(K) (adjoint of the potential)
+  // mode 0: Kh holds real quadrants (float [world][K][L/2 + 1][WH], 2-D): the spectrum of kernel
+  // k at frequency f of a line is the real value at row min(f, L - f); k_elems: the elements (float2,
+  // or float for quadrants) of one kernel's spectrum
+  int kquad;
+  int64_t k_elems;
   // mode 0, optional: the forward spectra of the input images, FA [img][half]
   float2* FA;
   // mode 0, optional: dspec[b][k] += X_k * conj(FAin[b][acc_cin[k]]) (kernel-spectrum gradient)
@@ -318,7 +323,7 @@ LineArgs line_args() {
   a.S = nullptr; a.U = nullptr; a.Kh = nullptr; a.k_bstride = 0; a.nlines = 0;
   a.WH = 0; a.A = 1; a.a_stride = 0; a.st = 0; a.img_elems = 0; a.C = 1; a.K = 0;
   for (int k = 0; k < LX_MAX_K; ++k) { a.cin[k] = -1; a.acc_cin[k] = 0; }
-  a.mode = 0; a.conjk = 0; a.FA = nullptr; a.FAin = nullptr; a.dspec = nullptr; a.C_fa = 1;
+  a.mode = 0; a.conjk = 0; a.kquad = 0; a.k_elems = 0; a.FA = nullptr; a.FAin = nullptr; a.dspec = nullptr; a.C_fa = 1;
   a.list = nullptr; a.nlive = nullptr; a.xcd = 1;
   return a;
 }
@@ -326,7 +331,9 @@ LineArgs line_args() {
 // One launch per mode (MODE 0: conv, 1: forward in place, 2: inverse), so that each keeps
 // only its own registers.  Offsets inside an image are 32-bit; loads are issued in batches
 // of 8 (sched_barrier) so that addresses and values are not all held at once.
-template <int L, int MODE, bool LIST = false, int T = line_tpr(L, MODE)>
+// KQ (mode 0): the kernels' spectra are real quadrants (a separate instantiation, so that the
+// complex layout's register allocation is its own)
+template <int L, int MODE, bool LIST = false, int T = line_tpr(L, MODE), bool KQ = false>
 __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
   using LC = Lanes<L, T>;
   constexpr int TPR = LC::TPR, P = LC::P;
@@ -403,12 +410,23 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
         if ((e & 7) == 7) __builtin_amdgcn_sched_barrier(0);
       }
     }
-    const float2* kk = a.Kh + bw * a.k_bstride + int64_t(k) * a.img_elems + off;
+    const int64_t kb = bw * a.k_bstride + int64_t(k) * a.k_elems + off;
+    if constexpr (KQ) {
+      const float* kq = reinterpret_cast<const float*>(a.Kh) + kb;
 #pragma unroll
-    for (int e = 0; e < P; ++e) {
-      const float2 kv = kk[freq_of<L, TPR>(t, e) * st];
-      y[e] = cmul(x[e], make_float2(kv.x, csign * kv.y));
-      if ((e & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      for (int e = 0; e < P; ++e) {
+        const int f = freq_of<L, TPR>(t, e);
+        y[e] = cmul(x[e], make_float2(kq[(f <= L / 2 ? f : L - f) * st], 0.f));
+        if ((e & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      }
+    } else {
+      const float2* kk = a.Kh + kb;
+#pragma unroll
+      for (int e = 0; e < P; ++e) {
+        const float2 kv = kk[freq_of<L, TPR>(t, e) * st];
+        y[e] = cmul(x[e], make_float2(kv.x, csign * kv.y));
+        if ((e & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      }
     }
     fft_group<L, true, TPR>(y, t, lane, xch);
     float2* dst = a.U + (b * a.K + k) * a.img_elems + off;
@@ -422,7 +440,7 @@ __global__ __launch_bounds__(256) void lines_kernel(LineArgs a) {
 // The convolution pass of one channel and one kernel on a latency-bound grid (a few worlds): the
 // line's kernel spectrum is loaded before the forward transform, so that its latency overlaps the
 // transform instead of following it.  The same arithmetic as lines_kernel's mode 0.
-template <int L, int T>
+template <int L, int T, bool KQ = false>
 __global__ __launch_bounds__(256) void lines_conv1_kernel(LineArgs a) {
   using LC = Lanes<L, T>;
   constexpr int TPR = LC::TPR, P = LC::P;
@@ -440,12 +458,21 @@ __global__ __launch_bounds__(256) void lines_conv1_kernel(LineArgs a) {
   const int off = aa * int(a.a_stride) + (rem - aa * a.WH);
   const int st = int(a.st);
   const float2* src = a.S + img * a.img_elems + off;
-  const float2* kk = a.Kh + img * a.k_bstride + off;
   float2 x[P], kv[P];
 #pragma unroll
   for (int m = 0; m < P; ++m) x[m] = src[(TPR * m + t) * st];
+  if constexpr (KQ) {
+    const float* kq = reinterpret_cast<const float*>(a.Kh) + img * a.k_bstride + off;
 #pragma unroll
-  for (int e = 0; e < P; ++e) kv[e] = kk[freq_of<L, T>(t, e) * st];
+    for (int e = 0; e < P; ++e) {
+      const int f = freq_of<L, T>(t, e);
+      kv[e] = make_float2(kq[(f <= L / 2 ? f : L - f) * st], 0.f);
+    }
+  } else {
+    const float2* kk = a.Kh + img * a.k_bstride + off;
+#pragma unroll
+    for (int e = 0; e < P; ++e) kv[e] = kk[freq_of<L, T>(t, e) * st];
+  }
   fft_group<L, false, T>(x, t, lane, xch);
   if (a.FA && live) {
     float2* fa = a.FA + img * a.img_elems + off;
@@ -532,7 +559,7 @@ constexpr bool conv_staged(int L) { return L == 256; }
 // steps 1.63 -> 2.05 ms staged)
 constexpr unsigned kStagedMinBlocks = 1024;
 
-template <int L, bool LIST = false>
+template <int L, bool LIST = false, bool KQ = false>
 __global__ __launch_bounds__(256) void conv_staged_kernel(LineArgs a) {
   using LC = Lanes<L, line_tpr(L, 0)>;
   constexpr int TPR = LC::TPR, P = LC::P, LB = 256 / TPR, GRS = TPR * LC::XRS;
@@ -592,8 +619,28 @@ __global__ __launch_bounds__(256) void conv_staged_kernel(LineArgs a) {
     if (!((mask >> ck) & 1u)) continue;  // no line of the block reads kernel k (block-uniform)
     const bool use_s = live_s && c_s == ck;
     __syncthreads();  // the previous transform or store step is done with the scratch
-    {
-      const float2* kk = a.Kh + bw_s * a.k_bstride + int64_t(k) * a.img_elems + off_s;
+    const int64_t kb = bw_s * a.k_bstride + int64_t(k) * a.k_elems + off_s;
+    if constexpr (KQ) {
+      // points 0..L/2 of the quadrant's column, each staged at f and L - f: points pl + TPR q
+      // (q < P/2) are below L/2 for every thread, so their loads issue together; L/2 is thread
+      // pl = 0's
+      static_assert(P % 2 == 0 && TPR * (P / 2) == L / 2, "half the points below L/2");
+      const float* kq = reinterpret_cast<const float*>(a.Kh) + kb;
+      float v[P / 2 + 1];
+#pragma unroll
+      for (int q = 0; q < P / 2; ++q) v[q] = use_s ? kq[(pl + TPR * q) * st] : 0.f;
+      v[P / 2] = (use_s && pl == 0) ? kq[(L / 2) * st] : 0.f;
+      if (use_s) {
+#pragma unroll
+        for (int q = 0; q < P / 2; ++q) {
+          const int pt = pl + TPR * q;
+          stage[srow(ls) + pt] = make_float2(v[q], 0.f);
+          if (pt > 0) stage[srow(ls) + L - pt] = make_float2(v[q], 0.f);
+        }
+        if (pl == 0) stage[srow(ls) + L / 2] = make_float2(v[P / 2], 0.f);
+      }
+    } else {
+      const float2* kk = a.Kh + kb;
 #pragma unroll
       for (int q = 0; q < P; ++q) {
         const int pt = pl + TPR * q;
@@ -1022,23 +1069,30 @@ int launch_lines_v(LineArgs la, int L, int64_t lines, LineVariant v, hipStream_t
       constexpr int T = decltype(tc)::value;
       const dim3 grid(blocks_for(lines * T));
       if (la.mode == 0) {
-        if constexpr (conv_staged(LL) && T == line_tpr(LL, 0)) {
-          if (v.staged) {
-            hipLaunchKernelGGL((conv_staged_kernel<LL, LIST>), grid, dim3(256), 0, st, la);
-            return launched("conv (staged)");
+        auto conv = [&](auto kq) {
+          constexpr bool KQ = decltype(kq)::value;
+          if constexpr (conv_staged(LL) && T == line_tpr(LL, 0)) {
+            if (v.staged) {
+              hipLaunchKernelGGL((conv_staged_kernel<LL, LIST, KQ>), grid, dim3(256), 0, st, la);
+              return launched("conv (staged)");
+            }
           }
-        }
-        // latency-bound grids (less than two blocks per CU) only: its registers cost occupancy
-        if (!LIST && la.dspec && !la.FA && la.C == 1 && la.K == 1 && la.C_fa == 1 && la.acc_cin[0] == 0 &&
-            la.cin[0] == 0 && grid.x < 512) {
-          hipLaunchKernelGGL((lines_gradk_kernel<LL, T>), grid, dim3(256), 0, st, la);
-          return launched("lines (kernel gradient)");
-        }
-        if (!LIST && !la.dspec && la.C == 1 && la.K == 1 && la.cin[0] == 0 && grid.x < 512) {
-          hipLaunchKernelGGL((lines_conv1_kernel<LL, T>), grid, dim3(256), 0, st, la);
-          return launched("lines (convolution, one kernel)");
-        }
-        hipLaunchKernelGGL((lines_kernel<LL, 0, LIST, T>), grid, dim3(256), 0, st, la);
+          // latency-bound grids (less than two blocks per CU) only: its registers cost occupancy
+          if constexpr (!KQ) {
+            if (!LIST && la.dspec && !la.FA && la.C == 1 && la.K == 1 && la.C_fa == 1 && la.acc_cin[0] == 0 &&
+                la.cin[0] == 0 && grid.x < 512) {
+              hipLaunchKernelGGL((lines_gradk_kernel<LL, T>), grid, dim3(256), 0, st, la);
+              return launched("lines (kernel gradient)");
+            }
+          }
+          if (!LIST && !la.dspec && la.C == 1 && la.K == 1 && la.cin[0] == 0 && grid.x < 512) {
+            hipLaunchKernelGGL((lines_conv1_kernel<LL, T, KQ>), grid, dim3(256), 0, st, la);
+            return launched("lines (convolution, one kernel)");
+          }
+          hipLaunchKernelGGL((lines_kernel<LL, 0, LIST, T, KQ>), grid, dim3(256), 0, st, la);
+          return launched("lines");
+        };
+        return la.kquad ? conv(std::true_type{}) : conv(std::false_type{});
       } else if (la.mode == 1) {
         hipLaunchKernelGGL((lines_kernel<LL, 1, LIST, T>), grid, dim3(256), 0, st, la);
       } else {
@@ -1064,8 +1118,8 @@ LineVariant default_variant(const LineArgs& la, int L, int64_t lines) {
 }
 
 void lines_key(char* key, size_t n, const LineArgs& la, int L, int64_t lines, bool list) {
-  snprintf(key, n, "L=%d:mode=%d:lines=%lld:K=%d:%s", L, la.mode, (long long)lines, la.mode == 0 ? la.K : 0,
-           list ? "list" : "plain");
+  snprintf(key, n, "L=%d:mode=%d:lines=%lld:K=%d:%s%s", L, la.mode, (long long)lines, la.mode == 0 ? la.K : 0,
+           list ? "list" : "plain", la.mode == 0 && la.kquad ? ":quadrant" : "");
 }
 
 // The variant of a line pass: forced knobs, then a recorded decision, then (tuning on) the fastest
@@ -1119,6 +1173,7 @@ LineVariant choose_lines(const LineArgs& la, int L, int64_t lines, bool list, bo
 // la: mode-specific fields filled by the caller; geometry of `axis` and the line count set here
 int launch_lines(const Geom& g, int axis, LineArgs la, int64_t nimg, hipStream_t st) {
   la.WH = g.WH; la.A = g.A[axis]; la.a_stride = g.a_stride[axis]; la.st = g.st[axis]; la.img_elems = g.half;
+  la.k_elems = la.kquad ? int64_t(g.L[0] / 2 + 1) * g.WH : g.half;
   la.nlines = nimg * g.A[axis] * int64_t(g.WH);
   const int L = g.L[axis];
   if (int rc = dispatch_len(L, [&](auto l) { return ensure_tw<decltype(l)::value>(st); })) return rc;
@@ -1231,7 +1286,7 @@ int plain_step(const lx_rule* rule, const float* A, int64_t B, const void* Kh, i
   if (!rc) {  // the convolution axis
     LineArgs la = line_args();
     la.S = S; la.U = U; la.Kh = static_cast<const float2*>(Kh); la.k_bstride = k_bs; la.C = C; la.K = K;
-    la.FA = FA;
+    la.kquad = rule->k_quadrant; la.FA = FA;
     for (int k = 0; k < LX_MAX_K; ++k) la.cin[k] = k < K ? rule->cin[k] : -1;
     rc = launch_lines(g, 0, la, B * C, st);
   }
@@ -1431,7 +1486,7 @@ extern "C" int lx_adjoint_potential(const lx_rule* rule, int64_t B, const void* 
                                     const float* dpot_d, const void* state_spec_d, void* dspec_d, float* lambda_d,
                                     void* work_d, void* stream) {
   int rc = lx_check_rule(rule);
-  if (rc) return rc;
+  if (rc || (rc = lx_no_quadrant(rule, "lx_adjoint_potential"))) return rc;
   if ((state_spec_d == nullptr) != (dspec_d == nullptr))
     return lx_fail(LX_EINVAL, "state_spec and dspec are given together or not at all");
   Geom g;
@@ -1667,6 +1722,7 @@ int run_lines(const PassesRun& r, const RunView& rv, int bucket, int axis, LineA
               hipStream_t st) {
   const Geom& g = r.g;
   la.WH = g.WH; la.A = g.A[axis]; la.a_stride = g.a_stride[axis]; la.st = g.st[axis]; la.img_elems = g.half;
+  la.k_elems = la.kquad ? int64_t(g.L[0] / 2 + 1) * g.WH : g.half;
   la.nlines = int64_t(slot_imgs) * g.A[axis] * g.WH;  // per slot: the kernels scale it by the live count
   la.list = rv.list; la.nlive = rv.nlive;
   const int L = g.L[axis];
@@ -1690,7 +1746,7 @@ int tune_run_lines(const PassesRun& r, int bucket, hipStream_t st) {
   if (!rc) {
     LineArgs la = line_args();
     la.S = r.S; la.U = r.U; la.Kh = static_cast<const float2*>(r.a->K_half_d); la.k_bstride = r.a->k_bstride;
-    la.C = C; la.K = K;
+    la.C = C; la.K = K; la.kquad = rule->k_quadrant;
     for (int k = 0; k < LX_MAX_K; ++k) la.cin[k] = k < K ? rule->cin[k] : -1;
     rc = run_lines(r, r.rv[0], bucket, 0, la, C, true, st);
   }
@@ -1727,7 +1783,7 @@ int enqueue_step(const PassesRun& r, int par, int bucket, hipStream_t st) {
   if (!rc) {
     LineArgs la = line_args();
     la.S = r.S; la.U = r.U; la.Kh = static_cast<const float2*>(a->K_half_d); la.k_bstride = a->k_bstride;
-    la.C = C; la.K = K;
+    la.C = C; la.K = K; la.kquad = rule->k_quadrant;
     for (int k = 0; k < LX_MAX_K; ++k) la.cin[k] = k < K ? rule->cin[k] : -1;
     rc = lines(0, la, C);
   }
@@ -2233,7 +2289,7 @@ extern "C" int lx_adjoint_step(const lx_rule* rule, int64_t B, const float* stat
                                float* lambda_d, float* dpot_d, float* grad_gf_d, float* grad_w_d, float* grad_dt_d,
                                void* work_d, void* adj_work_d, void* stream) {
   int rc = lx_check_rule(rule);
-  if (rc) return rc;
+  if (rc || (rc = lx_no_quadrant(rule, "lx_adjoint_step"))) return rc;
   if ((state_spec_d == nullptr) != (dspec_d == nullptr))
     return lx_fail(LX_EINVAL, "state_spec and dspec are given together or not at all");
   Geom g;

@@ -6,7 +6,7 @@ int lx_fused2d_launch_64(const lx_rule* rule, const lx_run_args* a, hipStream_t 
 int lx_fused2d_launch_128(const lx_rule* rule, const lx_run_args* a, hipStream_t stream);
 
 int lx_fused2d_supported(const lx_rule* rule) {
-  if (rule->ndim != 2 || rule->C != 1 || rule->K != 1) return 0;
+  if (rule->ndim != 2 || rule->C != 1 || rule->K != 1 || rule->k_quadrant) return 0;
   if (rule->dims[0] != rule->dims[1]) return 0;
   const int n = rule->dims[0];
   return n == 32 || n == 64 || n == 128;

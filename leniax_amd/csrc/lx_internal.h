@@ -427,5 +427,11 @@ int lx_passes_scratch(const lx_rule* rule, int64_t B, int32_t slots, size_t* byt
 int lx_passes_run(const lx_rule* rule, const lx_run_args* a, hipStream_t stream);
 
 // Fused persistent 2-D run (lx_fused2d.hip)
+// LX_ENOTSUP (with the message) for rules whose spectra are real quadrants: the adjoint passes
+// read complex half spectra only
+inline int lx_no_quadrant(const lx_rule* rule, const char* fn) {
+  return rule->k_quadrant ? lx_fail(LX_ENOTSUP, "%s: quadrant spectra (k_quadrant = 1) are not supported here", fn)
+                          : LX_OK;
+}
 int lx_fused2d_supported(const lx_rule* rule);
 int lx_fused2d_launch(const lx_rule* rule, const lx_run_args* a, hipStream_t stream);

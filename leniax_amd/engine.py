@@ -96,6 +96,31 @@ def prepare_spectra(rule, K_full, slots):
     return out
 
 
+def quadrant_spectra(rule, K_half, rtol=1e-6):
+    """Real quadrants [S, K, H/2 + 1, W/2 + 1] float32 of 2-D half spectra [S, K, H, W/2 + 1] whose
+    kernels are even along both axes (leniax's radially symmetric kernels: their spectra are real
+    and even), and a copy of `rule` with k_quadrant = 1 that the forward entry points read them
+    with.  Raises ValueError when a spectrum departs from a real even one by more than rtol of the
+    largest magnitude (a kernel that is not symmetric)."""
+    if rule.ndim != 2:
+        raise ValueError('quadrant spectra are for 2-D rules')
+    K_half = as_device(K_half, torch.complex64)
+    H, WH = int(rule.dims[0]), int(rule.dims[1]) // 2 + 1
+    if tuple(K_half.shape[-2:]) != (H, WH):
+        raise ValueError(f'half spectra of shape {tuple(K_half.shape)} do not match the rule ({H}, {WH})')
+    n = K_half.numel() // (H * WH)
+    out = torch.empty(tuple(K_half.shape[:-2]) + (H // 2 + 1, WH), dtype=torch.float32, device=K_half.device)
+    dev = torch.empty(2, dtype=torch.float32, device=K_half.device)
+    _lib.call('lx_quadrant_spectra', ctypes.byref(rule), ptr(K_half), n, ptr(out), ptr(dev), stream_ptr())
+    d, m = dev.tolist()
+    if not d <= rtol * m:
+        raise ValueError(f'spectra are not real and even within {rtol:g} of their largest magnitude ({d:g} vs {m:g})')
+    q = _lib.lx_rule()
+    ctypes.pointer(q)[0] = rule
+    q.k_quadrant = 1
+    return out, q
+
+
 def carry_init(B):
     nbytes = _lib.lib().lx_carry_bytes()
     buf = torch.empty((max(B, 1), nbytes // 4), dtype=torch.int32, device=device())

@@ -78,8 +78,10 @@ def _check_stats(dev_rows, ostats, C):
     return worst
 
 
-def test_3c6k_search_matches_oracle():
-    """Three worlds of the 256^2 three-channel six-kernel search, 32 steps through the batched run."""
+@pytest.mark.parametrize('quadrant', [False, True])
+def test_3c6k_search_matches_oracle(quadrant):
+    """Three worlds of the 256^2 three-channel six-kernel search, 32 steps through the batched run,
+    with complex half spectra and with real quadrants (k_quadrant = 1)."""
     import bench
     from leniax_amd import engine, kernels, _lib
     n, B, steps = 256, 3, 32
@@ -91,6 +93,8 @@ def test_3c6k_search_matches_oracle():
     gf = torch.stack([mp.get_gf_params() for _, mp in Kd]).contiguous()
     w = torch.stack([mp.get_kernels_weight_per_channel() for _, mp in Kd]).contiguous()
     dt = torch.tensor([np.float32(1) / np.float32(bench.C2_T)]).cuda()
+    if quadrant:
+        Kh, rule = engine.quadrant_spectra(rule, Kh)
     res = engine.run_batched(rule, torch.as_tensor(cells).cuda(), Kh, True, gf, w, dt, steps, 'none', 0, bench.C2_R,
                              bench.C2_T, want_stats=True, want_hist=True)
     torch.cuda.synchronize()

@@ -73,6 +73,43 @@ def test_argument_errors_are_reported_without_a_device():
     assert _lib.lib().lx_run_supported(ctypes.byref(r)) == 0
 
 
+def test_quadrant_spectra_rules_are_checked_without_a_device():
+    """k_quadrant: 0 or 1, 2-D rules with an even first dimension; the adjoint entry points refuse
+    quadrant spectra and lx_run takes its multi-pass path for them (the fused kernel reads complex
+    half spectra)."""
+    from leniax_amd import _lib, engine
+    if not _lib.available():
+        pytest.skip('libleniax_amd.so not built')
+    size = ctypes.byref(ctypes.c_size_t())
+    r = engine.make_rule([128, 128], 1, [0], [1])
+    assert engine.run_path(r) == 'fused'
+    r.k_quadrant = 2
+    with pytest.raises(_lib.LxError, match='k_quadrant'):
+        _lib.call('lx_workspace_size', ctypes.byref(r), 1, size)
+    r.k_quadrant = 1
+    _lib.call('lx_workspace_size', ctypes.byref(r), 1, size)
+    assert engine.run_path(r) == 'passes'
+    with pytest.raises(_lib.LxError, match='quadrant'):
+        _lib.call('lx_adjoint_potential', ctypes.byref(r), 1, 1, 0, 1, None, None, 1, 1, None)
+    a = _lib.lx_grad_run_args()
+    a.B, a.steps, a.ckpt_every = 1, 4, 2
+    with pytest.raises(_lib.LxError, match='quadrant'):
+        _lib.call('lx_grad_run_forward', ctypes.byref(r), ctypes.byref(a), None)
+    ra = _lib.lx_run_args()
+    ra.B, ra.max_steps = 1, 4
+    assert _lib.lib().lx_run_static(ctypes.byref(r), ctypes.byref(ra), None) == _lib.LX_ENOTSUP
+    r3 = engine.make_rule([32, 32, 32], 1, [0], [1])
+    r3.k_quadrant = 1
+    with pytest.raises(_lib.LxError, match='quadrant'):
+        _lib.call('lx_workspace_size', ctypes.byref(r3), 1, size)
+    r3.k_quadrant = 0
+    with pytest.raises(_lib.LxError, match='2-D'):
+        _lib.call('lx_quadrant_spectra', ctypes.byref(r3), 1, 0, 1, 1, None)
+    r5 = engine.make_rule([48, 64], 1, [0], [1])
+    with pytest.raises(_lib.LxError, match='bad arguments'):
+        _lib.call('lx_quadrant_spectra', ctypes.byref(r5), None, 2, None, None, None)
+
+
 def test_run_scratch_sizes():
     """lx_run_scratch_size: the fused path's queues and parked carries; the multi-pass path's second
     state buffer, carries and the per-slot spectra and records of the worlds in flight."""
@@ -136,6 +173,7 @@ def test_empty_batches_are_accepted_with_null_pointers():
         'lx_irfft2': lambda B: L_.lx_irfft2(2, dims, B, N, N, N, N),
         'lx_half_to_full': lambda B: L_.lx_half_to_full(2, dims, B, N, N, 1., N),
         'lx_kernel_spectrum': lambda B: L_.lx_kernel_spectrum(2, dims, B, N, N, N, N),
+        'lx_quadrant_spectra': lambda B: L_.lx_quadrant_spectra(r, N, B, N, N, N),
         'lx_adjoint_potential': lambda B: L_.lx_adjoint_potential(r, B, N, 0, N, N, N, N, N, N),
         'lx_adjoint_step': lambda B: L_.lx_adjoint_step(r, B, N, N, N, N, 0, N, 0, N, 0, N, 0, 1, N, N, N, N, N, N, N,
                                                         N, N, N),
