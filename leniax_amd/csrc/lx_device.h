@@ -675,3 +675,29 @@ LX_DEV T wave_sum_last(T v) {
   v += mv(v, 5);
   return v;
 }
+
+// wave_sum_last of N values at once, stage by stage: a stage of one value fills the wait states
+// that follow the previous stage of another.  Each value's additions are wave_sum_last's, in its
+// order, so the sums are the same bit for bit.
+template <int N, typename T>
+LX_DEV void wave_sum_last_n(T (&v)[N]) {
+  static_assert(sizeof(T) == 4, "32-bit values");
+  auto mv = [](T a, int ctrl_sel) -> T {
+    const int i = __builtin_bit_cast(int, a);
+    int r;
+    switch (ctrl_sel) {
+      case 0: r = __builtin_amdgcn_update_dpp(0, i, LX_QP(1, 0, 3, 2), 0xF, 0xF, false); break;
+      case 1: r = __builtin_amdgcn_update_dpp(0, i, LX_QP(2, 3, 0, 1), 0xF, 0xF, false); break;
+      case 2: r = __builtin_amdgcn_update_dpp(0, i, 0x141, 0xF, 0xF, false); break;
+      case 3: r = __builtin_amdgcn_update_dpp(0, i, 0x140, 0xF, 0xF, false); break;
+      case 4: r = __builtin_amdgcn_update_dpp(0, i, 0x142, 0xA, 0xF, false); break;
+      default: r = __builtin_amdgcn_update_dpp(0, i, 0x143, 0xC, 0xF, false); break;
+    }
+    return __builtin_bit_cast(T, r);
+  };
+#pragma unroll
+  for (int stage = 0; stage < 6; ++stage) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] += mv(v[j], stage);
+  }
+}

@@ -441,22 +441,18 @@ __global__ __launch_bounds__(F2<N>::NT) void fused2d_kernel(FusedParams p) {
       const float sAy0 = fmaf(sa0, yc0, sa1 * yc1);
       const float sGy0 = fmaf(sg0, yc0, sg1 * yc1);
       const float sAyy0 = fmaf(sa0 * yc0, yc0, sa1 * yc1 * yc1);
-      const float vals[8] = {sA, sG, sAy0, sAy1, sGy0, sGy1, sAyy0, sAyy1};
+      float vals[8] = {sA, sG, sAy0, sAy1, sGy0, sGy1, sAyy0, sAyy1};
       constexpr int slot[8] = {0, 1, 5, 6, 7, 8, 9, 10};
+      // per-wave counts are at most 2048: nA and nG share one 32-bit sum
+      uint32_t cnt[2] = {nA | (nG << 16), nU};
+      wave_sum_last_n(vals);
+      wave_sum_last_n(cnt);
+      if (lane == 63) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const float v = wave_sum_last(vals[i]);
-        if (lane == 63) red[wave][slot[i]] = v;
-      }
-      {
-        // per-wave counts are at most 2048: nA and nG share one 32-bit sum
-        const uint32_t ag = wave_sum_last(nA | (nG << 16));
-        const uint32_t u = wave_sum_last(nU);
-        if (lane == 63) {
-          red[wave][2] = float(ag & 0xFFFFu);
-          red[wave][3] = float(ag >> 16);
-          red[wave][4] = float(u);
-        }
+        for (int i = 0; i < 8; ++i) red[wave][slot[i]] = vals[i];
+        red[wave][2] = float(cnt[0] & 0xFFFFu);
+        red[wave][3] = float(cnt[0] >> 16);
+        red[wave][4] = float(cnt[1]);
       }
       ++t;  // no barrier: the next step's Z barrier orders the red writes before the closing
     }
