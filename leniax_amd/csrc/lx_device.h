@@ -648,6 +648,17 @@ LX_DEV float wave_sum(float v) {
   return v;
 }
 
+// wave_sum of N values at once, round by round: one value's exchange overlaps another's wait.
+// Each value's additions are wave_sum's, in its order (the same sums bit for bit).
+template <int N>
+LX_DEV void wave_sum_n(float (&v)[N]) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] += __shfl_xor(v[j], off, 64);
+  }
+}
+
 // Sum over the 64 lanes of a wave, complete in lane 63 only (the other lanes hold partial
 // sums): DPP butterflies within each row of 16 lanes (quad permutes, half-row and row
 // mirrors), then the row broadcasts of lanes 15 and 31.  No LDS traffic.

@@ -250,10 +250,10 @@ LX_DEV lx_sums lx_block_pair_sums(const float* __restrict__ rec, int C, int64_t 
 #pragma unroll
     for (int c2 = 0; c2 < LX_MAX_C; ++c2) v[14 + c2] += c2 == ch ? r[0] : 0.f;
   }
+  wave_sum_n(v);
+  if (lane == 0) {
 #pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    const float w = wave_sum(v[i]);
-    if (lane == 0) red[wave][i] = w;
+    for (int i = 0; i < NV; ++i) red[wave][i] = v[i];
   }
   __syncthreads();
   lx_sums s;
@@ -278,10 +278,13 @@ template <int NV>
 LX_DEV void lx_block_partial(float (&v)[NV], float* part, bool accumulate = false) {
   __shared__ float red[NV][4];
   const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  float w[NV];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    const float s = wave_sum(v[i]);
-    if (ln == 0) red[i][wv] = s;
+  for (int i = 0; i < NV; ++i) w[i] = v[i];
+  wave_sum_n(w);
+  if (ln == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) red[i][wv] = w[i];
   }
   __syncthreads();
   float* dst = part + (int64_t(blockIdx.y) * gridDim.x + blockIdx.x) * NV;

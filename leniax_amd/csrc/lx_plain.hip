@@ -250,10 +250,10 @@ __global__ __launch_bounds__(256) void stats_part_kernel(StatArgs a) {
     for (int i = i0 + tid; i < i1; i += 256) v[4] += U[i] > LX_EPS ? 1.f : 0.f;
   }
   __shared__ float red[4][LX_NSUM];
+  wave_sum_n(v);
+  if (lane == 0) {
 #pragma unroll
-  for (int i = 0; i < LX_NSUM; ++i) {
-    const float val = wave_sum(v[i]);
-    if (lane == 0) red[wave][i] = val;
+    for (int i = 0; i < LX_NSUM; ++i) red[wave][i] = v[i];
   }
   __syncthreads();
   if (tid < LX_NSUM) sc->part[part][tid] = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
